@@ -1,0 +1,268 @@
+/*
+ * mupsi.h — C ABI of the MI355X (gfx950) GPIS path tracer.
+ *
+ * This is the drop-in boundary for the CPU trace loop of mupsi.  The host
+ * (C++ scene loader, Python test harness, benchmark driver) describes a scene
+ * with the plain structs below, uploads it once, and then asks for spp passes.
+ * Every hot loop of the reference trace runs on the GPU behind these calls.
+ *
+ * Reference interfaces replaced (paths relative to the mupsi source tree):
+ *   mupsi_render_passes   <- Renderer::startRenderProgressive, the per-pass
+ *                            OpenMP pixel loop body   src/rendering/renderer.cpp:22-49
+ *   mupsi_trace_paths     <- PathTracer::trace(pixel, ctx, seed, k)
+ *                                                     src/rendering/trace.cpp:118-213
+ *   mupsi_read_accum      <- Framebuffer::accumulate / Renderer::getFramebuffer
+ *                                                     src/rendering/framebuffer.cpp:28-35,
+ *                                                     src/rendering/renderer.h:31
+ *   mupsi_scene_upload    <- Controller::load object graph + Renderer::prepareRender
+ *                            (Mesh::prepareForRender -> BVH::build)
+ *                                                     src/controller/controller.cpp:40-264,
+ *                                                     src/rendering/renderer.cpp:10-16
+ *   mupsi_psi_eval        <- SparseGPNoiseGenerator::RawNoise   src/gp/gpnoise.h:44,
+ *                                                     src/gp/gpnoise.cpp:51-87
+ *   mupsi_cell_impulses   <- the per-cell impulse loop of InternalNoise
+ *                                                     src/gp/gpnoise.cpp:57-74
+ *   mupsi_field_eval      <- GPMedium::evalWithGradient(p, seed, state)
+ *                                                     src/medium/gpmedium.cpp:19-41
+ *   mupsi_march           <- GPMedium::sampleDistance / GPMedium::transmittance
+ *                                                     src/medium/gpmedium.cpp:46-129
+ *   mupsi_condition       <- GPMedium::sampleCondition  src/medium/gpmedium.cpp:133-162
+ *   mupsi_mesh_intersect  <- Mesh::intersect -> BVH::intersect
+ *                                                     src/primitives/mesh.cpp:185-209,
+ *                                                     src/bvh/bvh.cpp:197-250
+ *   mupsi_mesh_occluded   <- Mesh::occluded -> BVH::occluded  src/primitives/mesh.cpp:236-250,
+ *                                                     src/bvh/bvh.cpp:254-300
+ *   mupsi_mesh_closest    <- Mesh::distToPoint -> BVH::closest src/primitives/mesh.cpp:325-339,
+ *                                                     src/bvh/bvh.cpp:302-353
+ *   mupsi_mesh_inside     <- Mesh::inside                src/primitives/mesh.cpp:341-362
+ *   mupsi_bvh_build       <- BVH::build / buildRecursive src/bvh/bvh.cpp:115-193
+ *   mupsi_scene_load_json <- Controller::load (scene.json schema)
+ *                                                     src/controller/controller.cpp:40-264
+ *
+ * Conventions
+ *   - Every function returns mupsi_status (0 = OK).  No C++ exception crosses
+ *     the ABI.  mupsi_last_error() returns a message for the last failure on
+ *     the calling thread.
+ *   - Scene descriptors are host memory owned by the caller; upload copies, so
+ *     the caller may free them on return.
+ *   - Batch entry points (psi_eval, march, mesh_* ...) take DEVICE pointers
+ *     (HBM-resident SoA arrays) and run synchronously on the context stream.
+ *   - One HIP stream per context; a context is not thread-safe.  Distinct
+ *     contexts may be used from distinct threads.
+ *   - The library is GPU-only: without a usable gfx950 device mupsi_open
+ *     fails with MUPSI_ERR_HIP.  There is no CPU fallback.
+ */
+#ifndef MUPSI_H_
+#define MUPSI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mupsi_status;
+enum {
+  MUPSI_OK = 0,
+  MUPSI_ERR_INVALID = 1,     /* bad argument / descriptor */
+  MUPSI_ERR_HIP = 2,         /* HIP runtime failure or no device */
+  MUPSI_ERR_NOMEM = 3,       /* device or host allocation failed */
+  MUPSI_ERR_ABORTED = 4,     /* abort flag observed; accumulation kept at last full pass */
+  MUPSI_ERR_UNSUPPORTED = 5, /* descriptor uses a feature this build does not lower */
+  MUPSI_ERR_IO = 6,          /* file could not be read */
+  MUPSI_ERR_PARSE = 7        /* malformed scene.json / OBJ / image */
+};
+
+typedef struct mupsi_ctx mupsi_ctx;
+typedef struct mupsi_scene mupsi_scene;
+
+/* ---- scene description (mirrors the scene.json schema, controller.cpp:40-264) ---- */
+
+enum { MUPSI_GP_SINGLE_REALIZATION = 0, MUPSI_GP_RENEWAL_PLUS = 1 };
+enum { MUPSI_MEAN_SPHERE = 0, MUPSI_MEAN_MESH = 1 };
+enum { MUPSI_PRIM_SPHERE = 0, MUPSI_PRIM_MESH = 1 };
+enum { MUPSI_BSDF_LAMBERTIAN = 0, MUPSI_BSDF_NULL = 1, MUPSI_BSDF_SPECULAR = 2 };
+
+/* SparseSEKernel(sigma, kernelRadius = kernel.length_scale, lengthScale = aniso)
+ * + SparseGPNoiseGenerator(kernel, points_per_cell)   controller.cpp:112-120 */
+typedef struct {
+  float sigma;           /* kernel.sigma */
+  float kernel_radius;   /* kernel.length_scale: the impulse cell size kr */
+  float aniso[3];        /* kernel.aniso: SE scale L^2 = |aniso|^2 */
+  int32_t points_per_cell;
+} mupsi_gp_params;
+
+/* Triangle mesh as parsed from OBJ (Mesh::fetchFrom, mesh.cpp:20-136) placed by
+ * Translation(position) * Scaling(scale) (controller.cpp:102-105,249-253). */
+typedef struct {
+  const float* vertices;    /* n_vertices x 3 */
+  const float* normals;     /* n_normals x 3, indexed by VERTEX index, or NULL */
+  const float* texcoords;   /* n_texcoords x 2, or NULL */
+  const int32_t* faces;     /* n_faces x 3 vertex indices (0-based) */
+  const int32_t* face_tex;  /* n_faces x 3 texcoord indices, or NULL */
+  int32_t n_vertices, n_normals, n_texcoords, n_faces;
+  float position[3];
+  float scale;
+} mupsi_mesh_desc;
+
+/* RGB float texture, row-major, row 0 at the top (BitmapTexture, texture.cpp:52-69). */
+typedef struct {
+  const float* rgb;  /* w x h x 3, or NULL for "absent" */
+  int32_t width, height;
+} mupsi_texture_desc;
+
+typedef struct {
+  int32_t type;            /* MUPSI_PRIM_* */
+  int32_t bsdf;            /* MUPSI_BSDF_* */
+  /* sphere */
+  float center[3];
+  float radius;
+  int32_t has_emission;    /* sphere "emission" key */
+  float emission[3];
+  int32_t int_medium;      /* sphere "int_medium" key present: GP medium inside */
+  /* mesh */
+  mupsi_mesh_desc mesh;
+  mupsi_texture_desc albedo; /* lambertian mesh "texture"; rgb NULL = constant 0.8 */
+} mupsi_primitive_desc;
+
+typedef struct {
+  int32_t width, height;
+  float cam_pos[3], cam_dir[3], cam_up[3];
+  float fov;                   /* degrees in the schema; the reference feeds it to tan() as radians */
+  int32_t max_bounce;          /* PathTracerSettings::max_bounce */
+  int32_t gp_mode;             /* MUPSI_GP_* */
+  int32_t skip_medium;         /* PathTracerSettings::skip_medium */
+  int32_t has_gp_medium;
+  int32_t mean_type;           /* MUPSI_MEAN_* */
+  float mean_center[3];
+  float mean_radius;
+  mupsi_mesh_desc mean_mesh;
+  mupsi_gp_params kernel;
+  mupsi_texture_desc sky;      /* skydrome texture; rgb NULL = no skydrome */
+  const mupsi_primitive_desc* primitives;
+  int32_t n_primitives;
+} mupsi_scene_desc;
+
+/* Conditioning state (GPConditioningState, medium.h:12-17) as SoA device arrays. */
+typedef struct {
+  const uint8_t* active;   /* n */
+  const float* cx; const float* cy; const float* cz;
+  const float* u_tilde;
+  const float* gx; const float* gy; const float* gz;
+} mupsi_cond_soa;
+
+/* ---- context ---- */
+mupsi_status mupsi_open(int32_t device, mupsi_ctx** out);
+void mupsi_close(mupsi_ctx* ctx);
+const char* mupsi_status_string(mupsi_status s);
+const char* mupsi_last_error(void);
+/* Device name and compute-unit count of the context's GPU. */
+mupsi_status mupsi_device_info(mupsi_ctx* ctx, char* name, int32_t name_len, int32_t* cu_count);
+
+/* ---- scene ---- */
+mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_desc* desc, mupsi_scene** out);
+/* Parse a scene.json file with the reference schema and upload it.  spp_out
+ * receives the "spp" key (1 if absent), as Controller::load stores spp_. */
+mupsi_status mupsi_scene_load_json(mupsi_ctx* ctx, const char* path, mupsi_scene** out, int32_t* spp_out);
+void mupsi_scene_free(mupsi_scene* scene);
+mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int32_t* height);
+
+/* Render spp passes k_begin .. k_begin+k_count-1 of every pixel and add the
+ * path radiance to the FP64 per-pixel sums.  abort_flag (host memory, may be
+ * NULL) is polled between bounce iterations and passes; on abort the sums are
+ * restored to the last completed call and MUPSI_ERR_ABORTED is returned.
+ * passes_per_launch bounds how many passes run as one wavefront (0 = auto). */
+mupsi_status mupsi_render_passes(mupsi_scene* scene, int32_t k_begin, int32_t k_count,
+                                 int32_t passes_per_launch, const volatile int32_t* abort_flag);
+mupsi_status mupsi_reset_accum(mupsi_scene* scene);
+/* rgb_sum: host, width*height*3 doubles (row-major, y = row); sample_count: passes accumulated. */
+mupsi_status mupsi_read_accum(mupsi_scene* scene, double* rgb_sum, int64_t* sample_count);
+/* Replace the sums (resume from a checkpoint written from mupsi_read_accum). */
+mupsi_status mupsi_write_accum(mupsi_scene* scene, const double* rgb_sum, int64_t sample_count);
+
+/* Per-path replay: radiance of PathTracer::trace((x[i], y[i]), ctx, 42, k[i]).
+ * Host arrays of length n; rgb_out host n x 3. */
+mupsi_status mupsi_trace_paths(mupsi_scene* scene, const int32_t* x, const int32_t* y,
+                               const int32_t* k, int64_t n, float* rgb_out);
+
+/* Work counters of the last mupsi_render_passes call (summed over its launches). */
+typedef struct {
+  int64_t paths;             /* paths traced */
+  int64_t march_calls;       /* sampleDistance invocations */
+  int64_t march_steps;       /* t-grid points visited (reference evaluations) */
+  int64_t field_evals;       /* field evaluations actually executed */
+  int64_t impulse_evals;     /* SE kernel evaluations (field_evals x 27 x ppc) */
+  int64_t march_kernel_launches;
+  double march_kernel_ms;    /* summed HIP-event time of the march kernel */
+  double total_ms;           /* wall time of the call */
+} mupsi_stats;
+mupsi_status mupsi_last_stats(mupsi_scene* scene, mupsi_stats* out);
+
+/* ---- batch entry points on device pointers ---- */
+
+/* RawNoise(p, seed): out4[i] = (psi, dpsi/dx, dpsi/dy, dpsi/dz).
+ * l1 (may be NULL): sigma/sqrt(Var) * sum_i |h_i| per point (error scale). */
+mupsi_status mupsi_psi_eval(mupsi_ctx* ctx, const mupsi_gp_params* gp,
+                            const float* px, const float* py, const float* pz,
+                            const uint32_t* seed, int64_t n, float* out4, float* l1);
+
+/* Impulses of cells cell3[i] = (cx, cy, cz) for realization seed[i]:
+ * cell_seed[i] = make_seed(cx, cy, cz, seed); pos[(i*ppc + j)*3 + a]; w[i*ppc + j] = +-1. */
+mupsi_status mupsi_cell_impulses(mupsi_ctx* ctx, const mupsi_gp_params* gp,
+                                 const int32_t* cell3, const uint32_t* seed, int64_t n,
+                                 uint32_t* cell_seed, float* pos, int8_t* w);
+
+/* GPMedium::evalWithGradient(p, seed, state) with the scene's GP medium. */
+mupsi_status mupsi_field_eval(mupsi_scene* scene, const float* px, const float* py,
+                              const float* pz, const uint32_t* seed,
+                              const mupsi_cond_soa* cond, int64_t n, float* out4);
+
+/* GPMedium::sampleDistance (mode 0) or ::transmittance (mode 1) of rays
+ * (ox..dz, near_t, far_t; directions are normalized as Ray's constructor does).
+ * Outputs: hit[i] (1 = sign change found; for mode 1 this means opaque),
+ * t[i], p[i*3], nrm[i*3] (mode 0 only), steps[i] = t-grid points visited. */
+mupsi_status mupsi_march(mupsi_scene* scene, int32_t mode,
+                         const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz,
+                         const float* near_t, const float* far_t, const uint32_t* seed,
+                         const mupsi_cond_soa* cond, int64_t n,
+                         uint8_t* hit, float* t, float* p, float* nrm, int32_t* steps);
+
+/* GPMedium::sampleCondition for medium hits at C with normal nrm, next-realization
+ * seed: the Renewal+ 4x4 system solved in FP64.  u_tilde[n], g_tilde[n*3] (double). */
+mupsi_status mupsi_condition(mupsi_scene* scene, const float* cx, const float* cy,
+                             const float* cz, const float* nx, const float* ny,
+                             const float* nz, const uint32_t* seed_next, int64_t n,
+                             double* u_tilde, double* g_tilde);
+
+/* Mesh queries against primitive index prim (MUPSI_PRIM_MESH) or, when
+ * prim == -1, the GP mean mesh.  World-space inputs, reference semantics. */
+mupsi_status mupsi_mesh_intersect(mupsi_scene* scene, int32_t prim,
+                                  const float* ox, const float* oy, const float* oz,
+                                  const float* dx, const float* dy, const float* dz,
+                                  const float* near_t, const float* far_t, int64_t n,
+                                  int32_t* tri, float* t, float* u, float* v);
+mupsi_status mupsi_mesh_occluded(mupsi_scene* scene, int32_t prim,
+                                 const float* ox, const float* oy, const float* oz,
+                                 const float* dx, const float* dy, const float* dz,
+                                 const float* near_t, const float* far_t, int64_t n,
+                                 uint8_t* occluded);
+mupsi_status mupsi_mesh_closest(mupsi_scene* scene, int32_t prim, const float* px,
+                                const float* py, const float* pz, int64_t n,
+                                float* dist, int32_t* tri, float* closest3);
+mupsi_status mupsi_mesh_inside(mupsi_scene* scene, int32_t prim, const float* px,
+                               const float* py, const float* pz, int64_t n, uint8_t* inside);
+
+/* Host-side BVH build with the reference topology (post-order node array, the
+ * root is the last node).  nodes_out: 8 floats/ints per node packed as
+ * {minx,miny,minz,maxx,maxy,maxz, left(u32 bits), right(u32 bits)};
+ * capacity 2*n_faces nodes.  tri_index_out: n_faces entries. */
+mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices, const int32_t* faces,
+                             int32_t n_faces, float* nodes_out, int32_t* n_nodes_out,
+                             uint32_t* tri_index_out, uint32_t* root_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUPSI_H_ */

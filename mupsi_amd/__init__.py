@@ -1,0 +1,197 @@
+"""mupsi_amd — MI355X (gfx950) GPIS path tracer, a drop-in for mupsi's CPU trace loop.
+
+Host-side mirror of the reference's render entry points over the C ABI of
+libmupsi_hip.so (include/mupsi.h):
+
+    ctx = Context()                       # one HIP stream on one GPU
+    scene = Scene(ctx, scene_dict)        # Controller::load + Renderer::prepareRender
+    r = Renderer(scene)                   # Renderer
+    r.start_render_progressive(spp, on_spp)   # Renderer::startRenderProgressive
+    img = r.framebuffer()                 # Renderer::getFramebuffer (mean radiance)
+
+There is no CPU fallback: every call fails with MupsiError when the HIP
+library or a gfx950 device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from pathlib import Path
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import MupsiError, check
+from .scene import build_desc, write_scene_files
+
+__all__ = ["Context", "Scene", "Renderer", "MupsiError", "build_desc", "write_scene_files",
+           "library_path"]
+
+
+def library_path() -> Path:
+    return _abi.LIB_PATH
+
+
+class Context:
+    """mupsi_open / mupsi_close: one HIP stream on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        check(self.lib.mupsi_open(device, C.byref(h)), "mupsi_open")
+        self.handle = h
+        self.device = device
+
+    def info(self) -> tuple[str, int]:
+        name = C.create_string_buffer(256)
+        cus = C.c_int32()
+        check(self.lib.mupsi_device_info(self.handle, name, 256, C.byref(cus)), "mupsi_device_info")
+        return name.value.decode(), cus.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.mupsi_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """An uploaded scene (mupsi_scene_upload / mupsi_scene_load_json)."""
+
+    def __init__(self, ctx: Context, scene: dict | str | os.PathLike):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        self.spp = 1
+        if isinstance(scene, (str, os.PathLike)):
+            spp = C.c_int32()
+            check(self.lib.mupsi_scene_load_json(ctx.handle, str(scene).encode(), C.byref(h),
+                                                 C.byref(spp)), "mupsi_scene_load_json")
+            self.spp = spp.value
+        else:
+            desc, holder = build_desc(scene)
+            check(self.lib.mupsi_scene_upload(ctx.handle, C.byref(desc), C.byref(h)),
+                  "mupsi_scene_upload")
+            del holder
+            self.spp = int(scene.get("spp", 1))
+        self.handle = h
+        w, hh = C.c_int32(), C.c_int32()
+        check(self.lib.mupsi_scene_size(h, C.byref(w), C.byref(hh)), "mupsi_scene_size")
+        self.width, self.height = w.value, hh.value
+
+    def render_passes(self, k_begin: int, k_count: int, passes_per_launch: int = 0,
+                      abort_flag: Optional[C.c_int32] = None) -> None:
+        ptr = C.cast(C.byref(abort_flag), C.c_void_p) if abort_flag is not None else None
+        check(self.lib.mupsi_render_passes(self.handle, k_begin, k_count, passes_per_launch, ptr),
+              "mupsi_render_passes")
+
+    def reset(self) -> None:
+        check(self.lib.mupsi_reset_accum(self.handle), "mupsi_reset_accum")
+
+    def read_accum(self) -> tuple[np.ndarray, int]:
+        buf = np.zeros((self.height, self.width, 3), np.float64)
+        cnt = C.c_int64()
+        check(self.lib.mupsi_read_accum(self.handle, buf.ctypes.data_as(C.c_void_p), C.byref(cnt)),
+              "mupsi_read_accum")
+        return buf, cnt.value
+
+    def write_accum(self, sums: np.ndarray, count: int) -> None:
+        sums = np.ascontiguousarray(sums, dtype=np.float64)
+        check(self.lib.mupsi_write_accum(self.handle, sums.ctypes.data_as(C.c_void_p), count),
+              "mupsi_write_accum")
+
+    def image(self) -> np.ndarray:
+        s, n = self.read_accum()
+        return s / max(n, 1)
+
+    def trace_paths(self, x, y, k) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        k = np.ascontiguousarray(k, np.int32)
+        out = np.zeros((x.size, 3), np.float32)
+        check(self.lib.mupsi_trace_paths(self.handle, x.ctypes.data_as(C.c_void_p),
+                                         y.ctypes.data_as(C.c_void_p),
+                                         k.ctypes.data_as(C.c_void_p), x.size,
+                                         out.ctypes.data_as(C.c_void_p)), "mupsi_trace_paths")
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.Stats()
+        check(self.lib.mupsi_last_stats(self.handle, C.byref(s)), "mupsi_last_stats")
+        return {name: getattr(s, name) for name, _ in s._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.mupsi_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Renderer:
+    """Renderer (src/rendering/renderer.h:14-40) on the GPU.
+
+    start_render_progressive(target_spp, on_spp) runs passes k = 0..target_spp-1,
+    calls on_spp(k+1) after each completed pass group and returns False if the
+    cancel flag was raised (the sums keep the last completed group), True otherwise.
+    """
+
+    def __init__(self, scene: Scene, passes_per_call: int = 0):
+        self.scene = scene
+        self._cancel = C.c_int32(0)
+        self.passes_per_call = passes_per_call
+        self._done = 0
+        self._target = 0
+
+    def set_cancel_flag(self, flag: threading.Event | None = None) -> None:
+        self._cancel_event = flag
+
+    def cancel(self) -> None:
+        self._cancel.value = 1
+
+    def prepare_render(self) -> None:
+        self.scene.reset()
+        self._done = 0
+
+    def start_render(self, spp: int) -> bool:
+        return self.start_render_progressive(spp)
+
+    def start_render_progressive(self, target_spp: int,
+                                 on_spp: Optional[Callable[[int], None]] = None) -> bool:
+        self._target = target_spp
+        self._done = 0
+        step = self.passes_per_call or (target_spp if on_spp is None else 1)
+        k = 0
+        while k < target_spp:
+            ev = getattr(self, "_cancel_event", None)
+            if self._cancel.value or (ev is not None and ev.is_set()):
+                return False
+            n = min(step, target_spp - k)
+            try:
+                self.scene.render_passes(k, n, 0, self._cancel)
+            except MupsiError as e:
+                if e.status == 4:  # aborted
+                    return False
+                raise
+            k += n
+            self._done = k
+            if on_spp:
+                on_spp(k)
+        return True
+
+    def progress(self) -> int:
+        return 0 if self._target == 0 else (100 * self._done) // self._target
+
+    def framebuffer(self) -> np.ndarray:
+        return self.scene.image()

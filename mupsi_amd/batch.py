@@ -1,0 +1,158 @@
+"""Batch entry points of the C ABI on HBM-resident torch tensors.
+
+Each function takes/returns CUDA(HIP) tensors and passes their device
+pointers straight to libmupsi_hip.so (no host staging).  torch is only the
+device-memory allocator here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _abi as A
+from ._abi import check
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev(t):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def gp_params(kernel: dict) -> A.GpParams:
+    g = A.GpParams()
+    g.sigma = float(kernel.get("sigma", 1.0))
+    g.kernel_radius = float(kernel.get("length_scale", 1.0))
+    g.aniso[:] = [float(x) for x in kernel.get("aniso", [1, 1, 1])]
+    g.points_per_cell = int(kernel.get("points_per_cell", 3))
+    return g
+
+
+def _cond(cond):
+    """cond: None or dict of device tensors active(u8), C(n,3), u(n), g(n,3)."""
+    if cond is None:
+        return None, []
+    torch = _torch()
+    keep = [cond["active"].to(torch.uint8).contiguous()]
+    Cc = cond["C"].float()
+    g = cond["g"].float()
+    cols = [Cc[:, 0].contiguous(), Cc[:, 1].contiguous(), Cc[:, 2].contiguous(),
+            cond["u"].float().contiguous(), g[:, 0].contiguous(), g[:, 1].contiguous(),
+            g[:, 2].contiguous()]
+    keep += cols
+    s = A.CondSoA(*[C.c_void_p(x.data_ptr()) for x in keep])
+    return s, keep
+
+
+def psi_eval(ctx, kernel: dict, p, seed):
+    torch = _torch()
+    n = p.shape[0]
+    px, py, pz = (p[:, i].contiguous() for i in range(3))
+    out = torch.empty((n, 4), dtype=torch.float32, device=p.device)
+    l1 = torch.empty(n, dtype=torch.float32, device=p.device)
+    g = gp_params(kernel)
+    check(ctx.lib.mupsi_psi_eval(ctx.handle, C.byref(g), _dev(px), _dev(py), _dev(pz),
+                                 _dev(seed), n, _dev(out), _dev(l1)), "mupsi_psi_eval")
+    return out, l1
+
+
+def cell_impulses(ctx, kernel: dict, cells, seed):
+    torch = _torch()
+    n = cells.shape[0]
+    ppc = int(kernel.get("points_per_cell", 3))
+    cs = torch.empty(n, dtype=torch.int32, device=cells.device)
+    pos = torch.empty((n, ppc, 3), dtype=torch.float32, device=cells.device)
+    w = torch.empty((n, ppc), dtype=torch.int8, device=cells.device)
+    g = gp_params(kernel)
+    check(ctx.lib.mupsi_cell_impulses(ctx.handle, C.byref(g), _dev(cells.contiguous()), _dev(seed),
+                                      n, _dev(cs), _dev(pos), _dev(w)), "mupsi_cell_impulses")
+    return cs, pos, w
+
+
+def field_eval(scene, p, seed, cond=None):
+    torch = _torch()
+    n = p.shape[0]
+    px, py, pz = (p[:, i].contiguous() for i in range(3))
+    out = torch.empty((n, 4), dtype=torch.float32, device=p.device)
+    cs, keep = _cond(cond)
+    check(scene.lib.mupsi_field_eval(scene.handle, _dev(px), _dev(py), _dev(pz), _dev(seed),
+                                     C.byref(cs) if cs is not None else None, n, _dev(out)),
+          "mupsi_field_eval")
+    return out
+
+
+def march(scene, rays, seed, cond=None, mode: int = 0):
+    """rays: (n, 8) device tensor {ox, oy, oz, dx, dy, dz, near_t, far_t}."""
+    torch = _torch()
+    n = rays.shape[0]
+    cols = [rays[:, i].contiguous() for i in range(8)]
+    dev = rays.device
+    hit = torch.empty(n, dtype=torch.uint8, device=dev)
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    p = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    nrm = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    steps = torch.empty(n, dtype=torch.int32, device=dev)
+    cs, keep = _cond(cond)
+    check(scene.lib.mupsi_march(scene.handle, mode, *[_dev(c) for c in cols], _dev(seed),
+                                C.byref(cs) if cs is not None else None, n, _dev(hit), _dev(t),
+                                _dev(p), _dev(nrm), _dev(steps)), "mupsi_march")
+    return {"hit": hit, "t": t, "p": p, "n": nrm, "steps": steps}
+
+
+def condition(scene, C3, N3, seed_next):
+    torch = _torch()
+    n = C3.shape[0]
+    cols = [C3[:, i].contiguous() for i in range(3)] + [N3[:, i].contiguous() for i in range(3)]
+    u = torch.empty(n, dtype=torch.float64, device=C3.device)
+    g = torch.empty((n, 3), dtype=torch.float64, device=C3.device)
+    check(scene.lib.mupsi_condition(scene.handle, *[_dev(c) for c in cols], _dev(seed_next), n,
+                                    _dev(u), _dev(g)), "mupsi_condition")
+    return u, g
+
+
+def mesh_intersect(scene, prim: int, rays):
+    torch = _torch()
+    n = rays.shape[0]
+    cols = [rays[:, i].contiguous() for i in range(8)]
+    tri = torch.empty(n, dtype=torch.int32, device=rays.device)
+    t, u, v = (torch.empty(n, dtype=torch.float32, device=rays.device) for _ in range(3))
+    check(scene.lib.mupsi_mesh_intersect(scene.handle, prim, *[_dev(c) for c in cols], n, _dev(tri),
+                                         _dev(t), _dev(u), _dev(v)), "mupsi_mesh_intersect")
+    return tri, t, u, v
+
+
+def mesh_occluded(scene, prim: int, rays):
+    torch = _torch()
+    n = rays.shape[0]
+    cols = [rays[:, i].contiguous() for i in range(8)]
+    occ = torch.empty(n, dtype=torch.uint8, device=rays.device)
+    check(scene.lib.mupsi_mesh_occluded(scene.handle, prim, *[_dev(c) for c in cols], n, _dev(occ)),
+          "mupsi_mesh_occluded")
+    return occ
+
+
+def mesh_closest(scene, prim: int, p):
+    torch = _torch()
+    n = p.shape[0]
+    px, py, pz = (p[:, i].contiguous() for i in range(3))
+    d = torch.empty(n, dtype=torch.float32, device=p.device)
+    tri = torch.empty(n, dtype=torch.int32, device=p.device)
+    cl = torch.empty((n, 3), dtype=torch.float32, device=p.device)
+    check(scene.lib.mupsi_mesh_closest(scene.handle, prim, _dev(px), _dev(py), _dev(pz), n, _dev(d),
+                                       _dev(tri), _dev(cl)), "mupsi_mesh_closest")
+    return d, tri, cl
+
+
+def mesh_inside(scene, prim: int, p):
+    torch = _torch()
+    n = p.shape[0]
+    px, py, pz = (p[:, i].contiguous() for i in range(3))
+    out = torch.empty(n, dtype=torch.uint8, device=p.device)
+    check(scene.lib.mupsi_mesh_inside(scene.handle, prim, _dev(px), _dev(py), _dev(pz), n, _dev(out)),
+          "mupsi_mesh_inside")
+    return out

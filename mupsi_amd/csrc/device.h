@@ -1,0 +1,586 @@
+// device.h — gfx950 device functions of the GPIS path tracer.
+//
+// Float expressions follow one fixed operation order (the library is compiled
+// with -ffp-contract=off; the only fused multiply-adds are the explicit
+// fmaf() calls), chosen to match what GCC -O3 -march=native makes of the
+// reference's Eigen expressions:
+//   |v|^2 = fmaf(x,x, fmaf(y,y, z*z)),  a.b likewise,  o + d*t = fmaf(d,t,o).
+// Integer work (cell index, make_seed, PCG draws, impulse positions and
+// weights, BVH traversal order) is bit-exact with the reference algorithm.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "host_bvh.h"
+
+namespace mupsi {
+namespace dev {
+
+// ------------------------------------------------------------------ vec3
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 mk(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 cmul(F3 a, F3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ F3 divs(F3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ F3 neg(F3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float sq3(F3 v) { return fmaf(v.x, v.x, fmaf(v.y, v.y, v.z * v.z)); }
+__device__ __forceinline__ float dot(F3 a, F3 b) {
+  return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+}
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return mk(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)),
+            fmaf(a.x, b.y, -(a.y * b.x)));
+}
+__device__ __forceinline__ float norm(F3 v) { return sqrtf(sq3(v)); }
+__device__ __forceinline__ F3 normalized(F3 v) {
+  float z = sq3(v);
+  return z > 0.0f ? divs(v, sqrtf(z)) : v;
+}
+__device__ __forceinline__ F3 madd(F3 d, float t, F3 o) {
+  return mk(fmaf(d.x, t, o.x), fmaf(d.y, t, o.y), fmaf(d.z, t, o.z));
+}
+__device__ __forceinline__ float comp(F3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ __forceinline__ float max3(F3 v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
+__device__ __forceinline__ F3 ld3(const float* p, int64_t i) {
+  return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+
+// ------------------------------------------------------------------ RNG  (random.h:13-119)
+__device__ __forceinline__ uint32_t hash_combine(uint32_t h, uint32_t v) {
+  return h ^ (v + 0x9e3779b9u + (h << 6) + (h >> 2));
+}
+__device__ __forceinline__ uint32_t make_seed4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  uint32_t h = hash_combine(0u, a);
+  h = hash_combine(h, b);
+  h = hash_combine(h, c);
+  return hash_combine(h, d);
+}
+__device__ __forceinline__ uint32_t make_seed5(uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                               uint32_t e) {
+  return hash_combine(make_seed4(a, b, c, d), e);
+}
+// PCG-XSH-RR 64/32, multiplier 6364136223846793005, increment 1.
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& s) {
+  const uint64_t old = s;
+  s = old * 6364136223846793005ULL + 1ULL;
+  const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  const uint32_t rot = (uint32_t)(old >> 59u);
+  return __builtin_amdgcn_alignbit(xs, xs, rot);  // rotate right by rot
+}
+// Random::normalizedUint: mantissa fill, [0, 1)
+__device__ __forceinline__ float u01(uint32_t i) {
+  return __uint_as_float((i >> 9u) | 0x3F800000u) - 1.0f;
+}
+struct Pcg {
+  uint64_t s;
+  __device__ __forceinline__ uint32_t nextI() { return pcg_next(s); }
+  __device__ __forceinline__ float next1D() { return u01(pcg_next(s)); }
+};
+// Random(seed): zero seed becomes 1, then two warm-up draws.
+__device__ __forceinline__ uint64_t pcg_ctor_state(uint64_t seed) {
+  uint64_t s = seed == 0 ? 1ULL : seed;
+  pcg_next(s);
+  pcg_next(s);
+  return s;
+}
+// ConstantSampler(seed).nextI(): the 5th output of Random(seed).
+__device__ __forceinline__ uint32_t constant_sampler_i(uint32_t seed) {
+  uint64_t s = pcg_ctor_state(seed);
+  pcg_next(s);
+  pcg_next(s);
+  return pcg_next(s);
+}
+
+// ------------------------------------------------------------------ scene structs
+struct DevXf {  // Translation(pos) * Scaling(s) and its inverse
+  float s, inv_s;
+  float pos[3], inv_t[3];
+  int identity;
+};
+struct DevTex {
+  const float* rgb;
+  int w, h;
+};
+struct DevMesh {
+  const BvhNode* nodes;
+  const uint8_t* order;      // per internal node: axis | (lc<rc)<<2 | (lc>rc)<<3
+  const float4* leaf_tris;   // 3 per leaf slot: (v0, id bits), (v1, 0), (v2, 0)
+  const float* vnormals;     // per vertex, or null
+  const int32_t* faces;
+  const float* fnormals;
+  const float* texcoords;    // or null
+  const int32_t* face_tex;   // or null
+  int n_nodes, n_faces;
+  uint32_t root;
+  DevXf xf;
+  float lbmin[3], lbmax[3];  // local-space bounds of the whole mesh
+};
+struct DevPrim {
+  int type, bsdf;
+  DevXf xf;
+  int has_em;
+  float em[3];
+  int int_medium;
+  int mesh;
+  DevTex albedo;
+};
+struct DevGP {
+  float sigma, kr, L2, nk2, sqrt_var;  // nk2 = -2/L2
+  float k4;                            // 4/(L2*L2)
+  float bound;                         // |psi| <= bound for every p
+  float grad_h_max;                    // max |grad h| = sqrt(2) e^{-1/2} / L
+  int ppc;
+};
+struct DevScene {
+  float cam_pos[3], cam_dir[3], cam_right[3], cam_tup[3];
+  float aspect, tan_half;
+  int w, h;
+  int max_bounce, gp_mode, skip_medium, has_gp;
+  int mean_type;
+  float mean_c[3], mean_r;
+  DevMesh mean_mesh;
+  DevGP gp;
+  DevTex sky;
+  const DevPrim* prims;
+  int n_prims;
+  const DevMesh* meshes;
+  int n_lights;
+  int lights[16];
+  float eps, near_eps;
+  uint32_t single_seed_i;  // ConstantSampler(42).nextI()
+};
+
+struct Cond {
+  bool active;
+  F3 C;
+  float u;
+  F3 g;
+};
+
+struct Ray {
+  F3 o, d;
+  float nearT, farT;
+};
+__device__ __forceinline__ Ray make_ray(F3 o, F3 d) {
+  return Ray{o, normalized(d), 0.0f, 3.402823466e+38f};
+}
+
+// ------------------------------------------------------------------ GP noise
+// One cell's impulses are regenerated from its seed (no table): cell seed
+// make_seed(cx, cy, cz, seed), state cell_seed + 1 (u32 wrap), per impulse
+// draws d0..d3, offset (d2, d1, d0) (GCC's right-to-left argument order),
+// weight -1 if d3 < 0.5 else +1, position fmaf(offset, kr, n * kr).
+// (gpnoise.cpp:57-74).  d3 < 0.5  <=>  the draw's top bit is clear.
+
+__device__ __forceinline__ void cell_index(const DevGP& g, F3 p, int& cx, int& cy, int& cz) {
+  cx = (int)(p.x / g.kr);
+  cy = (int)(p.y / g.kr);
+  cz = (int)(p.z / g.kr);
+}
+
+// psi value and gradient at p (RawNoise, gpnoise.h:44); sum of |h| in *mass.
+template <bool kGrad>
+__device__ __forceinline__ void psi_eval(const DevGP& g, F3 p, uint32_t seed, float& v,
+                                         F3& grad, float* mass) {
+  int cx, cy, cz;
+  cell_index(g, p, cx, cy, cz);
+  float sv = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, m = 0.0f;
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int nx = cx + dx;
+    const float bx = (float)nx * g.kr;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int ny = cy + dy;
+      const float by = (float)ny * g.kr;
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int nz = cz + dz;
+        const float bz = (float)nz * g.kr;
+        uint64_t s = (uint32_t)(make_seed4((uint32_t)nx, (uint32_t)ny, (uint32_t)nz, seed) + 1u);
+        for (int i = 0; i < g.ppc; ++i) {
+          const uint32_t r0 = pcg_next(s);
+          const uint32_t r1 = pcg_next(s);
+          const uint32_t r2 = pcg_next(s);
+          const uint32_t r3 = pcg_next(s);
+          const float ddx = p.x - fmaf(u01(r2), g.kr, bx);
+          const float ddy = p.y - fmaf(u01(r1), g.kr, by);
+          const float ddz = p.z - fmaf(u01(r0), g.kr, bz);
+          const float r2v = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+          const float h = expf(-r2v / g.L2);
+          const bool pos = (r3 >> 31) != 0;
+          sv = pos ? sv + h : sv - h;
+          if (kGrad) {
+            const float k = h * g.nk2;
+            sx = pos ? sx + ddx * k : sx - ddx * k;
+            sy = pos ? sy + ddy * k : sy - ddy * k;
+            sz = pos ? sz + ddz * k : sz - ddz * k;
+          }
+          if (mass) m += h;
+        }
+      }
+    }
+  }
+  v = g.sigma * sv / g.sqrt_var;
+  if (kGrad) grad = mk(g.sigma * sx / g.sqrt_var, g.sigma * sy / g.sqrt_var, g.sigma * sz / g.sqrt_var);
+  if (mass) *mass = g.sigma * m / g.sqrt_var;
+}
+
+__device__ __forceinline__ float kern_h(const DevGP& g, F3 s, F3 p) {
+  return expf(-sq3(sub(p, s)) / g.L2);
+}
+
+// ------------------------------------------------------------------ transforms
+__device__ __forceinline__ F3 xf_to_local(const DevXf& x, F3 p) {
+  return mk(fmaf(x.inv_s, p.x, x.inv_t[0]), fmaf(x.inv_s, p.y, x.inv_t[1]),
+            fmaf(x.inv_s, p.z, x.inv_t[2]));
+}
+__device__ __forceinline__ F3 xf_to_world(const DevXf& x, F3 p) {
+  return mk(fmaf(x.s, p.x, x.pos[0]), fmaf(x.s, p.y, x.pos[1]), fmaf(x.s, p.z, x.pos[2]));
+}
+
+// ------------------------------------------------------------------ BVH
+__device__ __forceinline__ bool aabb_hit(const BvhNode& n, const Ray& r) {
+  float tmin = -INFINITY, tmax = INFINITY;
+  const float o[3] = {r.o.x, r.o.y, r.o.z};
+  const float d[3] = {r.d.x, r.d.y, r.d.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (fabsf(d[a]) > 1e-8f) {
+      const float t1 = (n.bmin[a] - o[a]) / d[a];
+      const float t2 = (n.bmax[a] - o[a]) / d[a];
+      const float lo = t2 < t1 ? t2 : t1;  // std::min(t1, t2)
+      const float hi = t1 < t2 ? t2 : t1;  // std::max(t1, t2)
+      tmin = tmin < lo ? lo : tmin;
+      tmax = hi < tmax ? hi : tmax;
+    } else if (o[a] < n.bmin[a] || o[a] > n.bmax[a]) {
+      return false;
+    }
+  }
+  return tmax >= tmin && tmax >= r.nearT && tmin <= r.farT;
+}
+
+// Moller-Trumbore (bvh.cpp:82-111)
+__device__ __forceinline__ bool ray_tri(const Ray& r, F3 v0, F3 v1, F3 v2, float& t, float& u,
+                                        float& v) {
+  const F3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+  const F3 h = cross(r.d, e2);
+  const float a = dot(e1, h);
+  if (fabsf(a) < 1e-8f) return false;
+  const float f = 1.0f / a;
+  const F3 s = sub(r.o, v0);
+  u = f * dot(s, h);
+  if (u < 0.0f || u > 1.0f) return false;
+  const F3 q = cross(s, e1);
+  v = f * dot(r.d, q);
+  if (v < 0.0f || u + v > 1.0f) return false;
+  t = f * dot(e2, q);
+  return t > r.nearT && t < r.farT;
+}
+
+__device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, uint32_t i) {
+  const float4* p = reinterpret_cast<const float4*>(nodes + i);
+  const float4 a = p[0], b = p[1];
+  BvhNode n;
+  n.bmin[0] = a.x;
+  n.bmin[1] = a.y;
+  n.bmin[2] = a.z;
+  n.bmax[0] = a.w;
+  n.bmax[1] = b.x;
+  n.bmax[2] = b.y;
+  n.left = __float_as_uint(b.z);
+  n.right = __float_as_uint(b.w);
+  return n;
+}
+
+__device__ __forceinline__ void leaf_tri(const DevMesh& m, uint32_t slot, F3& v0, F3& v1, F3& v2,
+                                         uint32_t& id) {
+  const float4 a = m.leaf_tris[3 * slot], b = m.leaf_tris[3 * slot + 1],
+               c = m.leaf_tris[3 * slot + 2];
+  v0 = mk(a.x, a.y, a.z);
+  v1 = mk(b.x, b.y, b.z);
+  v2 = mk(c.x, c.y, c.z);
+  id = __float_as_uint(a.w);
+}
+
+constexpr int kStack = 64;
+
+__device__ __forceinline__ void push_children(const DevMesh& m, const BvhNode& nd, uint32_t ni,
+                                              const Ray& r, uint32_t* st, int& sp) {
+  const uint32_t o = m.order[ni];
+  const int axis = o & 3;
+  const bool nearFirst = (comp(r.d, axis) > 0) ? ((o >> 2) & 1) : ((o >> 3) & 1);
+  const uint32_t first = nearFirst ? nd.left : nd.right;
+  const uint32_t second = nearFirst ? nd.right : nd.left;
+  if (sp <= kStack - 2) {  // the reference's 64-entry stack; deeper trees are not traversed past it
+    st[sp++] = second;
+    st[sp++] = first;
+  }
+}
+
+// BVH::intersect (bvh.cpp:197-250): front-to-back stack traversal from the root.
+__device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, float& v) {
+  if (m.n_nodes == 0) return false;
+  bool hit = false;
+  uint32_t st[kStack];
+  int sp = 0;
+  st[sp++] = m.root;
+  while (sp > 0) {
+    const uint32_t ni = st[--sp];
+    const BvhNode nd = load_node(m.nodes, ni);
+    if (!aabb_hit(nd, r)) continue;
+    if (nd.right & kLeafFlag) {
+      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+      for (uint32_t i = s0; i < s0 + c; ++i) {
+        F3 v0, v1, v2;
+        uint32_t id;
+        leaf_tri(m, i, v0, v1, v2, id);
+        float t, uu, vv;
+        if (ray_tri(r, v0, v1, v2, t, uu, vv)) {
+          r.farT = t;
+          ti = id;
+          u = uu;
+          v = vv;
+          hit = true;
+        }
+      }
+    } else {
+      push_children(m, nd, ni, r, st, sp);
+    }
+  }
+  return hit;
+}
+
+// BVH::occluded (bvh.cpp:254-300): any hit, traversal starts at node 0.
+__device__ bool bvh_occluded(const DevMesh& m, const Ray& r) {
+  if (m.n_nodes == 0) return false;
+  uint32_t st[kStack];
+  int sp = 0;
+  st[sp++] = 0;
+  while (sp > 0) {
+    const uint32_t ni = st[--sp];
+    const BvhNode nd = load_node(m.nodes, ni);
+    if (!aabb_hit(nd, r)) continue;
+    if (nd.right & kLeafFlag) {
+      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+      for (uint32_t i = s0; i < s0 + c; ++i) {
+        F3 v0, v1, v2;
+        uint32_t id;
+        leaf_tri(m, i, v0, v1, v2, id);
+        float t, uu, vv;
+        if (ray_tri(r, v0, v1, v2, t, uu, vv)) return true;
+      }
+    } else {
+      push_children(m, nd, ni, r, st, sp);
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ float dist2_aabb(F3 p, const BvhNode& n) {
+  const float dx = fmaxf(fmaxf(n.bmin[0] - p.x, 0.0f), p.x - n.bmax[0]);
+  const float dy = fmaxf(fmaxf(n.bmin[1] - p.y, 0.0f), p.y - n.bmax[1]);
+  const float dz = fmaxf(fmaxf(n.bmin[2] - p.z, 0.0f), p.z - n.bmax[2]);
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+__device__ __forceinline__ float dist2_line(F3 p, F3 a, F3 b, F3& out) {
+  const F3 ab = sub(b, a), ap = sub(p, a);
+  const float ab2 = sq3(ab);
+  if (ab2 < 1e-6f) {
+    out = a;
+    return sq3(ap);
+  }
+  float t = dot(ap, ab) / ab2;
+  t = fminf(fmaxf(t, 0.0f), 1.0f);
+  const F3 proj = madd(ab, t, a);
+  out = proj;
+  return sq3(sub(p, proj));
+}
+// Point-triangle squared distance (bvh.cpp:42-80), including the reference's
+// |e1 x e2|^2 < 1e-6 "degenerate" test that falls back to vertex a.
+__device__ __forceinline__ float dist2_tri(F3 p, F3 a, F3 b, F3 c, F3& out) {
+  const F3 e1 = sub(a, b), e2 = sub(a, c);
+  F3 nrm = cross(e1, e2);
+  if (sq3(nrm) < 1e-6f) {
+    out = a;
+    return sq3(sub(p, a));
+  }
+  nrm = normalized(nrm);
+  const float dp = dot(sub(p, a), nrm);
+  const F3 pj = mk(fmaf(-dp, nrm.x, p.x), fmaf(-dp, nrm.y, p.y), fmaf(-dp, nrm.z, p.z));
+  const F3 ab = neg(e1), bc = sub(c, b), ca = e2;
+  const F3 c1 = cross(ab, sub(pj, a)), c2 = cross(bc, sub(pj, b)), c3 = cross(ca, sub(pj, c));
+  if (dot(c1, nrm) >= 0 && dot(c2, nrm) >= 0 && dot(c3, nrm) >= 0) {
+    out = pj;
+    return dp * dp;
+  }
+  float mn = INFINITY, d;
+  F3 tmp;
+  if ((d = dist2_line(p, a, b, tmp)) < mn) { mn = d; out = tmp; }
+  if ((d = dist2_line(p, b, c, tmp)) < mn) { mn = d; out = tmp; }
+  if ((d = dist2_line(p, c, a, tmp)) < mn) { mn = d; out = tmp; }
+  return mn;
+}
+
+// BVH::closest (bvh.cpp:302-353): best-first with |.|^2 pruning.
+__device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out) {
+  if (m.n_nodes == 0) return 0.0f;
+  uint32_t st[kStack];
+  int sp = 0;
+  st[sp++] = m.root;
+  float md = 3.402823466e+38f;
+  while (sp > 0) {
+    const BvhNode nd = load_node(m.nodes, st[--sp]);
+    if (dist2_aabb(p, nd) >= md) continue;
+    if (nd.right & kLeafFlag) {
+      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+      for (uint32_t i = s0; i < s0 + c; ++i) {
+        F3 v0, v1, v2;
+        uint32_t id;
+        leaf_tri(m, i, v0, v1, v2, id);
+        F3 tmp = mk(0, 0, 0);
+        const float d = dist2_tri(p, v0, v1, v2, tmp);
+        if (d < md) {
+          md = d;
+          ti = id;
+          out = tmp;
+        }
+      }
+    } else {
+      const float dl = dist2_aabb(p, load_node(m.nodes, nd.left));
+      const float dr = dist2_aabb(p, load_node(m.nodes, nd.right));
+      if (sp > kStack - 2) continue;
+      if (dl < dr) {
+        if (dr < md) st[sp++] = nd.right;
+        if (dl < md) st[sp++] = nd.left;
+      } else {
+        if (dl < md) st[sp++] = nd.left;
+        if (dr < md) st[sp++] = nd.right;
+      }
+    }
+  }
+  return sqrtf(md);
+}
+
+// Mesh::intersect (mesh.cpp:185-209): narrows r.farT on hit.
+__device__ bool mesh_intersect(const DevMesh& m, Ray& ray, uint32_t& ti, float& u, float& v) {
+  Ray lr = ray;
+  float tcorr = 1.0f;
+  if (!m.xf.identity) {
+    const F3 lo = xf_to_local(m.xf, ray.o);
+    const F3 ld = mul(ray.d, m.xf.inv_s);
+    const float len = norm(ld);
+    tcorr = norm(mul(divs(ld, len), m.xf.s));
+    lr = make_ray(lo, ld);
+    lr.nearT = ray.nearT * len;
+    lr.farT = ray.farT * len;
+  }
+  const bool hit = bvh_intersect(m, lr, ti, u, v);
+  if (hit) ray.farT = lr.farT * tcorr;
+  return hit;
+}
+__device__ bool mesh_occluded(const DevMesh& m, const Ray& ray) {
+  if (m.xf.identity) return bvh_occluded(m, ray);
+  const F3 lo = xf_to_local(m.xf, ray.o);
+  const F3 ld = mul(ray.d, m.xf.inv_s);
+  const float len = norm(ld);
+  Ray lr = make_ray(lo, ld);
+  lr.nearT = ray.nearT * len;
+  lr.farT = ray.farT * len;
+  return bvh_occluded(m, lr);
+}
+// Mesh::distToPoint (mesh.cpp:325-339)
+__device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out) {
+  const F3 pl = xf_to_local(m.xf, p);
+  F3 ol = mk(0, 0, 0);
+  ti = 0;
+  bvh_closest(m, pl, ti, ol);
+  out = xf_to_world(m.xf, ol);
+  return norm(sub(out, p));
+}
+// Mesh::inside (mesh.cpp:341-362): parity of +x crossings, restarting 1e-4 past each hit.
+__device__ bool mesh_inside(const DevMesh& m, F3 p) {
+  const F3 pl = xf_to_local(m.xf, p);
+  Ray r = make_ray(pl, mk(1, 0, 0));
+  int hits = 0;
+  uint32_t ti;
+  float u, v;
+  if (bvh_intersect(m, r, ti, u, v)) {
+    ++hits;
+    while (r.farT < INFINITY) {
+      r.nearT = r.farT + 1e-4f;
+      r.farT = INFINITY;
+      if (!bvh_intersect(m, r, ti, u, v)) break;
+      ++hits;
+    }
+  }
+  return (hits % 2) == 1;
+}
+__device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {
+  if (m.xf.identity) return n;
+  return normalized(mul(n, m.xf.inv_s));
+}
+
+// ------------------------------------------------------------------ mean + field
+// mu and grad mu (meanfunction.cpp:13-39).  For a mesh mean the reference calls
+// inside() and distToPoint() once for the value and again for the gradient;
+// both calls return the same bits, so one pair serves both here.
+template <bool kGrad>
+__device__ __forceinline__ float mean_eval(const DevScene& S, F3 p, F3* grad) {
+  if (S.mean_type == 0) {
+    const F3 r = sub(p, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
+    if (kGrad) *grad = normalized(r);
+    return norm(r) - S.mean_r;
+  }
+  const float sg = mesh_inside(S.mean_mesh, p) ? -1.0f : 1.0f;
+  uint32_t ti = 0;
+  F3 o;
+  const float d = mesh_dist(S.mean_mesh, p, ti, o);
+  if (kGrad) {
+    if (d < 1e-6f) {
+      const float* fnp = S.mean_mesh.fnormals + 3 * ti;
+      *grad = mesh_world_normal(S.mean_mesh, mk(fnp[0], fnp[1], fnp[2]));
+    } else {
+      *grad = normalized(mul(sub(p, o), sg));
+    }
+  }
+  return sg * d;
+}
+
+// GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).
+template <bool kGrad>
+__device__ __forceinline__ float field_eval(const DevScene& S, F3 p, uint32_t seed, const Cond& c,
+                                            F3* grad) {
+  F3 mg = mk(0, 0, 0), pg = mk(0, 0, 0);
+  const float mu = mean_eval<kGrad>(S, p, &mg);
+  float pv;
+  psi_eval<kGrad>(S.gp, p, seed, pv, pg, nullptr);
+  float f = mu + pv;
+  F3 g = mk(mg.x + pg.x, mg.y + pg.y, mg.z + pg.z);
+  if (c.active) {
+    const DevGP& gp = S.gp;
+    const F3 r = sub(p, c.C);
+    const float h = kern_h(gp, c.C, p);
+    const F3 hg = mul(r, h * gp.nk2);
+    f = f + fmaf(c.u, h, dot(c.g, hg));
+    if (kGrad) {
+      const float k4 = gp.k4 * dot(r, c.g);
+      const F3 in = mk(fmaf(k4, r.x, gp.nk2 * c.g.x), fmaf(k4, r.y, gp.nk2 * c.g.y),
+                       fmaf(k4, r.z, gp.nk2 * c.g.z));
+      const F3 Hg = mul(in, h);
+      g = mk(g.x + fmaf(c.u, hg.x, Hg.x), g.y + fmaf(c.u, hg.y, Hg.y), g.z + fmaf(c.u, hg.z, Hg.z));
+    }
+  }
+  if (kGrad) *grad = g;
+  return f;
+}
+
+// Bound on |conditioning term| for every p: |u| h + |g| |grad h| <= |u| + |g| max|grad h|.
+__device__ __forceinline__ float cond_bound(const DevGP& g, const Cond& c) {
+  if (!c.active) return 0.0f;
+  return (fabsf(c.u) + norm(c.g) * g.grad_h_max) * 1.0001f + 1e-6f;
+}
+
+}  // namespace dev
+}  // namespace mupsi

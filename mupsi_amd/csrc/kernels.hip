@@ -1,0 +1,1644 @@
+// kernels.hip — wavefront GPIS path tracer for gfx950 and its C ABI.
+//
+// Data flow of one mupsi_render_passes launch (paths = pixels x passes):
+//   k_init   : camera ray + path RNG for every path; all paths enter queue A.
+//   per bounce iteration (<= max_bounce):
+//     k_march  : persistent, lane-per-ray GP-medium marcher over queue M
+//                (paths whose ray is inside the GP medium).  Lanes refill from
+//                the queue as their rays finish, so uneven march lengths do
+//                not idle the wave.
+//     k_shade  : medium hits (normal, NEE, Lambertian sample, RR, Renewal+
+//                FP64 solve) and surface/sky events for queue A; survivors are
+//                compacted into the next A and M queues with wave ballots.
+//   k_accum  : per-pixel FP64 sum of the path radiances (fixed pass order).
+// Path state is SoA in HBM, one slot per path of the launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mupsi.h"
+#include "device.h"
+#include "host_bvh.h"
+#include "scene_host.h"
+
+using namespace mupsi;
+using namespace mupsi::dev;
+
+// ====================================================================== device: path state
+
+constexpr uint32_t kBounceMask = 0xffffu;
+constexpr uint32_t kFlagMedium = 1u << 16;
+constexpr uint32_t kFlagCond = 1u << 17;
+
+struct PathSoA {
+  uint64_t* rng;
+  float *ox, *oy, *oz, *dx, *dy, *dz, *nt;
+  float *Tx, *Ty, *Tz, *Lx, *Ly, *Lz;
+  uint32_t* flags;
+  float *cx, *cy, *cz, *cu, *cgx, *cgy, *cgz;
+  uint32_t* mseed;
+  uint8_t* mhit;
+  float *mt, *mpx, *mpy, *mpz;
+};
+
+// Path id -> (x, y, k): explicit lists (replay) or the pixel grid of a launch.
+struct PathSrc {
+  int W, npix, k_begin;
+  const int32_t *xs, *ys, *ks;
+};
+__device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int& y, int& k) {
+  if (s.xs) {
+    x = s.xs[pid];
+    y = s.ys[pid];
+    k = s.ks[pid];
+  } else {
+    const int pix = pid % s.npix;
+    x = pix % s.W;
+    y = pix / s.W;
+    k = s.k_begin + pid / s.npix;
+  }
+}
+
+// Append value to queue if pred; one atomic per wave.
+__device__ __forceinline__ void wave_push(bool pred, int value, int* queue, int* count) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader);
+  if (pred) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  return v;
+}
+
+// ====================================================================== device: camera, sky, BSDF
+
+// Camera::generateRay (camera.cpp:5-19); tan(fov/2) is precomputed on the host
+// with the reference's degrees-as-radians argument.
+__device__ __forceinline__ F3 camera_dir(const DevScene& S, int x, int y) {
+  const float cxf = (x + 0.5f) / S.w;
+  const float cyf = (y + 0.5f) / S.h;
+  const float nx = fmaf(2.0f, cxf, -1.0f) * S.aspect * S.tan_half;
+  const float ny = fmaf(2.0f, cyf, -1.0f) * S.tan_half;
+  const F3 rc = normalized(mk(nx, ny, 1.0f));
+  const F3 w = mk(fmaf(S.cam_dir[0], rc.z, fmaf(S.cam_tup[0], rc.y, S.cam_right[0] * rc.x)),
+                  fmaf(S.cam_dir[1], rc.z, fmaf(S.cam_tup[1], rc.y, S.cam_right[1] * rc.x)),
+                  fmaf(S.cam_dir[2], rc.z, fmaf(S.cam_tup[2], rc.y, S.cam_right[2] * rc.x)));
+  return normalized(w);
+}
+
+// BitmapTexture::operator[] (texture.cpp:52-69)
+__device__ __forceinline__ F3 tex_at(const DevTex& t, float u, float v, int* texel) {
+  u = u - floorf(u);
+  v = v - floorf(v);
+  v = 1.0f - v;
+  int x = (int)(u * t.w) % t.w;
+  int y = (int)(v * t.h) % t.h;
+  if (x < 0) x += t.w;
+  if (y < 0) y += t.h;
+  const int i = y * t.w + x;
+  if (texel) *texel = i;
+  return mk(t.rgb[3 * i], t.rgb[3 * i + 1], t.rgb[3 * i + 2]);
+}
+// Skydrome::intersect (skydrome.cpp:9-18): the float library calls are taken in
+// double and rounded once, which reproduces the correctly rounded results the
+// host math library returns.
+__device__ __forceinline__ F3 sky_lookup(const DevScene& S, F3 d, int* texel) {
+  const float at = (float)atan2((double)d.z, (double)d.x);
+  const float u = (float)(((double)at + M_PI) / (2.0f * M_PI));
+  const float cl = fminf(fmaxf(-d.y, -1.0f), 1.0f);
+  const float ac = (float)acos((double)cl);
+  const float v = (float)((double)ac / M_PI);
+  return tex_at(S.sky, u, v, texel);
+}
+
+struct Frame {
+  F3 n, t, b;
+};
+// TangentFrame(n), Duff et al. 2017 (sample.h:25-35)
+__device__ __forceinline__ Frame make_frame(F3 nin) {
+  Frame f;
+  f.n = normalized(nin);
+  const float sign = copysignf(1.0f, f.n.z);
+  const float a = -1.0f / (sign + f.n.z);
+  const float b = f.n.x * f.n.y * a;
+  f.t = normalized(mk(fmaf(sign * f.n.x * f.n.x, a, 1.0f), sign * b, -sign * f.n.x));
+  f.b = normalized(mk(b, fmaf(f.n.y * f.n.y, a, sign), -f.n.y));
+  return f;
+}
+__device__ __forceinline__ F3 to_global(const Frame& f, F3 p) {
+  return mk(fmaf(f.n.x, p.z, fmaf(f.b.x, p.y, f.t.x * p.x)),
+            fmaf(f.n.y, p.z, fmaf(f.b.y, p.y, f.t.y * p.x)),
+            fmaf(f.n.z, p.z, fmaf(f.b.z, p.y, f.t.z * p.x)));
+}
+
+// Bsdf::weight for a light direction (bsdf.h:51-55, bsdf.cpp:14-21,39-50,57-74)
+__device__ F3 bsdf_weight(int bsdf, F3 alb, F3 wo, F3 n, F3 wi) {
+  if (bsdf == MUPSI_BSDF_LAMBERTIAN) {
+    const float cosv = dot(wi, n);
+    const float pdf = (float)((double)fmaxf(cosv, 0.0f) / M_PI);
+    if (pdf == 0.0f) return mk(0, 0, 0);
+    const float pif = (float)M_PI;
+    return divs(mk(alb.x * cosv / pif, alb.y * cosv / pif, alb.z * cosv / pif), pdf);
+  }
+  if (bsdf == MUPSI_BSDF_NULL) return dot(wi, neg(wo)) > 0.999f ? mk(1, 1, 1) : mk(0, 0, 0);
+  const float k = 2.0f * dot(wi, n);
+  const F3 wr = mk(fmaf(k, n.x, -wi.x), fmaf(k, n.y, -wi.y), fmaf(k, n.z, -wi.z));
+  return dot(wr, wo) > 0.999f ? mk(1, 1, 1) : mk(0, 0, 0);
+}
+// Bsdf::sample (bsdf.cpp:23-37,51-55,75-79).  The Lambertian sample takes
+// phi, sqrt, cos and sin in double as the reference's unqualified calls do.
+__device__ void bsdf_sample(int bsdf, F3 alb, F3 wo, F3 n, Pcg& rng, F3& wi, F3& weight) {
+  if (bsdf == MUPSI_BSDF_LAMBERTIAN) {
+    const float r1 = rng.next1D();
+    const float r2 = rng.next1D();
+    const double phi = (double)(float)(2 * M_PI * (double)r1);
+    const float cosT = (float)sqrt((double)(1 - r2));
+    const float sinT = (float)sqrt((double)r2);
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    const F3 lw = mk((float)((double)sinT * cp), (float)((double)sinT * sp), cosT);
+    wi = to_global(make_frame(n), lw);
+    weight = alb;
+  } else if (bsdf == MUPSI_BSDF_NULL) {
+    wi = neg(wo);
+    weight = mk(1, 1, 1);
+  } else {
+    const float k = 2.0f * dot(wo, n);
+    wi = mk(fmaf(k, n.x, -wo.x), fmaf(k, n.y, -wo.y), fmaf(k, n.z, -wo.z));
+    weight = mk(1, 1, 1);
+  }
+}
+
+// ====================================================================== device: primitives
+
+// intersectUnitSphere (sphere.cpp:11-30)
+__device__ __forceinline__ float unit_sphere_t(float nearT, float farT, F3 O, F3 D) {
+  const float a = sq3(D);
+  const float b = dot(O, D);
+  const float c = sq3(O) - 1.0f;
+  const float disc = fmaf(b, b, -(a * c));
+  if (disc < 0.0f) return -1.0f;
+  const float sd = sqrtf(disc);
+  float t = (-b - sd) / a;
+  if (t > nearT && t < farT) return t;
+  t = (-b + sd) / a;
+  if (t > nearT && t < farT) return t;
+  return -1.0f;
+}
+__device__ __forceinline__ void sphere_local(const DevPrim& pr, const Ray& r, F3& O, F3& D) {
+  O = r.o;
+  D = r.d;
+  if (!pr.xf.identity) {
+    O = xf_to_local(pr.xf, r.o);
+    D = mul(r.d, pr.xf.inv_s);
+  }
+}
+
+// Scene::intersect (scene.cpp:25-39) + intersectInfo of the closest primitive.
+__device__ int scene_intersect(const DevScene& S, Ray& ray, F3& P, F3& Ng, float& uu, float& vv) {
+  float mint = 3.402823466e+38f;
+  int hitp = -1;
+  uint32_t hit_tri = 0;
+  float hu = 0, hv = 0;
+  for (int i = 0; i < S.n_prims; ++i) {
+    const DevPrim& pr = S.prims[i];
+    bool h = false;
+    uint32_t ti = 0;
+    float u = 0, v = 0;
+    if (pr.type == MUPSI_PRIM_SPHERE) {
+      F3 O, D;
+      sphere_local(pr, ray, O, D);
+      const float t = unit_sphere_t(ray.nearT, ray.farT, O, D);
+      if (t >= 0.0f) {
+        ray.farT = t;
+        h = true;
+      }
+    } else {
+      h = mesh_intersect(S.meshes[pr.mesh], ray, ti, u, v);
+    }
+    if (h && ray.farT < mint) {
+      mint = ray.farT;
+      hitp = i;
+      hit_tri = ti;
+      hu = u;
+      hv = v;
+    }
+  }
+  if (hitp < 0) return -1;
+  P = madd(ray.d, ray.farT, ray.o);
+  const DevPrim& pr = S.prims[hitp];
+  if (pr.type == MUPSI_PRIM_SPHERE) {
+    const F3 pl = pr.xf.identity ? P : xf_to_local(pr.xf, P);
+    const F3 ln = normalized(pl);
+    Ng = pr.xf.identity ? ln : normalized(mul(ln, pr.xf.inv_s));
+    uu = 0.0f;
+    vv = 0.0f;
+  } else {
+    const DevMesh& m = S.meshes[pr.mesh];
+    F3 ln;
+    if (!m.vnormals) {
+      ln = ld3(m.fnormals, hit_tri);
+    } else {  // Mesh::interpolateNormal (mesh.cpp:163-172)
+      const int32_t* f = m.faces + 3 * hit_tri;
+      const float w = 1.0f - hu - hv;
+      const F3 n0 = ld3(m.vnormals, f[0]), n1 = ld3(m.vnormals, f[1]), n2 = ld3(m.vnormals, f[2]);
+      const F3 n = mk(fmaf(hv, n2.x, fmaf(hu, n1.x, w * n0.x)),
+                      fmaf(hv, n2.y, fmaf(hu, n1.y, w * n0.y)),
+                      fmaf(hv, n2.z, fmaf(hu, n1.z, w * n0.z)));
+      const float len = norm(n);
+      ln = len > 1e-12f ? divs(n, len) : ld3(m.fnormals, hit_tri);
+    }
+    Ng = mesh_world_normal(m, ln);
+    if (!m.texcoords || !m.face_tex) {
+      uu = hu;
+      vv = hv;
+    } else {  // Mesh::interpolateTexcoord (mesh.cpp:174-181)
+      const int32_t* t = m.face_tex + 3 * hit_tri;
+      const float w = 1.0f - hu - hv;
+      uu = fmaf(hv, m.texcoords[2 * t[2]], fmaf(hu, m.texcoords[2 * t[1]], w * m.texcoords[2 * t[0]]));
+      vv = fmaf(hv, m.texcoords[2 * t[2] + 1],
+                fmaf(hu, m.texcoords[2 * t[1] + 1], w * m.texcoords[2 * t[0] + 1]));
+    }
+  }
+  return hitp;
+}
+
+// Scene::occluded (scene.cpp:41-48)
+__device__ bool scene_occluded(const DevScene& S, const Ray& r) {
+  for (int i = 0; i < S.n_prims; ++i) {
+    const DevPrim& pr = S.prims[i];
+    if (pr.type == MUPSI_PRIM_SPHERE) {
+      F3 O, D;
+      sphere_local(pr, r, O, D);
+      if (unit_sphere_t(r.nearT, r.farT, O, D) >= 0.0f) return true;
+    } else if (mesh_occluded(S.meshes[pr.mesh], r)) {
+      return true;
+    }
+  }
+  return false;
+}
+
+struct LightSample {
+  F3 w, d;
+  float dist, pdf;
+};
+
+// Sphere::sampleDirect (sphere.cpp:91-133)
+__device__ bool sphere_sample_direct(const DevPrim& pr, F3 pw, Pcg& rng, LightSample& ls) {
+  const F3 pl = pr.xf.identity ? pw : xf_to_local(pr.xf, pw);
+  F3 dir = neg(pl);
+  const float dist2 = sq3(dir);
+  const float d2 = dist2 - 1.0f;
+  if (d2 < 1e-6f) return false;
+  const float d = sqrtf(d2);
+  const float dist = sqrtf(dist2);
+  dir = divs(dir, dist);
+  const float cosT = d / dist;
+  const float x1 = rng.next1D();
+  const float x2 = rng.next1D();
+  const float phi = (float)((double)(x1 * 2) * M_PI);
+  const float z = fmaf(x2, 1.0f - cosT, cosT);
+  const float r = sqrtf(fmaxf(fmaf(-z, z, 1.0f), 0.0f));
+  const F3 sd = mk((float)cos((double)phi) * r, (float)sin((double)phi) * r, z);
+  const float B = dist * sd.z;
+  const float det = sqrtf(fmaxf(fmaf(B, B, -d2), 0.0f));
+  ls.dist = B - det;
+  ls.d = to_global(make_frame(dir), sd);
+  if (!pr.xf.identity) {
+    const F3 spw = xf_to_world(pr.xf, madd(ls.d, ls.dist, pl));
+    const F3 wd = sub(spw, pw);
+    ls.dist = norm(wd);
+    ls.d = divs(wd, ls.dist);
+  }
+  ls.pdf = (float)(1.0f / (2.0f * M_PI * (1.0f - cosT)));
+  const F3 em = pr.has_em ? mk(pr.em[0], pr.em[1], pr.em[2]) : mk(0, 0, 0);
+  ls.w = divs(em, fmaxf(ls.pdf, 1e-6f));
+  return true;
+}
+
+// Scene::chooseLight (scene.cpp:50-61): no draw when the scene has no emitter.
+__device__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls) {
+  if (S.n_prims == 0 || S.n_lights == 0) return false;
+  const int idx = (int)(rng.nextI() % (uint32_t)S.n_lights);
+  const DevPrim& pr = S.prims[S.lights[idx]];
+  if (pr.type != MUPSI_PRIM_SPHERE) return false;
+  return sphere_sample_direct(pr, p, rng, ls);
+}
+
+// ====================================================================== device: march
+
+// Decision margin of the certain-sign test (see k_march).
+__device__ __forceinline__ void sign_shell(const DevScene& S, const Cond& c, F3 o, float ft,
+                                          float& outer2, float& inner2) {
+  if (S.mean_type != MUPSI_MEAN_SPHERE) {
+    outer2 = INFINITY;
+    inner2 = -1.0f;
+    return;
+  }
+  const float B = S.gp.bound + cond_bound(S.gp, c);
+  const F3 oc = sub(o, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
+  const float m = 0.01f + 2e-6f * (norm(oc) + ft + S.mean_r + B);
+  const float ro = S.mean_r + B + m;
+  outer2 = ro * ro;
+  const float ri = S.mean_r - B - m;
+  inner2 = ri > 0.0f ? ri * ri : -1.0f;
+}
+
+// Serial march of one ray (used for NEE shadow rays in k_shade): the same
+// decisions as k_march without the certain-sign skip.
+__device__ bool march_serial(const DevScene& S, const Ray& ray, uint32_t seed, const Cond& c) {
+  const float nt = ray.nearT, ft = fminf(ray.farT, 2000.0f);
+  const float dt = fminf(0.5f, (ft - nt) / 100);
+  float t = nt;
+  const float fprev = field_eval<false>(S, ray.o, seed, c, nullptr);
+  const bool sp = fprev > 0.0f;
+  while (t < ft) {
+    const float fc = field_eval<false>(S, madd(ray.d, t, ray.o), seed, c, nullptr);
+    if ((fc > 0.0f) != sp && fabsf(fprev - fc) > 1e-6f) return true;
+    if (!(t + dt > t)) return false;
+    t += dt;
+  }
+  return false;
+}
+
+struct MarchArgs {
+  const int* queue;  // null: identity
+  const int* count_dev;
+  int count_host;
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *nt, *ft;  // ft null: FLT_MAX
+  const uint32_t* seed;
+  const uint32_t* flags;  // render: conditioning-active bit
+  const uint8_t* active;  // batch API: conditioning active, or null
+  const float *cx, *cy, *cz, *cu, *cgx, *cgy, *cgz;
+  uint8_t* hit;
+  float *t, *px, *py, *pz;
+  int32_t* steps;  // t-grid points visited up to and including the hit, or null
+  int* work;
+  unsigned long long* stats;  // [0] field evals, [1] t-grid points (reference work), [2] rays
+};
+
+// GPMedium::sampleDistance / transmittance (gpmedium.cpp:46-129), lane per ray.
+//
+// Reference semantics: dt = min(0.5, (min(farT,2000) - nearT)/100); f_prev =
+// f(origin) stays fixed; t starts at nearT and advances by FP32 t += dt; the
+// first grid point with sign(f) != sign(f_prev) (sign(x) = x > 0) and
+// |f_prev - f| > 1e-6 is the hit; the lerp refinement result is discarded.
+//
+// Certain-sign skip (same outputs, fewer evaluations): |psi| <= gp.bound and
+// |conditioning| <= cond_bound everywhere, so for a sphere mean every grid point
+// with | |p-c| - R | > B + margin has a known sign.  Such points are not
+// evaluated: same sign as f_prev -> no hit; opposite sign -> the hit is there.
+// Past the outer shell and moving away, no later point can change sign.
+__global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
+  const int n = a.count_dev ? *a.count_dev : a.count_host;
+  const int lane = __lane_id();
+  int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
+  int rid = 0;
+  F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  bool sp = false;
+  uint32_t seed = 0;
+  Cond c;
+  c.active = false;
+  int nsteps = 0;
+  unsigned long long evals = 0, refsteps = 0, rays = 0;
+  const F3 mc = mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]);
+
+  auto finish = [&](bool hit, F3 p) {
+    a.hit[rid] = hit ? 1 : 0;
+    if (hit) {
+      a.t[rid] = t;
+      a.px[rid] = p.x;
+      a.py[rid] = p.y;
+      a.pz[rid] = p.z;
+      refsteps += nsteps;
+    } else {
+      refsteps += nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
+    }
+    if (a.steps) a.steps[rid] = hit ? nsteps : -1;
+    phase = 0;
+  };
+
+  while (true) {
+    bool do_eval = false;
+    F3 q = mk(0, 0, 0);
+    for (int round = 0; round < 2; ++round) {
+      const bool need = phase == 0;
+      const uint64_t m = __ballot(need);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.work, __popcll(m));
+        base = __shfl(base, leader);
+        if (need) {
+          const int qi = base + __popcll(m & ((1ull << lane) - 1ull));
+          if (qi < n) {
+            rid = a.queue ? a.queue[qi] : qi;
+            o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
+            d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
+            nt = a.nt[rid];
+            ft = fminf(a.ft ? a.ft[rid] : 3.402823466e+38f, 2000.0f);
+            dt = fminf(0.5f, (ft - nt) / 100);
+            seed = a.seed[rid];
+            c.active = a.flags ? (a.flags[rid] & kFlagCond) != 0 : (a.active && a.active[rid]);
+            if (c.active) {
+              c.C = mk(a.cx[rid], a.cy[rid], a.cz[rid]);
+              c.u = a.cu[rid];
+              c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
+            }
+            sign_shell(S, c, o, ft, outer2, inner2);
+            nsteps = 0;
+            ++rays;
+            phase = 1;
+          } else {
+            phase = 3;
+          }
+        }
+      }
+      if (phase == 1) {
+        q = o;
+        do_eval = true;
+      } else if (phase == 2 && !do_eval) {
+        while (true) {
+          if (!(t < ft)) {
+            finish(false, q);
+            break;
+          }
+          const F3 p = madd(d, t, o);
+          const F3 r = sub(p, mc);
+          const float q2 = sq3(r);
+          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          if (cs == 0) {
+            q = p;
+            do_eval = true;
+            break;
+          }
+          ++nsteps;
+          if ((cs > 0) != sp) {
+            finish(true, p);
+            break;
+          }
+          if (cs > 0 && dot(d, r) >= 0.0f) {
+            finish(false, p);
+            break;
+          }
+          if (!(t + dt > t)) {  // dt below half an ulp of t: the grid cannot advance
+            finish(false, p);
+            break;
+          }
+          t += dt;
+        }
+      }
+      if (__ballot(phase == 0) == 0) break;
+    }
+    if (__ballot(phase != 3) == 0) break;
+    if (do_eval) {
+      const float f = field_eval<false>(S, q, seed, c, nullptr);
+      ++evals;
+      if (phase == 1) {
+        fprev = f;
+        sp = f > 0.0f;
+        t = nt;
+        phase = 2;
+      } else {
+        ++nsteps;
+        if ((f > 0.0f) != sp && fabsf(fprev - f) > 1e-6f) {
+          finish(true, q);
+        } else if (!(t + dt > t)) {  // the grid cannot advance: no further point exists
+          finish(false, q);
+        } else {
+          t += dt;
+        }
+      }
+    }
+  }
+  evals = wave_sum(evals);
+  refsteps = wave_sum(refsteps);
+  rays = wave_sum(rays);
+  if (lane == 0 && a.stats) {
+    atomicAdd(&a.stats[0], evals);
+    atomicAdd(&a.stats[1], refsteps);
+    atomicAdd(&a.stats[2], rays);
+  }
+}
+
+// ====================================================================== device: Renewal+
+
+// GPMedium::sampleCondition (gpmedium.cpp:133-162) as a 4x4 FP64 solve of
+//   [[h(C,C), grad h(C,C)^T], [grad h(C,C), H_h(C,C)]] [u; g] =
+//   [-mu(C)/sigma - psi(C); n - grad mu(C) - grad psi(C)]
+// with psi the next realization.  At r = 0 the matrix is diag(1, -2/L^2 I),
+// which gives the reference's closed form u = -mu/sigma - psi,
+// g = (-L^2/2)(n - grad mu - grad psi).
+__device__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next, double& u,
+                              double g[3]) {
+  F3 pg;
+  float pv;
+  psi_eval<true>(S.gp, C, seed_next, pv, pg, nullptr);
+  F3 mg;
+  const float mu = mean_eval<true>(S, C, &mg);
+  const double L2 = S.gp.L2;
+  const double r[3] = {0.0, 0.0, 0.0};  // C - C
+  const double h = exp(-(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) / L2);
+  double A[4][5];
+  A[0][0] = h;
+  for (int j = 0; j < 3; ++j) {
+    const double gh = h * (-2.0 / L2) * r[j];
+    A[0][1 + j] = gh;
+    A[1 + j][0] = gh;
+    for (int k = 0; k < 3; ++k)
+      A[1 + j][1 + k] = h * ((j == k ? -2.0 / L2 : 0.0) + 4.0 / (L2 * L2) * r[j] * r[k]);
+  }
+  A[0][4] = -(double)mu / (double)S.gp.sigma - (double)pv;
+  A[1][4] = (double)n.x - (double)mg.x - (double)pg.x;
+  A[2][4] = (double)n.y - (double)mg.y - (double)pg.y;
+  A[3][4] = (double)n.z - (double)mg.z - (double)pg.z;
+  for (int col = 0; col < 4; ++col) {  // Gaussian elimination, partial pivoting
+    int piv = col;
+    for (int r2 = col + 1; r2 < 4; ++r2)
+      if (fabs(A[r2][col]) > fabs(A[piv][col])) piv = r2;
+    if (piv != col)
+      for (int k = 0; k < 5; ++k) {
+        const double tmp = A[col][k];
+        A[col][k] = A[piv][k];
+        A[piv][k] = tmp;
+      }
+    for (int r2 = 0; r2 < 4; ++r2) {
+      if (r2 == col) continue;
+      const double f = A[r2][col] / A[col][col];
+      for (int k = col; k < 5; ++k) A[r2][k] -= f * A[col][k];
+    }
+  }
+  u = A[0][4] / A[0][0];
+  for (int j = 0; j < 3; ++j) g[j] = A[1 + j][4] / A[1 + j][1 + j];
+}
+
+// ====================================================================== kernels: wavefront
+
+struct InitArgs {
+  PathSoA P;
+  PathSrc src;
+  int n;
+  int* qA;
+};
+__global__ __launch_bounds__(256) void k_init(DevScene S, InitArgs a) {
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= a.n) return;
+  int x, y, k;
+  path_xyk(a.src, pid, x, y, k);
+  const PathSoA& P = a.P;
+  P.rng[pid] = pcg_ctor_state(make_seed4((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u));
+  const F3 d = normalized(camera_dir(S, x, y));
+  P.ox[pid] = S.cam_pos[0];
+  P.oy[pid] = S.cam_pos[1];
+  P.oz[pid] = S.cam_pos[2];
+  P.dx[pid] = d.x;
+  P.dy[pid] = d.y;
+  P.dz[pid] = d.z;
+  P.nt[pid] = 0.0f;
+  P.Tx[pid] = 1.0f;
+  P.Ty[pid] = 1.0f;
+  P.Tz[pid] = 1.0f;
+  P.Lx[pid] = 0.0f;
+  P.Ly[pid] = 0.0f;
+  P.Lz[pid] = 0.0f;
+  P.flags[pid] = 0u;
+  a.qA[pid] = pid;
+}
+
+struct ShadeArgs {
+  PathSoA P;
+  PathSrc src;
+  const int* qA;
+  const int* nA;
+  int* qA2;
+  int* nA2;
+  int* qM2;
+  int* nM2;
+};
+
+// One iteration of the bounce loop of PathTracer::trace (trace.cpp:138-210) after
+// the march: handleVolume + sampleCondition for medium hits, else
+// Scene::intersect + handleSurface + medium switch, or the skydrome.
+__global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *a.nA;
+  if (qi >= n) return;
+  const int pid = a.qA[qi];
+  const PathSoA& P = a.P;
+  int x, y, k;
+  path_xyk(a.src, pid, x, y, k);
+  uint32_t fl = P.flags[pid];
+  int bounce = (int)(fl & kBounceMask);
+  bool medium = (fl & kFlagMedium) != 0;
+  Cond cur;
+  cur.active = (fl & kFlagCond) != 0;
+  if (cur.active) {
+    cur.C = mk(P.cx[pid], P.cy[pid], P.cz[pid]);
+    cur.u = P.cu[pid];
+    cur.g = mk(P.cgx[pid], P.cgy[pid], P.cgz[pid]);
+  }
+  Pcg rng{P.rng[pid]};
+  F3 T = mk(P.Tx[pid], P.Ty[pid], P.Tz[pid]);
+  F3 L = mk(P.Lx[pid], P.Ly[pid], P.Lz[pid]);
+  Ray ray{mk(P.ox[pid], P.oy[pid], P.oz[pid]), mk(P.dx[pid], P.dy[pid], P.dz[pid]), P.nt[pid],
+          3.402823466e+38f};
+  const bool single = S.gp_mode != MUPSI_GP_RENEWAL_PLUS;
+  bool alive = true, hasHit = false, exited = true;
+
+  if (medium && !S.skip_medium && P.mhit[pid]) {
+    // handleVolume (trace.cpp:77-116)
+    const uint32_t iseed = P.mseed[pid];
+    const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
+    F3 gr;
+    field_eval<true>(S, Pp, iseed, cur, &gr);
+    const F3 nrm = normalized(gr);
+    const F3 wo = neg(ray.d);
+    const F3 alb = mk(0.8f, 0.8f, 0.8f);
+    LightSample ls;
+    if (choose_light(S, Pp, rng, ls)) {
+      Ray sr = make_ray(madd(nrm, S.eps, Pp), ls.d);
+      const float bias = ls.dist - S.eps * fabsf(dot(ls.d, nrm));
+      sr.farT = bias * 0.99f;
+      const float trans = march_serial(S, sr, iseed, cur) ? 0.0f : 1.0f;
+      const F3 f = bsdf_weight(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, ls.d);
+      const float ac = fabsf(dot(ls.d, nrm));
+      L = mk(fmaf(trans * (T.x * (f.x * ls.w.x)), ac, L.x),
+             fmaf(trans * (T.y * (f.y * ls.w.y)), ac, L.y),
+             fmaf(trans * (T.z * (f.z * ls.w.z)), ac, L.z));
+    }
+    F3 wi, wt;
+    bsdf_sample(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, rng, wi, wt);
+    T = cmul(T, wt);
+    const float surv = max3(T);
+    if (rng.next1D() > surv) {
+      alive = false;
+    } else {
+      T = divs(T, surv);
+      if (!single) {
+        const uint32_t ns =
+            constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u, (uint32_t)(bounce + 1)));
+        double u, g[3];
+        renewal_solve(S, Pp, nrm, ns, u, g);
+        cur.active = true;
+        cur.C = Pp;
+        cur.u = (float)u;
+        cur.g = mk((float)g[0], (float)g[1], (float)g[2]);
+      } else {
+        cur.active = false;
+      }
+      ray = make_ray(madd(nrm, S.eps, Pp), wi);
+      ray.nearT = S.near_eps;
+      ++bounce;
+      exited = false;
+      hasHit = true;
+    }
+  }
+
+  if (alive && exited) {
+    F3 Pp, Ng;
+    float uu = 0.f, vv = 0.f;
+    const int hp = scene_intersect(S, ray, Pp, Ng, uu, vv);
+    if (hp >= 0) {
+      // handleSurface (trace.cpp:32-75)
+      const DevPrim& pr = S.prims[hp];
+      const F3 wo = neg(ray.d), nrm = Ng;
+      F3 alb = mk(1, 1, 1);
+      if (pr.bsdf == MUPSI_BSDF_LAMBERTIAN)
+        alb = pr.albedo.rgb ? tex_at(pr.albedo, uu, vv, nullptr) : mk(0.8f, 0.8f, 0.8f);
+      LightSample ls;
+      if (choose_light(S, Pp, rng, ls)) {
+        Ray sr = make_ray(madd(nrm, S.eps, Pp), ls.d);
+        const float bias = ls.dist - S.eps * fabsf(dot(ls.d, nrm));
+        sr.farT = bias * 0.99f;
+        if (!scene_occluded(S, sr)) {
+          const F3 w = bsdf_weight(pr.bsdf, alb, wo, nrm, ls.d);
+          const float ac = fabsf(dot(ls.d, nrm));
+          L = mk(fmaf(T.x * (w.x * ls.w.x), ac, L.x), fmaf(T.y * (w.y * ls.w.y), ac, L.y),
+                 fmaf(T.z * (w.z * ls.w.z), ac, L.z));
+        }
+      }
+      F3 wi, wt;
+      bsdf_sample(pr.bsdf, alb, wo, nrm, rng, wi, wt);
+      if (pr.has_em) {
+        const float mm = fmaxf(0.0f, dot(nrm, wi));
+        L = mk(fmaf(T.x, pr.em[0] * mm, L.x), fmaf(T.y, pr.em[1] * mm, L.y),
+               fmaf(T.z, pr.em[2] * mm, L.z));
+      }
+      T = cmul(T, wt);
+      const float surv = max3(T);
+      if (rng.next1D() > surv) {
+        alive = false;
+      } else {
+        T = divs(T, surv);
+        const bool back = dot(wi, Ng) < 0.0f;
+        medium = back ? (pr.int_medium != 0 && S.has_gp != 0) : false;
+        ray = make_ray(madd(mul(Ng, back ? -1.0f : 1.0f), S.eps, Pp), wi);
+        ray.nearT = S.near_eps;
+        ++bounce;
+        hasHit = true;
+      }
+    } else if (S.sky.rgb) {
+      const F3 e = sky_lookup(S, ray.d, nullptr);
+      L = mk(fmaf(T.x, e.x, L.x), fmaf(T.y, e.y, L.y), fmaf(T.z, e.z, L.z));
+    }
+  }
+
+  const bool cont = alive && hasHit && bounce < S.max_bounce;
+  P.Lx[pid] = L.x;
+  P.Ly[pid] = L.y;
+  P.Lz[pid] = L.z;
+  if (cont) {
+    P.rng[pid] = rng.s;
+    P.Tx[pid] = T.x;
+    P.Ty[pid] = T.y;
+    P.Tz[pid] = T.z;
+    P.ox[pid] = ray.o.x;
+    P.oy[pid] = ray.o.y;
+    P.oz[pid] = ray.o.z;
+    P.dx[pid] = ray.d.x;
+    P.dy[pid] = ray.d.y;
+    P.dz[pid] = ray.d.z;
+    P.nt[pid] = ray.nearT;
+    P.flags[pid] = (uint32_t)bounce | (medium ? kFlagMedium : 0u) | (cur.active ? kFlagCond : 0u);
+    if (cur.active) {
+      P.cx[pid] = cur.C.x;
+      P.cy[pid] = cur.C.y;
+      P.cz[pid] = cur.C.z;
+      P.cu[pid] = cur.u;
+      P.cgx[pid] = cur.g.x;
+      P.cgy[pid] = cur.g.y;
+      P.cgz[pid] = cur.g.z;
+    }
+    if (medium && !S.skip_medium) {
+      P.mseed[pid] = single ? S.single_seed_i
+                            : constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k,
+                                                            42u, (uint32_t)bounce));
+    }
+  }
+  wave_push(cont, pid, a.qA2, a.nA2);
+  wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
+}
+
+// Per-pixel FP64 accumulation over the passes of one launch, in pass order.
+__global__ __launch_bounds__(256) void k_accum(const float* Lx, const float* Ly, const float* Lz,
+                                               int npix, int kcount, double* acc) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= npix) return;
+  double sx = acc[3 * pix], sy = acc[3 * pix + 1], sz = acc[3 * pix + 2];
+  for (int kk = 0; kk < kcount; ++kk) {
+    const int64_t i = (int64_t)kk * npix + pix;
+    sx += (double)Lx[i];
+    sy += (double)Ly[i];
+    sz += (double)Lz[i];
+  }
+  acc[3 * pix] = sx;
+  acc[3 * pix + 1] = sy;
+  acc[3 * pix + 2] = sz;
+}
+
+__global__ void k_set_counters(int* c, int n, int a0, int i0) {
+  const int i = threadIdx.x;
+  if (i < n) c[i] = (i == i0) ? a0 : 0;
+}
+
+// ====================================================================== kernels: batch API
+
+__global__ __launch_bounds__(256) void k_psi(DevGP g, const float* px, const float* py,
+                                             const float* pz, const uint32_t* seed, int64_t n,
+                                             float* out4, float* l1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v, m;
+  F3 gr;
+  psi_eval<true>(g, mk(px[i], py[i], pz[i]), seed[i], v, gr, l1 ? &m : nullptr);
+  out4[4 * i] = v;
+  out4[4 * i + 1] = gr.x;
+  out4[4 * i + 2] = gr.y;
+  out4[4 * i + 3] = gr.z;
+  if (l1) l1[i] = m;
+}
+
+__global__ __launch_bounds__(256) void k_cell_impulses(DevGP g, const int32_t* cell3,
+                                                       const uint32_t* seed, int64_t n,
+                                                       uint32_t* cell_seed, float* pos,
+                                                       int8_t* w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int cx = cell3[3 * i], cy = cell3[3 * i + 1], cz = cell3[3 * i + 2];
+  const uint32_t cs = make_seed4((uint32_t)cx, (uint32_t)cy, (uint32_t)cz, seed[i]);
+  cell_seed[i] = cs;
+  uint64_t s = (uint32_t)(cs + 1u);
+  const float bx = (float)cx * g.kr, by = (float)cy * g.kr, bz = (float)cz * g.kr;
+  for (int j = 0; j < g.ppc; ++j) {
+    const uint32_t r0 = pcg_next(s), r1 = pcg_next(s), r2 = pcg_next(s), r3 = pcg_next(s);
+    const int64_t o = i * g.ppc + j;
+    pos[3 * o] = fmaf(u01(r2), g.kr, bx);
+    pos[3 * o + 1] = fmaf(u01(r1), g.kr, by);
+    pos[3 * o + 2] = fmaf(u01(r0), g.kr, bz);
+    w[o] = (r3 >> 31) ? 1 : -1;
+  }
+}
+
+struct CondSoA {
+  const uint8_t* active;
+  const float *cx, *cy, *cz, *u, *gx, *gy, *gz;
+};
+__device__ __forceinline__ Cond load_cond(const CondSoA& c, int64_t i) {
+  Cond r;
+  r.active = c.active && c.active[i];
+  if (r.active) {
+    r.C = mk(c.cx[i], c.cy[i], c.cz[i]);
+    r.u = c.u[i];
+    r.g = mk(c.gx[i], c.gy[i], c.gz[i]);
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_field(DevScene S, const float* px, const float* py,
+                                               const float* pz, const uint32_t* seed, CondSoA c,
+                                               int64_t n, float* out4) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  F3 g;
+  const float v = field_eval<true>(S, mk(px[i], py[i], pz[i]), seed[i], load_cond(c, i), &g);
+  out4[4 * i] = v;
+  out4[4 * i + 1] = g.x;
+  out4[4 * i + 2] = g.y;
+  out4[4 * i + 3] = g.z;
+}
+
+// Hit normal of sampleDistance: normalize(grad f(p)) (gpmedium.cpp:81-82).
+__global__ __launch_bounds__(256) void k_hit_normals(DevScene S, const uint8_t* hit,
+                                                     const float* p3x, const float* p3y,
+                                                     const float* p3z, const uint32_t* seed,
+                                                     CondSoA c, int64_t n, float* nrm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  F3 g = mk(0, 0, 0);
+  if (hit[i]) {
+    field_eval<true>(S, mk(p3x[i], p3y[i], p3z[i]), seed[i], load_cond(c, i), &g);
+    g = normalized(g);
+  }
+  nrm[3 * i] = g.x;
+  nrm[3 * i + 1] = g.y;
+  nrm[3 * i + 2] = g.z;
+}
+
+__global__ void k_interleave3(const float* x, const float* y, const float* z, int64_t n,
+                              float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[3 * i] = x[i];
+  out[3 * i + 1] = y[i];
+  out[3 * i + 2] = z[i];
+}
+
+__global__ __launch_bounds__(256) void k_condition(DevScene S, const float* cx, const float* cy,
+                                                   const float* cz, const float* nx,
+                                                   const float* ny, const float* nz,
+                                                   const uint32_t* seed_next, int64_t n,
+                                                   double* u, double* g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double uu, gg[3];
+  renewal_solve(S, mk(cx[i], cy[i], cz[i]), mk(nx[i], ny[i], nz[i]), seed_next[i], uu, gg);
+  u[i] = uu;
+  g[3 * i] = gg[0];
+  g[3 * i + 1] = gg[1];
+  g[3 * i + 2] = gg[2];
+}
+
+// Ray(o, d) construction for the batch march: copy o, normalize d.
+__global__ void k_normalize_rays(const float* ox, const float* oy, const float* oz,
+                                 const float* dx, const float* dy, const float* dz, int64_t n,
+                                 float* a, float* b, float* c, float* e, float* f, float* g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const F3 d = normalized(mk(dx[i], dy[i], dz[i]));
+  a[i] = ox[i];
+  b[i] = oy[i];
+  c[i] = oz[i];
+  e[i] = d.x;
+  f[i] = d.y;
+  g[i] = d.z;
+}
+
+struct RaySoA {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *nt, *ft;
+};
+__device__ __forceinline__ Ray load_ray(const RaySoA& r, int64_t i) {
+  Ray ray = make_ray(mk(r.ox[i], r.oy[i], r.oz[i]), mk(r.dx[i], r.dy[i], r.dz[i]));
+  ray.nearT = r.nt[i];
+  ray.farT = r.ft[i];
+  return ray;
+}
+
+__global__ __launch_bounds__(256) void k_mesh_intersect(DevMesh m, RaySoA rs, int64_t n,
+                                                        int32_t* tri, float* t, float* u,
+                                                        float* v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Ray r = load_ray(rs, i);
+  uint32_t ti = 0;
+  float uu = 0.f, vv = 0.f;
+  const bool h = mesh_intersect(m, r, ti, uu, vv);
+  tri[i] = h ? (int32_t)ti : -1;
+  t[i] = h ? r.farT : 0.0f;
+  u[i] = uu;
+  v[i] = vv;
+}
+__global__ __launch_bounds__(256) void k_mesh_occluded(DevMesh m, RaySoA rs, int64_t n,
+                                                       uint8_t* occ) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  occ[i] = mesh_occluded(m, load_ray(rs, i)) ? 1 : 0;
+}
+__global__ __launch_bounds__(256) void k_mesh_closest(DevMesh m, const float* px, const float* py,
+                                                      const float* pz, int64_t n, float* dist,
+                                                      int32_t* tri, float* cl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t ti = 0;
+  F3 o = mk(0, 0, 0);
+  dist[i] = mesh_dist(m, mk(px[i], py[i], pz[i]), ti, o);
+  tri[i] = (int32_t)ti;
+  cl[3 * i] = o.x;
+  cl[3 * i + 1] = o.y;
+  cl[3 * i + 2] = o.z;
+}
+__global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px, const float* py,
+                                                     const float* pz, int64_t n, uint8_t* in) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
+}
+
+// ====================================================================== host: errors
+
+
+static mupsi_status fail(mupsi_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? MUPSI_ERR_NOMEM : MUPSI_ERR_HIP,               \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+static inline unsigned grid_for(int64_t n, int block = 256) {
+  return (unsigned)std::max<int64_t>(1, (n + block - 1) / block);
+}
+
+// ====================================================================== host: context
+
+struct mupsi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string name;
+  int cus = 0;
+  int march_blocks = 0;  // persistent grid of k_march
+};
+
+extern "C" const char* mupsi_status_string(mupsi_status s) {
+  switch (s) {
+    case MUPSI_OK: return "ok";
+    case MUPSI_ERR_INVALID: return "invalid argument";
+    case MUPSI_ERR_HIP: return "HIP runtime error";
+    case MUPSI_ERR_NOMEM: return "out of memory";
+    case MUPSI_ERR_ABORTED: return "aborted";
+    case MUPSI_ERR_UNSUPPORTED: return "unsupported";
+    case MUPSI_ERR_IO: return "I/O error";
+    case MUPSI_ERR_PARSE: return "parse error";
+    default: return "unknown status";
+  }
+}
+extern "C" const char* mupsi_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
+  if (!out) return fail(MUPSI_ERR_INVALID, "mupsi_open: out is null");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(MUPSI_ERR_HIP, "mupsi_open: no HIP device");
+  if (device < 0 || device >= count) return fail(MUPSI_ERR_INVALID, "mupsi_open: bad device index");
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(MUPSI_ERR_HIP, std::string("mupsi_open: built for gfx950, device is ") + prop.gcnArchName);
+  auto ctx = std::make_unique<mupsi_ctx>();
+  ctx->device = device;
+  ctx->name = prop.name;
+  ctx->cus = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  int per_cu = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 256, 0));
+  ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
+  *out = ctx.release();
+  return MUPSI_OK;
+}
+
+extern "C" void mupsi_close(mupsi_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" mupsi_status mupsi_device_info(mupsi_ctx* ctx, char* name, int32_t name_len,
+                                          int32_t* cu_count) {
+  if (!ctx) return fail(MUPSI_ERR_INVALID, "mupsi_device_info: null ctx");
+  if (name && name_len > 0) {
+    std::strncpy(name, ctx->name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cu_count) *cu_count = ctx->cus;
+  return MUPSI_OK;
+}
+
+// ====================================================================== host: scene
+
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  ~DeviceBuffers() {
+    for (void* p : ptrs) hipFree(p);
+  }
+  template <typename T>
+  hipError_t upload(const T* src, size_t count, const T** dst) {
+    *dst = nullptr;
+    if (count == 0 || !src) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) return e;
+    ptrs.push_back(p);
+    e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    *dst = static_cast<const T*>(p);
+    return e;
+  }
+  template <typename T>
+  hipError_t alloc(size_t count, T** dst) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
+    if (e != hipSuccess) return e;
+    ptrs.push_back(p);
+    *dst = static_cast<T*>(p);
+    return hipSuccess;
+  }
+};
+
+struct PathBuffers {
+  DeviceBuffers mem;
+  int64_t capacity = 0;
+  PathSoA P{};
+  int* qA[2]{};
+  int* qM[2]{};
+  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work]
+  unsigned long long* stats = nullptr;
+};
+
+struct mupsi_scene {
+  mupsi_ctx* ctx = nullptr;
+  DevScene ds{};
+  DeviceBuffers mem;
+  HostScene host;  // host-side lowered description (meshes keep the BVH for the batch API)
+  std::vector<DevMesh> meshes;
+  DevMesh mean_mesh{};
+  double* accum = nullptr;
+  int64_t sample_count = 0;
+  std::unique_ptr<PathBuffers> pb;
+  mupsi_stats last{};
+};
+
+static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh* out) {
+  DevMesh m{};
+  const BvhNode* nodes;
+  HIP_TRY(mem.upload(hm.bvh.nodes.data(), hm.bvh.nodes.size(), &nodes));
+  m.nodes = nodes;
+  HIP_TRY(mem.upload(hm.order.data(), hm.order.size(), &m.order));
+  HIP_TRY(mem.upload(hm.leaf_tris.data(), hm.leaf_tris.size(), &m.leaf_tris));
+  HIP_TRY(mem.upload(hm.vnormals.data(), hm.vnormals.size(), &m.vnormals));
+  HIP_TRY(mem.upload(hm.faces.data(), hm.faces.size(), &m.faces));
+  HIP_TRY(mem.upload(hm.fnormals.data(), hm.fnormals.size(), &m.fnormals));
+  HIP_TRY(mem.upload(hm.texcoords.data(), hm.texcoords.size(), &m.texcoords));
+  HIP_TRY(mem.upload(hm.face_tex.data(), hm.face_tex.size(), &m.face_tex));
+  m.n_nodes = (int)hm.bvh.nodes.size();
+  m.n_faces = (int)(hm.faces.size() / 3);
+  m.root = hm.bvh.root;
+  m.xf = hm.xf;
+  for (int a = 0; a < 3; ++a) {
+    m.lbmin[a] = hm.bvh.nodes.empty() ? 0.f : hm.bvh.nodes[hm.bvh.root].bmin[a];
+    m.lbmax[a] = hm.bvh.nodes.empty() ? 0.f : hm.bvh.nodes[hm.bvh.root].bmax[a];
+  }
+  *out = m;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_desc* desc,
+                                           mupsi_scene** out) {
+  if (!ctx || !desc || !out) return fail(MUPSI_ERR_INVALID, "mupsi_scene_upload: null argument");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto sc = std::make_unique<mupsi_scene>();
+  sc->ctx = ctx;
+  std::string err;
+  mupsi_status st = lower_scene(*desc, &sc->host, &err);
+  if (st != MUPSI_OK) return fail(st, err);
+  const HostScene& H = sc->host;
+  DevScene& D = sc->ds;
+  D = H.ds;  // constants computed on the host
+
+  for (const HostMesh& hm : H.meshes) {
+    DevMesh m;
+    st = upload_mesh(sc->mem, hm, &m);
+    if (st != MUPSI_OK) return st;
+    sc->meshes.push_back(m);
+  }
+  if (H.has_mean_mesh) {
+    st = upload_mesh(sc->mem, H.mean_mesh, &sc->mean_mesh);
+    if (st != MUPSI_OK) return st;
+  }
+  D.mean_mesh = sc->mean_mesh;
+  const DevMesh* dmeshes;
+  HIP_TRY(sc->mem.upload(sc->meshes.data(), sc->meshes.size(), &dmeshes));
+  D.meshes = dmeshes;
+
+  std::vector<DevPrim> prims = H.prims;
+  for (size_t i = 0; i < prims.size(); ++i) {
+    const std::vector<float>& tex = H.prim_albedo[i];
+    const float* dtex = nullptr;
+    HIP_TRY(sc->mem.upload(tex.data(), tex.size(), &dtex));
+    prims[i].albedo.rgb = dtex;
+  }
+  const DevPrim* dprims;
+  HIP_TRY(sc->mem.upload(prims.data(), prims.size(), &dprims));
+  D.prims = dprims;
+  const float* dsky = nullptr;
+  HIP_TRY(sc->mem.upload(H.sky.data(), H.sky.size(), &dsky));
+  D.sky.rgb = dsky;
+
+  const size_t npix = (size_t)D.w * D.h;
+  HIP_TRY(sc->mem.alloc(npix * 3, &sc->accum));
+  HIP_TRY(hipMemset(sc->accum, 0, npix * 3 * sizeof(double)));
+  *out = sc.release();
+  return MUPSI_OK;
+}
+
+extern "C" void mupsi_scene_free(mupsi_scene* scene) {
+  if (!scene) return;
+  hipSetDevice(scene->ctx->device);
+  hipStreamSynchronize(scene->ctx->stream);
+  delete scene;
+}
+
+extern "C" mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int32_t* height) {
+  if (!scene) return fail(MUPSI_ERR_INVALID, "mupsi_scene_size: null scene");
+  if (width) *width = scene->ds.w;
+  if (height) *height = scene->ds.h;
+  return MUPSI_OK;
+}
+
+static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
+  if (sc->pb && sc->pb->capacity >= n) return MUPSI_OK;
+  sc->pb.reset();
+  auto pb = std::make_unique<PathBuffers>();
+  DeviceBuffers& m = pb->mem;
+  PathSoA& P = pb->P;
+  HIP_TRY(m.alloc(n, &P.rng));
+  float** f32[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.nt, &P.Tx, &P.Ty, &P.Tz,
+                   &P.Lx, &P.Ly, &P.Lz, &P.cx, &P.cy, &P.cz, &P.cu, &P.cgx, &P.cgy, &P.cgz,
+                   &P.mt, &P.mpx, &P.mpy, &P.mpz};
+  for (float** p : f32) HIP_TRY(m.alloc(n, p));
+  HIP_TRY(m.alloc(n, &P.flags));
+  HIP_TRY(m.alloc(n, &P.mseed));
+  HIP_TRY(m.alloc(n, &P.mhit));
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(m.alloc(n, &pb->qA[i]));
+    HIP_TRY(m.alloc(n, &pb->qM[i]));
+  }
+  HIP_TRY(m.alloc(8, &pb->counters));
+  HIP_TRY(m.alloc(8, &pb->stats));
+  pb->capacity = n;
+  sc->pb = std::move(pb);
+  return MUPSI_OK;
+}
+
+// Run the wavefront over n paths described by src; leaves per-path radiance in P.L*.
+static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
+                                  const volatile int32_t* abort_flag, double* march_ms,
+                                  int* launches) {
+  mupsi_ctx* ctx = sc->ctx;
+  hipStream_t st = ctx->stream;
+  PathBuffers& pb = *sc->pb;
+  const DevScene& S = sc->ds;
+  const int ni = (int)n;
+  hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, pb.counters, 8, ni, 0);
+  InitArgs ia{pb.P, src, ni, pb.qA[0]};
+  hipLaunchKernelGGL(k_init, dim3(grid_for(n)), dim3(256), 0, st, S, ia);
+  HIP_TRY(hipGetLastError());
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+  mupsi_status status = MUPSI_OK;
+  const int iters = std::max(1, S.max_bounce);
+  for (int it = 0; it < iters; ++it) {
+    if (abort_flag && *abort_flag) {
+      status = MUPSI_ERR_ABORTED;
+      break;
+    }
+    const int cur = it & 1, nxt = cur ^ 1;
+    int* nA = pb.counters + 2 * cur;
+    int* nM = pb.counters + 2 * cur + 1;
+    int* nA2 = pb.counters + 2 * nxt;
+    int* nM2 = pb.counters + 2 * nxt + 1;
+    int* work = pb.counters + 4;
+    if (it > 0) {
+      // zero the work counter
+      hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work, 1, 0, -1);
+      MarchArgs ma{};
+      ma.queue = pb.qM[cur];
+      ma.count_dev = nM;
+      ma.ox = pb.P.ox; ma.oy = pb.P.oy; ma.oz = pb.P.oz;
+      ma.dx = pb.P.dx; ma.dy = pb.P.dy; ma.dz = pb.P.dz;
+      ma.nt = pb.P.nt;
+      ma.ft = nullptr;
+      ma.seed = pb.P.mseed;
+      ma.flags = pb.P.flags;
+      ma.cx = pb.P.cx; ma.cy = pb.P.cy; ma.cz = pb.P.cz; ma.cu = pb.P.cu;
+      ma.cgx = pb.P.cgx; ma.cgy = pb.P.cgy; ma.cgz = pb.P.cgz;
+      ma.hit = pb.P.mhit;
+      ma.t = pb.P.mt; ma.px = pb.P.mpx; ma.py = pb.P.mpy; ma.pz = pb.P.mpz;
+      ma.steps = nullptr;
+      ma.work = work;
+      ma.stats = pb.stats;
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      evs.emplace_back(e0, e1);
+      HIP_TRY(hipEventRecord(e0, st));
+      hipLaunchKernelGGL(k_march, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
+      HIP_TRY(hipEventRecord(e1, st));
+      HIP_TRY(hipGetLastError());
+    }
+    // zero next queue counters
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nA2, 2, 0, -1);
+    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2};
+    hipLaunchKernelGGL(k_shade, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  double ms = 0;
+  for (auto& e : evs) {
+    float t = 0;
+    hipEventElapsedTime(&t, e.first, e.second);
+    ms += t;
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  if (march_ms) *march_ms += ms;
+  if (launches) *launches += (int)evs.size();
+  return status;
+}
+
+extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, int32_t k_count,
+                                            int32_t passes_per_launch,
+                                            const volatile int32_t* abort_flag) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_render_passes: null scene");
+  if (k_begin < 0 || k_count < 0) return fail(MUPSI_ERR_INVALID, "mupsi_render_passes: negative pass range");
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  const int64_t npix = (int64_t)sc->ds.w * sc->ds.h;
+  int ppl = passes_per_launch;
+  if (ppl <= 0) ppl = (int)std::max<int64_t>(1, std::min<int64_t>(k_count, (int64_t)(8 << 20) / npix));
+  ppl = std::max(1, std::min(ppl, std::max(1, k_count)));
+  if ((int64_t)ppl * npix > INT32_MAX) ppl = (int)std::max<int64_t>(1, INT32_MAX / npix);
+  mupsi_status st = ensure_paths(sc, (int64_t)ppl * npix);
+  if (st != MUPSI_OK) return st;
+  PathBuffers& pb = *sc->pb;
+  HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), sc->ctx->stream));
+  mupsi_stats stats{};
+  double march_ms = 0;
+  int launches = 0;
+  for (int kb = k_begin; kb < k_begin + k_count; kb += ppl) {
+    if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
+    const int kc = std::min(ppl, k_begin + k_count - kb);
+    PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr};
+    st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches);
+    if (st == MUPSI_ERR_ABORTED) return fail(st, "render aborted");
+    if (st != MUPSI_OK) return st;
+    hipLaunchKernelGGL(k_accum, dim3(grid_for(npix)), dim3(256), 0, sc->ctx->stream, pb.P.Lx,
+                       pb.P.Ly, pb.P.Lz, (int)npix, kc, sc->accum);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+    sc->sample_count += kc;
+    stats.paths += (int64_t)kc * npix;
+  }
+  unsigned long long hs[8];
+  HIP_TRY(hipMemcpy(hs, pb.stats, sizeof(hs), hipMemcpyDeviceToHost));
+  stats.field_evals = (int64_t)hs[0];
+  stats.march_steps = (int64_t)hs[1];
+  stats.march_calls = (int64_t)hs[2];
+  stats.impulse_evals = stats.field_evals * 27 * sc->ds.gp.ppc;
+  stats.march_kernel_launches = launches;
+  stats.march_kernel_ms = march_ms;
+  stats.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  sc->last = stats;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_last_stats(mupsi_scene* sc, mupsi_stats* out) {
+  if (!sc || !out) return fail(MUPSI_ERR_INVALID, "mupsi_last_stats: null argument");
+  *out = sc->last;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_reset_accum(mupsi_scene* sc) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_reset_accum: null scene");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  HIP_TRY(hipMemset(sc->accum, 0, (size_t)sc->ds.w * sc->ds.h * 3 * sizeof(double)));
+  sc->sample_count = 0;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_read_accum(mupsi_scene* sc, double* rgb_sum, int64_t* count) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_read_accum: null scene");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  if (rgb_sum)
+    HIP_TRY(hipMemcpy(rgb_sum, sc->accum, (size_t)sc->ds.w * sc->ds.h * 3 * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  if (count) *count = sc->sample_count;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_write_accum(mupsi_scene* sc, const double* rgb_sum, int64_t count) {
+  if (!sc || !rgb_sum || count < 0) return fail(MUPSI_ERR_INVALID, "mupsi_write_accum: bad argument");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  HIP_TRY(hipMemcpy(sc->accum, rgb_sum, (size_t)sc->ds.w * sc->ds.h * 3 * sizeof(double),
+                    hipMemcpyHostToDevice));
+  sc->sample_count = count;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_trace_paths(mupsi_scene* sc, const int32_t* x, const int32_t* y,
+                                          const int32_t* k, int64_t n, float* rgb_out) {
+  if (!sc || (n > 0 && (!x || !y || !k || !rgb_out)))
+    return fail(MUPSI_ERR_INVALID, "mupsi_trace_paths: null argument");
+  if (n == 0) return MUPSI_OK;
+  if (n > INT32_MAX) return fail(MUPSI_ERR_INVALID, "mupsi_trace_paths: too many paths");
+  for (int64_t i = 0; i < n; ++i)
+    if (x[i] < 0 || y[i] < 0 || k[i] < 0 || x[i] >= sc->ds.w || y[i] >= sc->ds.h)
+      return fail(MUPSI_ERR_INVALID, "mupsi_trace_paths: pixel outside the image");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  mupsi_status st = ensure_paths(sc, n);
+  if (st != MUPSI_OK) return st;
+  DeviceBuffers tmp;
+  const int32_t *dx, *dy, *dk;
+  HIP_TRY(tmp.upload(x, n, &dx));
+  HIP_TRY(tmp.upload(y, n, &dy));
+  HIP_TRY(tmp.upload(k, n, &dk));
+  PathBuffers& pb = *sc->pb;
+  HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), sc->ctx->stream));
+  PathSrc src{sc->ds.w, sc->ds.w * sc->ds.h, 0, dx, dy, dk};
+  st = run_wavefront(sc, src, n, nullptr, nullptr, nullptr);
+  if (st != MUPSI_OK) return st;
+  float* drgb;
+  HIP_TRY(tmp.alloc(3 * n, &drgb));
+  hipLaunchKernelGGL(k_interleave3, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, pb.P.Lx,
+                     pb.P.Ly, pb.P.Lz, n, drgb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  HIP_TRY(hipMemcpy(rgb_out, drgb, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+  return MUPSI_OK;
+}
+
+// ====================================================================== host: batch API
+
+static DevGP gp_from_params(const mupsi_gp_params& p) { return make_dev_gp(p); }
+
+extern "C" mupsi_status mupsi_psi_eval(mupsi_ctx* ctx, const mupsi_gp_params* gp, const float* px,
+                                       const float* py, const float* pz, const uint32_t* seed,
+                                       int64_t n, float* out4, float* l1) {
+  if (!ctx || !gp) return fail(MUPSI_ERR_INVALID, "mupsi_psi_eval: null argument");
+  std::string err;
+  if (!validate_gp(*gp, &err)) return fail(MUPSI_ERR_INVALID, err);
+  if (n <= 0) return MUPSI_OK;
+  if (!px || !py || !pz || !seed || !out4) return fail(MUPSI_ERR_INVALID, "mupsi_psi_eval: null array");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_psi, dim3(grid_for(n)), dim3(256), 0, ctx->stream, gp_from_params(*gp), px,
+                     py, pz, seed, n, out4, l1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_cell_impulses(mupsi_ctx* ctx, const mupsi_gp_params* gp,
+                                            const int32_t* cell3, const uint32_t* seed, int64_t n,
+                                            uint32_t* cell_seed, float* pos, int8_t* w) {
+  if (!ctx || !gp) return fail(MUPSI_ERR_INVALID, "mupsi_cell_impulses: null argument");
+  std::string err;
+  if (!validate_gp(*gp, &err)) return fail(MUPSI_ERR_INVALID, err);
+  if (n <= 0) return MUPSI_OK;
+  if (!cell3 || !seed || !cell_seed || !pos || !w)
+    return fail(MUPSI_ERR_INVALID, "mupsi_cell_impulses: null array");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_cell_impulses, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                     gp_from_params(*gp), cell3, seed, n, cell_seed, pos, w);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MUPSI_OK;
+}
+
+static CondSoA cond_soa(const mupsi_cond_soa* c) {
+  CondSoA r{};
+  if (c) r = CondSoA{c->active, c->cx, c->cy, c->cz, c->u_tilde, c->gx, c->gy, c->gz};
+  return r;
+}
+static bool gp_ready(mupsi_scene* sc) { return sc && sc->ds.has_gp; }
+
+extern "C" mupsi_status mupsi_field_eval(mupsi_scene* sc, const float* px, const float* py,
+                                         const float* pz, const uint32_t* seed,
+                                         const mupsi_cond_soa* cond, int64_t n, float* out4) {
+  if (!gp_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_field_eval: scene has no GP medium");
+  if (n <= 0) return MUPSI_OK;
+  if (!px || !py || !pz || !seed || !out4) return fail(MUPSI_ERR_INVALID, "mupsi_field_eval: null array");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipLaunchKernelGGL(k_field, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, sc->ds, px, py, pz,
+                     seed, cond_soa(cond), n, out4);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* ox,
+                                    const float* oy, const float* oz, const float* dx,
+                                    const float* dy, const float* dz, const float* near_t,
+                                    const float* far_t, const uint32_t* seed,
+                                    const mupsi_cond_soa* cond, int64_t n, uint8_t* hit, float* t,
+                                    float* p, float* nrm, int32_t* steps) {
+  if (!gp_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_march: scene has no GP medium");
+  if (mode != 0 && mode != 1) return fail(MUPSI_ERR_INVALID, "mupsi_march: mode must be 0 or 1");
+  if (n <= 0) return MUPSI_OK;
+  if (n > INT32_MAX) return fail(MUPSI_ERR_INVALID, "mupsi_march: too many rays");
+  if (!ox || !oy || !oz || !dx || !dy || !dz || !near_t || !far_t || !seed || !hit || !t || !p)
+    return fail(MUPSI_ERR_INVALID, "mupsi_march: null array");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipStream_t st = sc->ctx->stream;
+  DeviceBuffers tmp;
+  int* work;
+  unsigned long long* stats;
+  float *dox, *doy, *doz, *ddx, *ddy, *ddz, *px, *py, *pz;
+  HIP_TRY(tmp.alloc(1, &work));
+  HIP_TRY(tmp.alloc(4, &stats));
+  for (float** q : {&dox, &doy, &doz, &ddx, &ddy, &ddz, &px, &py, &pz}) HIP_TRY(tmp.alloc(n, q));
+  HIP_TRY(hipMemsetAsync(work, 0, sizeof(int), st));
+  HIP_TRY(hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), st));
+  // Ray(o, d) normalizes the direction; the batch copies o and normalized d.
+  hipLaunchKernelGGL(k_normalize_rays, dim3(grid_for(n)), dim3(256), 0, st, ox, oy, oz, dx, dy, dz, n, dox,
+                     doy, doz, ddx, ddy, ddz);
+  CondSoA cs = cond_soa(cond);
+  MarchArgs ma{};
+  ma.queue = nullptr;
+  ma.count_dev = nullptr;
+  ma.count_host = (int)n;
+  ma.ox = dox; ma.oy = doy; ma.oz = doz; ma.dx = ddx; ma.dy = ddy; ma.dz = ddz;
+  ma.nt = near_t;
+  ma.ft = far_t;
+  ma.seed = seed;
+  ma.flags = nullptr;
+  ma.active = cs.active;
+  ma.cx = cs.cx; ma.cy = cs.cy; ma.cz = cs.cz; ma.cu = cs.u; ma.cgx = cs.gx; ma.cgy = cs.gy; ma.cgz = cs.gz;
+  ma.hit = hit;
+  ma.t = t;
+  ma.px = px; ma.py = py; ma.pz = pz;
+  ma.steps = steps;
+  ma.work = work;
+  ma.stats = stats;
+  hipLaunchKernelGGL(k_march, dim3(sc->ctx->march_blocks), dim3(256), 0, st, sc->ds, ma);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_interleave3, dim3(grid_for(n)), dim3(256), 0, st, px, py, pz, n, p);
+  if (mode == 0 && nrm)
+    hipLaunchKernelGGL(k_hit_normals, dim3(grid_for(n)), dim3(256), 0, st, sc->ds, hit, px, py, pz,
+                       seed, cs, n, nrm);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_condition(mupsi_scene* sc, const float* cx, const float* cy,
+                                        const float* cz, const float* nx, const float* ny,
+                                        const float* nz, const uint32_t* seed_next, int64_t n,
+                                        double* u, double* g) {
+  if (!gp_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_condition: scene has no GP medium");
+  if (n <= 0) return MUPSI_OK;
+  if (!cx || !cy || !cz || !nx || !ny || !nz || !seed_next || !u || !g)
+    return fail(MUPSI_ERR_INVALID, "mupsi_condition: null array");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipLaunchKernelGGL(k_condition, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, sc->ds, cx, cy,
+                     cz, nx, ny, nz, seed_next, n, u, g);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+static const DevMesh* pick_mesh(mupsi_scene* sc, int32_t prim) {
+  if (!sc) return nullptr;
+  if (prim == -1) return sc->host.has_mean_mesh ? &sc->mean_mesh : nullptr;
+  if (prim < 0 || prim >= (int)sc->host.prims.size()) return nullptr;
+  const DevPrim& pr = sc->host.prims[prim];
+  if (pr.type != MUPSI_PRIM_MESH) return nullptr;
+  return &sc->meshes[pr.mesh];
+}
+
+extern "C" mupsi_status mupsi_mesh_intersect(mupsi_scene* sc, int32_t prim, const float* ox,
+                                             const float* oy, const float* oz, const float* dx,
+                                             const float* dy, const float* dz,
+                                             const float* near_t, const float* far_t, int64_t n,
+                                             int32_t* tri, float* t, float* u, float* v) {
+  const DevMesh* m = pick_mesh(sc, prim);
+  if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_intersect: no such mesh");
+  if (n <= 0) return MUPSI_OK;
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
+  hipLaunchKernelGGL(k_mesh_intersect, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, rs, n,
+                     tri, t, u, v);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_mesh_occluded(mupsi_scene* sc, int32_t prim, const float* ox,
+                                            const float* oy, const float* oz, const float* dx,
+                                            const float* dy, const float* dz, const float* near_t,
+                                            const float* far_t, int64_t n, uint8_t* occ) {
+  const DevMesh* m = pick_mesh(sc, prim);
+  if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_occluded: no such mesh");
+  if (n <= 0) return MUPSI_OK;
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
+  hipLaunchKernelGGL(k_mesh_occluded, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, rs, n,
+                     occ);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_mesh_closest(mupsi_scene* sc, int32_t prim, const float* px,
+                                           const float* py, const float* pz, int64_t n,
+                                           float* dist, int32_t* tri, float* closest3) {
+  const DevMesh* m = pick_mesh(sc, prim);
+  if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_closest: no such mesh");
+  if (n <= 0) return MUPSI_OK;
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipLaunchKernelGGL(k_mesh_closest, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, px, py,
+                     pz, n, dist, tri, closest3);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_mesh_inside(mupsi_scene* sc, int32_t prim, const float* px,
+                                          const float* py, const float* pz, int64_t n,
+                                          uint8_t* inside) {
+  const DevMesh* m = pick_mesh(sc, prim);
+  if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_inside: no such mesh");
+  if (n <= 0) return MUPSI_OK;
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, px, py,
+                     pz, n, inside);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices,
+                                        const int32_t* faces, int32_t n_faces, float* nodes_out,
+                                        int32_t* n_nodes_out, uint32_t* tri_index_out,
+                                        uint32_t* root_out) {
+  if (!vertices || !faces || n_faces < 0 || n_vertices < 0)
+    return fail(MUPSI_ERR_INVALID, "mupsi_bvh_build: bad argument");
+  for (int64_t i = 0; i < 3LL * n_faces; ++i)
+    if (faces[i] < 0 || faces[i] >= n_vertices)
+      return fail(MUPSI_ERR_INVALID, "mupsi_bvh_build: face index out of range");
+  BvhResult b = build_bvh(vertices, faces, (uint32_t)n_faces);
+  if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
+  if (n_nodes_out) *n_nodes_out = (int32_t)b.nodes.size();
+  if (tri_index_out) std::memcpy(tri_index_out, b.tri_index.data(), b.tri_index.size() * 4);
+  if (root_out) *root_out = b.root;
+  return MUPSI_OK;
+}

@@ -1,0 +1,165 @@
+"""Procedural inputs of the benchmark configurations (seeded, bit-reproducible).
+
+- `env_map`: the 64x32 FP32 environment map: a sky gradient plus one Gaussian
+  sun lobe of peak 50.
+- `icosphere`: a radius-1 icosphere subdivided n times, optionally displaced
+  along the vertex normals by seeded 3-octave value noise.
+- `config_scene(i)`: the scene dicts of configurations 1-3 (sphere mean,
+  single realization / renewal_plus; displaced-icosphere mesh mean).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ENV_SEED = 20240611
+MESH_SEED = 7
+
+
+def env_map(width: int = 64, height: int = 32, seed: int = ENV_SEED) -> np.ndarray:
+    """Sky gradient + Gaussian sun lobe of peak 50; (H, W, 3) float32, row 0 = up (+y)."""
+    rng = np.random.default_rng(seed)
+    sun_theta = rng.uniform(0.35, 0.6)     # angle from the zenith
+    sun_phi = rng.uniform(0.0, 2 * np.pi)
+    sun = np.array([np.sin(sun_theta) * np.cos(sun_phi), np.cos(sun_theta),
+                    np.sin(sun_theta) * np.sin(sun_phi)])
+    # texel centres -> directions with the skydrome mapping inverted:
+    # u = (atan2(z, x) + pi) / 2pi, v_img = 1 - acos(-y)/pi, row = v_img * h
+    u = (np.arange(width) + 0.5) / width
+    v_img = (np.arange(height) + 0.5) / height
+    phi = u * 2 * np.pi - np.pi
+    theta = np.pi * (1.0 - v_img)          # acos(-y)
+    y = -np.cos(theta)[:, None] * np.ones((1, width))
+    s = np.sin(theta)[:, None]
+    x = s * np.cos(phi)[None, :]
+    z = s * np.sin(phi)[None, :]
+    up = np.clip(y, -1, 1)
+    zenith = np.array([0.25, 0.45, 0.95])
+    horizon = np.array([0.9, 0.9, 0.85])
+    ground = np.array([0.18, 0.16, 0.14])
+    t = np.clip(up, 0, 1)[..., None] ** 0.5
+    sky = horizon * (1 - t) + zenith * t
+    sky = np.where(up[..., None] >= 0, sky, ground * (0.6 + 0.4 * (1 + up[..., None])))
+    cosang = np.clip(x * sun[0] + y * sun[1] + z * sun[2], -1, 1)
+    ang = np.arccos(cosang)
+    lobe = 50.0 * np.exp(-0.5 * (ang / 0.06) ** 2)
+    img = sky + lobe[..., None] * np.array([1.0, 0.95, 0.85])
+    return np.ascontiguousarray(img.astype(np.float32))
+
+
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                 dtype=np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9],
+                  [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2],
+                  [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+                  [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    return v / np.linalg.norm(v, axis=1, keepdims=True), f
+
+
+def _subdivide(v: np.ndarray, f: np.ndarray):
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    e.sort(axis=1)
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+    mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+    base = v.shape[0]
+    nf = f.shape[0]
+    a = base + inv[:nf]
+    b = base + inv[nf:2 * nf]
+    c = base + inv[2 * nf:]
+    f0, f1, f2 = f[:, 0], f[:, 1], f[:, 2]
+    nfaces = np.concatenate([np.stack([f0, a, c], 1), np.stack([f1, b, a], 1),
+                             np.stack([f2, c, b], 1), np.stack([a, b, c], 1)])
+    return np.concatenate([v, mid]), nfaces
+
+
+def _lattice(ix, iy, iz, seed):
+    h = (ix.astype(np.uint64) * np.uint64(73856093)) ^ (iy.astype(np.uint64) * np.uint64(19349663)) \
+        ^ (iz.astype(np.uint64) * np.uint64(83492791)) ^ np.uint64(seed * 2654435761 % (1 << 32))
+    h = (h ^ (h >> np.uint64(13))) * np.uint64(0x5bd1e995)
+    h = h ^ (h >> np.uint64(15))
+    return (h & np.uint64(0xFFFFFF)).astype(np.float64) / float(0xFFFFFF) * 2.0 - 1.0
+
+
+def value_noise(p: np.ndarray, seed: int, octaves: int = 3) -> np.ndarray:
+    """Seeded 3-octave trilinear value noise, amplitude sum 1, range [-1, 1]."""
+    out = np.zeros(p.shape[0])
+    amp, freq, total = 1.0, 4.0, 0.0
+    for o in range(octaves):
+        q = p * freq + 1000.0
+        i = np.floor(q).astype(np.int64)
+        fr = q - i
+        s = fr * fr * (3 - 2 * fr)
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    w = (s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) \
+                        * (s[:, 2] if dz else 1 - s[:, 2])
+                    acc += w * _lattice(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed + o)
+        out += amp * acc
+        total += amp
+        amp *= 0.5
+        freq *= 2.0
+    return out / total
+
+
+def icosphere(subdivisions: int, displacement: float = 0.0, seed: int = MESH_SEED) -> dict:
+    """Radius-1 icosphere (20 * 4^n faces), displaced along normals by value noise."""
+    v, f = _icosahedron()
+    for _ in range(subdivisions):
+        v, f = _subdivide(v, f)
+    if displacement:
+        v = v * (1.0 + displacement * value_noise(v, seed))[:, None]
+    return {"vertices": v.astype(np.float32), "faces": f.astype(np.int32),
+            "normals": None, "texcoords": None, "face_tex": None}
+
+
+def ground_quad(y: float, half: float = 40.0, z0: float = -60.0, z1: float = 10.0) -> dict:
+    v = np.array([[-half, y, z0], [half, y, z0], [half, y, z1], [-half, y, z1]], np.float32)
+    f = np.array([[0, 2, 1], [0, 3, 2]], np.int32)   # normals point up (+y)
+    return {"vertices": v, "faces": f, "normals": None, "texcoords": None, "face_tex": None}
+
+
+CAMERA = {"pos": [0.0, 0.0, 0.0], "dir": [0.0, 0.0, -1.0], "up": [0.0, 1.0, 0.0], "fov": 45.0}
+
+
+def config_scene(cfg: int, *, width: int | None = None, height: int | None = None,
+                 mesh_subdivisions: int = 7) -> dict:
+    """Scene dicts of the benchmark configurations (spp is stored under "spp")."""
+    sky = env_map()
+    if cfg in (1, 2):
+        single = cfg == 1
+        res = 128 if single else 256
+        return {
+            "width": width or res, "height": height or res,
+            "spp": 16 if single else 64, "max_bounce": 10, "max_medium_bounce": 10,
+            "gp_mode": "single_realization" if single else "renewal_plus",
+            "camera": dict(CAMERA),
+            "kernel": {"sigma": 1.0 if single else 1.5, "length_scale": 1.0 if single else 2.0,
+                       "aniso": [1, 1, 1] if single else [1, 1, 4], "points_per_cell": 3},
+            "gp_medium": {"mean_type": "sphere", "mean_center": [0, 0, -500], "mean_radius": 70},
+            "skydrome": sky,
+            "primitives": [{"type": "sphere", "center": [0, 0, -500], "radius": 130,
+                            "bsdf": "null", "int_medium": True}],
+        }
+    if cfg == 3:
+        return {
+            "width": width or 512, "height": height or 512, "spp": 64, "max_bounce": 10,
+            "max_medium_bounce": 10, "gp_mode": "renewal_plus", "camera": dict(CAMERA),
+            "kernel": {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},
+            "gp_medium": {"mean_type": "mesh",
+                          "mesh": icosphere(mesh_subdivisions, displacement=0.05),
+                          "transform": {"scale": 2.0, "position": [0.0, 0.0, -5.0]}},
+            "skydrome": sky,
+            "primitives": [
+                {"type": "sphere", "center": [0, 0, -5], "radius": 3, "bsdf": "null",
+                 "int_medium": True},
+                {"type": "mesh", "mesh": ground_quad(-2.2), "bsdf": "lambertian",
+                 "texture": np.full((1, 1, 3), 0.7, np.float32)},
+            ],
+        }
+    raise ValueError(f"configuration {cfg} has no scene generator here")

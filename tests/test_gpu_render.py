@@ -1,0 +1,111 @@
+"""GPU integrator parity: per-path replay vs the oracle, sky-only exactness, determinism."""
+import numpy as np
+import pytest
+
+from mupsi_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _medium_pixels(W, H, rng, n):
+    """Pixels biased to the enclosing-sphere disc (about 18% of the image)."""
+    cx, cy = W / 2, H / 2
+    r = 0.45 * W * rng.uniform(0, 1, n) ** 0.5
+    a = rng.uniform(0, 2 * np.pi, n)
+    x = np.clip((cx + r * np.cos(a)).astype(np.int32), 0, W - 1)
+    y = np.clip((cy + r * np.sin(a)).astype(np.int32), 0, H - 1)
+    return x, y
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_path_replay_matches_oracle(ctx, cfg):
+    from mupsi_amd import Scene
+    sc = scenes.config_scene(cfg)
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(40 + cfg)
+    n = 1500
+    x, y = _medium_pixels(sc["width"], sc["height"], rng, n)
+    k = rng.integers(0, 64, n).astype(np.int32)
+    got = s.trace_paths(x, y, k)
+    ref = osc.trace_paths(x, y, k)
+    close = np.all(np.abs(got - ref) <= 1e-4 * (1 + np.abs(ref)), axis=1)
+    frac = close.mean()
+    print(f"cfg{cfg}: {close.sum()}/{n} paths agree; nonzero {np.count_nonzero(ref.sum(1))}")
+    assert frac >= 0.97, frac
+
+
+def test_sky_only_scene_is_exact(ctx):
+    from mupsi_amd import Scene
+    sky = scenes.env_map()
+    sc = {"width": 48, "height": 32, "camera": dict(scenes.CAMERA), "skydrome": sky,
+          "primitives": [], "max_bounce": 4}
+    s = Scene(ctx, sc)
+    s.render_passes(0, 2)
+    img = s.image()
+    osc = O.OracleScene(sc)
+    xs, ys = np.meshgrid(np.arange(48), np.arange(32))
+    dirs = osc.camera_rays(xs.ravel(), ys.ravel())
+    rgb, _ = osc.sky_lookup(dirs)
+    np.testing.assert_array_equal(img.reshape(-1, 3).astype(np.float32), rgb)
+
+
+def test_render_matches_oracle_small(ctx):
+    from mupsi_amd import Scene
+    sc = scenes.config_scene(2, width=40, height=40)
+    s = Scene(ctx, sc)
+    s.render_passes(0, 4)
+    sums, cnt = s.read_accum()
+    assert cnt == 4
+    osc = O.OracleScene(sc)
+    ref, paths, _ = osc.render(0, 4)
+    assert paths == 40 * 40 * 4
+    diff = np.abs(sums - ref)
+    bad = np.any(diff > 1e-4 * (1 + np.abs(ref)), axis=2)
+    assert bad.mean() <= 0.03, bad.mean()
+
+
+def test_render_deterministic_and_chunked(ctx):
+    from mupsi_amd import Scene
+    sc = scenes.config_scene(2, width=64, height=64)
+    s = Scene(ctx, sc)
+    s.render_passes(0, 8, passes_per_launch=8)
+    a, _ = s.read_accum()
+    s.reset()
+    s.render_passes(0, 8, passes_per_launch=3)
+    b, _ = s.read_accum()
+    s.reset()
+    for k in range(8):
+        s.render_passes(k, 1)
+    c, n = s.read_accum()
+    assert n == 8
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(a, c, rtol=1e-12, atol=1e-12)
+
+
+def test_abort_flag_keeps_last_pass(ctx):
+    import ctypes
+    from mupsi_amd import Scene, MupsiError
+    sc = scenes.config_scene(2, width=32, height=32)
+    s = Scene(ctx, sc)
+    s.render_passes(0, 2)
+    before, n0 = s.read_accum()
+    flag = ctypes.c_int32(1)
+    with pytest.raises(MupsiError):
+        s.render_passes(2, 4, 1, flag)
+    after, n1 = s.read_accum()
+    assert n0 == n1 == 2
+    np.testing.assert_array_equal(before, after)
+
+
+def test_scene_json_matches_dict(ctx, tmp_path):
+    from mupsi_amd import Scene, write_scene_files
+    sc = scenes.config_scene(2, width=24, height=24)
+    path = write_scene_files(sc, tmp_path)
+    a = Scene(ctx, sc)
+    b = Scene(ctx, path)
+    assert b.spp == 64 and (b.width, b.height) == (24, 24)
+    a.render_passes(0, 2)
+    b.render_passes(0, 2)
+    np.testing.assert_array_equal(a.read_accum()[0], b.read_accum()[0])
