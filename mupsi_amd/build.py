@@ -55,6 +55,21 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_cli(verbose: bool = False) -> Path:
+    """`mupsi [scene.json] [out.png]` linked against the in-tree library."""
+    exe = PKG / "mupsi"
+    tmp = PKG / "mupsi.tmp"
+    cmd = ["g++", "-O2", "-std=c++20", "-o", str(tmp), str(CSRC / "cli_main.cpp"),
+           f"-L{PKG}", "-lmupsi_hip", "-lz", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if verbose or res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+    if res.returncode != 0:
+        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}")
+    os.replace(tmp, exe)
+    return exe
+
+
 def build_oracle(with_reference: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
     if with_reference and Path("/root/reference/src/math/random.h").exists():
@@ -64,3 +79,4 @@ def build_oracle(with_reference: bool = True) -> None:
 if __name__ == "__main__":
     build_oracle()
     print(build_library(verbose="-v" in sys.argv))
+    print(build_cli())

@@ -1,0 +1,95 @@
+"""Pinning the oracle's integrator (trace.cpp:32-213) with scenes whose radiance has a
+closed form: sky only, a null sphere (pass-through), a mirror sphere, and a
+Lambertian furnace with Russian roulette."""
+import numpy as np
+
+from mupsi_amd import scenes
+from oracle import oracle as O
+
+
+def _grid(w, h):
+    xs, ys = np.meshgrid(np.arange(w), np.arange(h))
+    return xs.ravel().astype(np.int32), ys.ravel().astype(np.int32)
+
+
+def test_sky_only_paths_return_the_sky_texel():
+    sky = scenes.env_map()
+    sc = {"width": 40, "height": 30, "camera": dict(scenes.CAMERA), "skydrome": sky,
+          "primitives": [], "max_bounce": 3}
+    osc = O.OracleScene(sc)
+    x, y = _grid(40, 30)
+    rgb = osc.trace_paths(x, y, np.zeros_like(x))
+    want, _ = osc.sky_lookup(osc.camera_rays(x, y))
+    np.testing.assert_array_equal(rgb, want)
+
+
+def test_null_sphere_passes_rays_through():
+    sky = scenes.env_map()
+    sc = {"width": 32, "height": 32, "camera": dict(scenes.CAMERA), "skydrome": sky,
+          "max_bounce": 10,
+          "primitives": [{"type": "sphere", "center": [0, 0, -10], "radius": 6, "bsdf": "null"}]}
+    osc = O.OracleScene(sc)
+    x, y = _grid(32, 32)
+    rgb = osc.trace_paths(x, y, np.zeros_like(x))
+    want, _ = osc.sky_lookup(osc.camera_rays(x, y))
+    assert (np.abs(rgb - want).max(axis=1) == 0).mean() > 0.99
+
+
+def test_mirror_sphere_reflects_the_sky():
+    sky = scenes.env_map()
+    c = np.array([0.0, 0.0, -10.0])
+    sc = {"width": 24, "height": 24, "camera": dict(scenes.CAMERA), "skydrome": sky,
+          "max_bounce": 10,
+          "primitives": [{"type": "sphere", "center": c.tolist(), "radius": 4, "bsdf": "specular"}]}
+    osc = O.OracleScene(sc)
+    x, y = _grid(24, 24)
+    d = osc.camera_rays(x, y).astype(np.float64)
+    rgb = osc.trace_paths(x, y, np.zeros_like(x))
+    # analytic reflection off the sphere (one bounce: a convex mirror)
+    b = d @ c
+    disc = b * b - (c @ c - 16.0)
+    hit = disc > 0
+    t = b - np.sqrt(np.where(hit, disc, 0))
+    p = d * t[:, None]
+    n = (p - c) / 4.0
+    r = d - 2 * np.sum(d * n, 1)[:, None] * n
+    want_hit, _ = osc.sky_lookup(r.astype(np.float32))
+    want_miss, _ = osc.sky_lookup(d.astype(np.float32))
+    want = np.where(hit[:, None], want_hit, want_miss)
+    edge = hit & (disc < 0.05 * 16)
+    match = np.all(np.isclose(rgb, want, rtol=1e-6, atol=1e-7), axis=1)
+    assert match[~edge].mean() > 0.97
+    assert hit.sum() > 50
+
+
+def test_lambertian_furnace_with_russian_roulette():
+    """Convex Lambertian sphere (albedo 0.8) under a constant sky c: every path that hits it
+    returns 0 (RR, p = 0.2) or exactly c (T = 0.8 / 0.8); the mean is 0.8 c."""
+    c = 0.625
+    sky = np.full((4, 8, 3), c, np.float32)
+    sc = {"width": 16, "height": 16, "camera": dict(scenes.CAMERA), "skydrome": sky,
+          "max_bounce": 10,
+          "primitives": [{"type": "sphere", "center": [0, 0, -3], "radius": 2.5,
+                          "bsdf": "lambertian"}]}
+    osc = O.OracleScene(sc)
+    x, y = _grid(16, 16)
+    xs = np.tile(x, 64)
+    ys = np.tile(y, 64)
+    ks = np.repeat(np.arange(64, dtype=np.int32), x.size)
+    rgb = osc.trace_paths(xs, ys, ks)
+    vals = np.unique(rgb)
+    assert set(np.round(vals, 6)) <= {0.0, np.float32(c)}
+    frac = (rgb[:, 0] > 0).mean()
+    assert abs(frac - 0.8) < 0.02, frac
+
+
+def test_render_sums_equal_trace_paths():
+    sc = scenes.config_scene(2, width=12, height=10)
+    osc = O.OracleScene(sc)
+    sums, paths, _ = osc.render(0, 3)
+    assert paths == 12 * 10 * 3
+    x, y = _grid(12, 10)
+    acc = np.zeros((10, 12, 3))
+    for k in range(3):
+        acc += osc.trace_paths(x, y, np.full_like(x, k)).reshape(10, 12, 3)
+    np.testing.assert_allclose(sums, acc, rtol=1e-12, atol=1e-12)
