@@ -26,6 +26,7 @@
 #include "../../include/mupsi.h"
 #include "device.h"
 #include "host_bvh.h"
+#include "march_cached.h"
 #include "scene_host.h"
 
 using namespace mupsi;
@@ -535,6 +536,207 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
   }
 }
 
+// Cached-window marcher (see march_cached.h): same decisions as k_march, two rays
+// per 64-lane workgroup, impulses regenerated once per cell entering the window.
+// Used for sphere means with points_per_cell <= 8.
+struct CachedConsts {
+  float inv_kr;   // 1 / kr (rounded)
+  int kr_pow2;    // kr is a power of two: q * inv_kr == q / kr exactly
+  float psi_k;    // -log2(e) / L^2: h = exp2(|p - s|^2 * psi_k)
+};
+
+__global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, CachedConsts cc) {
+  __shared__ float4 win[2][27 * kMaxPpcCached];
+  const int n = a.count_dev ? *a.count_dev : a.count_host;
+  const int lane = threadIdx.x;
+  const int grp = lane >> 5;
+  const int gl = lane & (kGroup - 1);
+  float4* W = win[grp];
+  const DevGP& G = S.gp;
+  const int ppc = G.ppc;
+  const int nslots = 27 * ppc;
+  const F3 mc = mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]);
+  // ray state: identical in the 32 lanes of a group
+  int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
+  int rid = 0;
+  F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  bool sp = false;
+  uint32_t seed = 0;
+  Cond c;
+  c.active = false;
+  int nsteps = 0;
+  int wx = 0, wy = 0, wz = 0;
+  bool wvalid = false;
+  unsigned long long evals = 0, refsteps = 0, rays = 0;
+
+  auto finish = [&](bool hit, F3 p) {
+    if (gl == 0) {
+      a.hit[rid] = hit ? 1 : 0;
+      if (hit) {
+        a.t[rid] = t;
+        a.px[rid] = p.x;
+        a.py[rid] = p.y;
+        a.pz[rid] = p.z;
+      }
+      if (a.steps) a.steps[rid] = hit ? nsteps : -1;
+      refsteps += hit ? (unsigned long long)nsteps
+                      : nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
+    }
+    phase = 0;
+  };
+
+  while (true) {
+    bool do_eval = false;
+    F3 q = mk(0, 0, 0);
+    for (int round = 0; round < 2; ++round) {
+      const bool need = phase == 0;
+      const uint64_t m = __ballot(need && gl == 0);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.work, __popcll(m));
+        base = __shfl(base, leader);
+        if (need) {
+          const int qi = base + __popcll(m & ((1ull << (grp * kGroup)) - 1ull));
+          if (qi < n) {
+            rid = a.queue ? a.queue[qi] : qi;
+            o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
+            d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
+            nt = a.nt[rid];
+            ft = fminf(a.ft ? a.ft[rid] : 3.402823466e+38f, 2000.0f);
+            dt = fminf(0.5f, (ft - nt) / 100);
+            seed = a.seed[rid];
+            c.active = a.flags ? (a.flags[rid] & kFlagCond) != 0 : (a.active && a.active[rid]);
+            if (c.active) {
+              c.C = mk(a.cx[rid], a.cy[rid], a.cz[rid]);
+              c.u = a.cu[rid];
+              c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
+            }
+            sign_shell(S, c, o, ft, outer2, inner2);
+            nsteps = 0;
+            wvalid = false;  // new realization seed
+            if (gl == 0) ++rays;
+            phase = 1;
+          } else {
+            phase = 3;
+          }
+        }
+      }
+      if (phase == 1) {
+        q = o;
+        do_eval = true;
+      } else if (phase == 2 && !do_eval) {
+        while (true) {
+          if (!(t < ft)) {
+            finish(false, q);
+            break;
+          }
+          const F3 p = madd(d, t, o);
+          const F3 r = sub(p, mc);
+          const float q2 = sq3(r);
+          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          if (cs == 0) {
+            q = p;
+            do_eval = true;
+            break;
+          }
+          ++nsteps;
+          if ((cs > 0) != sp) {
+            finish(true, p);
+            break;
+          }
+          if ((cs > 0 && dot(d, r) >= 0.0f) || !(t + dt > t)) {
+            finish(false, p);
+            break;
+          }
+          t += dt;
+        }
+      }
+      if (__ballot(phase == 0) == 0) break;
+    }
+    if (__ballot(phase != 3) == 0) break;
+
+    // Window: cells entering the 27-cell neighbourhood of the evaluation point.
+    int newcells = -1;
+    int cx = 0, cy = 0, cz = 0;
+    if (do_eval) {
+      cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
+      cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
+      cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
+      if (!wvalid) {
+        newcells = 27;
+      } else {
+        const int ex = cx - wx, ey = cy - wy, ez = cz - wz;
+        if (ex | ey | ez) {
+          newcells = (abs(ex) <= 1 && abs(ey) <= 1 && abs(ez) <= 1) ? (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)
+                                                                   : 27;
+        }
+      }
+    }
+    __syncthreads();  // the previous evaluation's reads of the window are complete
+    if (newcells >= 0) {
+      const int items = kNewCells.n[newcells] * ppc;
+      for (int it = gl; it < items; it += kGroup) {
+        const int k = it / ppc;
+        const int i = it - k * ppc;
+        const int ni = kNewCells.idx[newcells][k];
+        const int nx = cx + ni / 9 - 1, ny = cy + (ni / 3) % 3 - 1, nz = cz + ni % 3 - 1;
+        W[(mod3(nx) * 9 + mod3(ny) * 3 + mod3(nz)) * ppc + i] = make_impulse(G, nx, ny, nz, seed, i);
+      }
+      wx = cx;
+      wy = cy;
+      wz = cz;
+      wvalid = true;
+    }
+    __syncthreads();
+
+    float acc = 0.0f;
+    if (do_eval) {
+      for (int sl = gl; sl < nslots; sl += kGroup) {
+        const float4 im = W[sl];
+        const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
+        const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+        acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * cc.psi_k), acc);
+      }
+    }
+    for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+
+    if (do_eval) {
+      float f = mean_eval<false>(S, q, nullptr) + G.sigma * acc / G.sqrt_var;
+      if (c.active) {
+        const F3 r = sub(q, c.C);
+        const float h = kern_h(G, c.C, q);
+        f = f + fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
+      }
+      if (gl == 0) ++evals;
+      if (phase == 1) {
+        fprev = f;
+        sp = f > 0.0f;
+        t = nt;
+        phase = 2;
+      } else {
+        ++nsteps;
+        if ((f > 0.0f) != sp && fabsf(fprev - f) > 1e-6f) {
+          finish(true, q);
+        } else if (!(t + dt > t)) {
+          finish(false, q);
+        } else {
+          t += dt;
+        }
+      }
+    }
+  }
+  evals = wave_sum(evals);
+  refsteps = wave_sum(refsteps);
+  rays = wave_sum(rays);
+  if (lane == 0 && a.stats) {
+    atomicAdd(&a.stats[0], evals);
+    atomicAdd(&a.stats[1], refsteps);
+    atomicAdd(&a.stats[2], rays);
+  }
+}
+
 // ====================================================================== device: Renewal+
 
 // GPMedium::sampleCondition (gpmedium.cpp:133-162) as a 4x4 FP64 solve of
@@ -1014,7 +1216,8 @@ struct mupsi_ctx {
   hipStream_t stream = nullptr;
   std::string name;
   int cus = 0;
-  int march_blocks = 0;  // persistent grid of k_march
+  int march_blocks = 0;         // persistent grid of k_march
+  int march_cached_blocks = 0;  // persistent grid of k_march_cached
 };
 
 extern "C" const char* mupsi_status_string(mupsi_status s) {
@@ -1052,6 +1255,8 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_cached, 64, 0));
+  ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   *out = ctx.release();
   return MUPSI_OK;
 }
@@ -1240,6 +1445,21 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   return MUPSI_OK;
 }
 
+// The cached-window marcher serves sphere means with points_per_cell <= 8; mesh
+// means (BVH queries per evaluation) and larger ppc use the lane-per-ray k_march.
+static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma, hipStream_t st) {
+  if (S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached) {
+    CachedConsts cc;
+    cc.inv_kr = 1.0f / S.gp.kr;
+    int e = 0;
+    cc.kr_pow2 = std::frexp(S.gp.kr, &e) == 0.5f ? 1 : 0;
+    cc.psi_k = (float)(-1.4426950408889634 / (double)S.gp.L2);
+    hipLaunchKernelGGL(k_march_cached, dim3(ctx->march_cached_blocks), dim3(64), 0, st, S, ma, cc);
+  } else {
+    hipLaunchKernelGGL(k_march, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
+  }
+}
+
 // Run the wavefront over n paths described by src; leaves per-path radiance in P.L*.
 static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
                                   const volatile int32_t* abort_flag, double* march_ms,
@@ -1291,7 +1511,7 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
       HIP_TRY(hipEventCreate(&e1));
       evs.emplace_back(e0, e1);
       HIP_TRY(hipEventRecord(e0, st));
-      hipLaunchKernelGGL(k_march, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
+      launch_march(ctx, S, ma, st);
       HIP_TRY(hipEventRecord(e1, st));
       HIP_TRY(hipGetLastError());
     }
@@ -1529,7 +1749,7 @@ extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* 
   ma.steps = steps;
   ma.work = work;
   ma.stats = stats;
-  hipLaunchKernelGGL(k_march, dim3(sc->ctx->march_blocks), dim3(256), 0, st, sc->ds, ma);
+  launch_march(sc->ctx, sc->ds, ma, st);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_interleave3, dim3(grid_for(n)), dim3(256), 0, st, px, py, pz, n, p);
   if (mode == 0 && nrm)

@@ -1,0 +1,107 @@
+// march_cached.h — GP-medium marcher with an LDS impulse window (gfx950).
+//
+// Cost model that motivates it: regenerating one impulse (make_seed, a 64-bit
+// LCG jump and four PCG-XSH-RR draws) costs ~110 VALU slots, mostly quarter-rate
+// 32-bit integer multiplies, while evaluating it costs ~12.  A ray advancing by
+// dt = 0.5 through cells of size kr keeps 2/3 or more of its 27-cell neighbourhood
+// from one step to the next, so the impulses are generated once per cell visit
+// and evaluated from LDS at every step.
+//
+// Mapping: 32 lanes (a half-wave "group") march one ray; a 64-lane workgroup holds
+// two groups.  Each group owns a window of 27 cells x ppc impulses in LDS, slot
+// (n_x mod 3, n_y mod 3, n_z mod 3) x ppc + i, so moving the window centre by one
+// cell along an axis overwrites exactly the slab of cells that left it.  The
+// impulses of a cell are written as float4 (position, weight); evaluation reads
+// them with one ds_read_b128 per slot, 27*ppc slots over 32 lanes, and reduces
+// the group's partial sums with cross-lane shuffles.
+//
+// Results (hit flag, hit t bits) are the reference's sampleDistance/transmittance
+// decisions; only the summation order of psi differs from the reference's
+// sequential loop (within the FP32 error bound the parity tests use).
+#pragma once
+#include "device.h"
+
+namespace mupsi {
+namespace dev {
+
+constexpr int kGroup = 32;        // lanes per ray
+constexpr int kMaxPpcCached = 8;  // window capacity: 27 * 8 impulses
+
+struct LcgJump {
+  uint64_t a[kMaxPpcCached];
+  uint64_t c[kMaxPpcCached];
+};
+// Jump of 4*i LCG steps: s_{4i} = A^{4i} s_0 + (A^{4i-1} + ... + 1).
+constexpr LcgJump make_jumps() {
+  LcgJump j{};
+  uint64_t a = 1, c = 0;
+  for (int i = 0; i < kMaxPpcCached; ++i) {
+    j.a[i] = a;
+    j.c[i] = c;
+    for (int k = 0; k < 4; ++k) {
+      c = c * 6364136223846793005ULL + 1ULL;
+      a = a * 6364136223846793005ULL;
+    }
+  }
+  return j;
+}
+__constant__ constexpr LcgJump kJump = make_jumps();
+
+// New cells of the window when its centre moves by delta (each |delta_a| <= 1):
+// {d in {-1,0,1}^3 : some axis a has delta_a != 0 and d_a == delta_a}.
+// Entry [delta index][k] is the neighbour index 9(dx+1) + 3(dy+1) + (dz+1).
+struct NewCells {
+  uint8_t n[28];
+  uint8_t idx[28][27];
+};
+constexpr NewCells make_new_cells() {
+  NewCells t{};
+  for (int di = 0; di < 28; ++di) {
+    int cnt = 0;
+    for (int ni = 0; ni < 27; ++ni) {
+      bool add = di == 27;  // 27: refill the whole window
+      if (di < 27) {
+        const int dd[3] = {di / 9 - 1, (di / 3) % 3 - 1, di % 3 - 1};
+        const int nn[3] = {ni / 9 - 1, (ni / 3) % 3 - 1, ni % 3 - 1};
+        for (int a = 0; a < 3; ++a) add = add || (dd[a] != 0 && nn[a] == dd[a]);
+      }
+      if (add) t.idx[di][cnt++] = (uint8_t)ni;
+    }
+    t.n[di] = (uint8_t)cnt;
+  }
+  return t;
+}
+__constant__ constexpr NewCells kNewCells = make_new_cells();
+
+__device__ __forceinline__ int mod3(int n) {
+  const int r = n % 3;
+  return r < 0 ? r + 3 : r;
+}
+
+// trunc(q / kr) with the reference's correctly rounded division.  A product with
+// the rounded reciprocal is within 2 ulp of the quotient; it is used unless an
+// integer lies that close to it (or kr is a power of two, where it is exact).
+__device__ __forceinline__ int cell_coord(float q, const DevGP& g, float inv_kr, bool pow2) {
+  const float a = q * inv_kr;
+  if (pow2) return (int)a;
+  const float r = rintf(a);
+  if (fabsf(a - r) > 4.0f * 1.1920929e-7f * fmaxf(1.0f, fabsf(a))) return (int)a;
+  return (int)(q / g.kr);
+}
+
+// Impulse i of cell (nx, ny, nz) of realization seed as (position, weight).
+__device__ __forceinline__ float4 make_impulse(const DevGP& g, int nx, int ny, int nz,
+                                               uint32_t seed, int i) {
+  const uint32_t cs = make_seed4((uint32_t)nx, (uint32_t)ny, (uint32_t)nz, seed);
+  uint64_t s = (uint32_t)(cs + 1u);
+  s = s * kJump.a[i] + kJump.c[i];
+  const uint32_t r0 = pcg_next(s);
+  const uint32_t r1 = pcg_next(s);
+  const uint32_t r2 = pcg_next(s);
+  const uint32_t r3 = pcg_next(s);
+  return make_float4(fmaf(u01(r2), g.kr, (float)nx * g.kr), fmaf(u01(r1), g.kr, (float)ny * g.kr),
+                     fmaf(u01(r0), g.kr, (float)nz * g.kr), (r3 >> 31) ? 1.0f : -1.0f);
+}
+
+}  // namespace dev
+}  // namespace mupsi

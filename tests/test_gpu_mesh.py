@@ -1,0 +1,89 @@
+"""GPU parity of the BVH / mesh queries (bit-exact triangle ids) and of a mesh-mean march."""
+import numpy as np
+import pytest
+
+from mupsi_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh_scene(sub=4):
+    m = scenes.icosphere(sub, displacement=0.05)
+    sc = {"width": 16, "height": 16, "camera": dict(scenes.CAMERA), "max_bounce": 4,
+          "gp_mode": "renewal_plus",
+          "kernel": {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},
+          "gp_medium": {"mean_type": "mesh", "mesh": m,
+                        "transform": {"scale": 2.0, "position": [0.0, 0.0, -5.0]}},
+          "primitives": [{"type": "mesh", "mesh": m, "bsdf": "lambertian",
+                          "transform": {"scale": 2.0, "position": [0.0, 0.0, -5.0]}}]}
+    return sc
+
+
+def _rays(rng, n, center, spread):
+    o = (np.asarray(center) + rng.normal(0, spread, (n, 3))).astype(np.float32)
+    d = (np.asarray(center) + rng.normal(0, spread * 0.3, (n, 3)) - o).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 3:6] = d
+    rays[:, 6] = 0.0
+    rays[:, 7] = np.finfo(np.float32).max
+    return rays
+
+
+@pytest.mark.parametrize("prim", [0, -1])
+def test_mesh_queries_bit_exact(ctx, dev, prim):
+    import torch
+    from mupsi_amd import Scene, batch
+    sc = _mesh_scene()
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(7)
+    rays = _rays(rng, 20000, [0, 0, -5.0], 4.0)
+    rt = torch.from_numpy(rays).to(dev)
+    tri, t, u, v = (x.cpu().numpy() for x in batch.mesh_intersect(s, prim, rt))
+    otri, ot, ou, ov = osc.mesh_intersect(prim, rays)
+    assert np.array_equal(tri, otri)
+    h = tri >= 0
+    assert h.sum() > 5000
+    assert np.array_equal(t[h].view(np.uint32), ot[h].view(np.uint32))
+    assert np.array_equal(u[h].view(np.uint32), ou[h].view(np.uint32))
+    occ = batch.mesh_occluded(s, prim, rt).cpu().numpy()
+    assert np.array_equal(occ, osc.mesh_occluded(prim, rays))
+    p = (np.array([0, 0, -5.0]) + rng.normal(0, 2.5, (20000, 3))).astype(np.float32)
+    pt = torch.from_numpy(p).to(dev)
+    d, ctri, cl = (x.cpu().numpy() for x in batch.mesh_closest(s, prim, pt))
+    od, otri2, ocl = osc.mesh_closest(prim, p)
+    assert np.array_equal(d.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(ctri, otri2)
+    ins = batch.mesh_inside(s, prim, pt).cpu().numpy()
+    assert np.array_equal(ins, osc.mesh_inside(prim, p))
+
+
+def test_mesh_mean_march(ctx, dev):
+    import torch
+    from mupsi_amd import Scene, batch
+    sc = _mesh_scene(sub=3)
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(8)
+    n = 300
+    c = np.array([0, 0, -5.0])
+    d0 = rng.normal(size=(n, 3))
+    d0 /= np.linalg.norm(d0, axis=1, keepdims=True)
+    o = c + d0 * 2.9
+    d = (c + rng.normal(0, 0.8, (n, 3))) - o
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = o
+    rays[:, 3:6] = d
+    rays[:, 6] = 1.5e-3
+    rays[:, 7] = rng.uniform(3.0, 12.0, n)   # bounded march lengths keep the oracle fast
+    seeds = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    ref64 = osc.march(rays, seeds, None, fp64=True, track=True)
+    got = batch.march(s, torch.from_numpy(rays).to(dev), torch.from_numpy(seeds.view(np.int32)).to(dev))
+    hit = got["hit"].cpu().numpy()
+    t = got["t"].cpu().numpy()
+    unamb = (ref64["amb"] > 2e-6) & (ref64["cell"] > 1e-6)
+    ok = (hit == ref64["hit"]) & ((~hit.astype(bool)) | (t.view(np.uint32) == ref64["t"].view(np.uint32)))
+    assert ok[unamb].all()
+    assert hit.sum() > 20
