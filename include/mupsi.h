@@ -154,6 +154,7 @@ typedef struct {
 
 /* ---- context ---- */
 mupsi_status mupsi_open(int32_t device, mupsi_ctx** out);
+/* Frees the context and every scene still open on it; those scene handles become invalid. */
 void mupsi_close(mupsi_ctx* ctx);
 const char* mupsi_status_string(mupsi_status s);
 const char* mupsi_last_error(void);

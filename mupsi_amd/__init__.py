@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 from pathlib import Path
 from typing import Callable, Optional
 
@@ -43,6 +44,7 @@ class Context:
         check(self.lib.mupsi_open(device, C.byref(h)), "mupsi_open")
         self.handle = h
         self.device = device
+        self._scenes: "weakref.WeakSet[Scene]" = weakref.WeakSet()
 
     def info(self) -> tuple[str, int]:
         name = C.create_string_buffer(256)
@@ -51,7 +53,10 @@ class Context:
         return name.value.decode(), cus.value
 
     def close(self) -> None:
+        """Free every scene of this context, then the context (mupsi_close)."""
         if self.handle:
+            for s in list(self._scenes):
+                s.close()
             self.lib.mupsi_close(self.handle)
             self.handle = None
 
@@ -82,6 +87,7 @@ class Scene:
             del holder
             self.spp = int(scene.get("spp", 1))
         self.handle = h
+        ctx._scenes.add(self)
         w, hh = C.c_int32(), C.c_int32()
         check(self.lib.mupsi_scene_size(h, C.byref(w), C.byref(hh)), "mupsi_scene_size")
         self.width, self.height = w.value, hh.value
@@ -129,7 +135,8 @@ class Scene:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
-            self.lib.mupsi_scene_free(self.handle)
+            if self.ctx.handle:           # mupsi_close already freed it otherwise
+                self.lib.mupsi_scene_free(self.handle)
             self.handle = None
 
     def __del__(self):

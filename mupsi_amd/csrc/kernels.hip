@@ -341,21 +341,89 @@ __device__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls)
 
 // ====================================================================== device: march
 
-// Decision margin of the certain-sign test (see k_march).
-__device__ __forceinline__ void sign_shell(const DevScene& S, const Cond& c, F3 o, float ft,
-                                          float& outer2, float& inner2) {
-  if (S.mean_type != MUPSI_MEAN_SPHERE) {
-    outer2 = INFINITY;
-    inner2 = -1.0f;
-    return;
-  }
+// Certain-sign test of one grid point (see k_march).  |psi| <= gp.bound and the
+// conditioning term is bounded by cond_bound, so wherever |mu| provably exceeds
+// B = their sum plus a rounding margin the sign of f is the sign of mu.
+//  - sphere mean: a shell test on |p - c|^2;
+//  - mesh mean: mu = (inside ? -1 : 1) * dist and dist >= scale * dist(p_local,
+//    root AABB).  Outside the +x shadow slab of the root box the parity ray
+//    cannot reach the box, so inside() is exactly false; inside the slab one
+//    inside() call (the reference's own parity test) decides the sign.
+struct SkipInfo {
+  int kind;            // 0 none, 1 sphere mean, 2 mesh mean
+  float outer2, inner2;
+  float Bm;            // mesh: B + margin (world units)
+  F3 cb;               // mesh: centre of the world bounding ball of the root box
+  float rb;            // mesh: its radius
+  float wmin[3], wmax[3];  // mesh: root box in world space (s > 0)
+};
+
+__device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, F3 o, float ft) {
+  SkipInfo k;
+  k.kind = 0;
+  k.outer2 = INFINITY;
+  k.inner2 = -1.0f;
   const float B = S.gp.bound + cond_bound(S.gp, c);
-  const F3 oc = sub(o, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
-  const float m = 0.01f + 2e-6f * (norm(oc) + ft + S.mean_r + B);
-  const float ro = S.mean_r + B + m;
-  outer2 = ro * ro;
-  const float ri = S.mean_r - B - m;
-  inner2 = ri > 0.0f ? ri * ri : -1.0f;
+  if (S.mean_type == MUPSI_MEAN_SPHERE) {
+    const F3 oc = sub(o, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
+    const float m = 0.01f + 2e-6f * (norm(oc) + ft + S.mean_r + B);
+    const float ro = S.mean_r + B + m;
+    k.outer2 = ro * ro;
+    const float ri = S.mean_r - B - m;
+    k.inner2 = ri > 0.0f ? ri * ri : -1.0f;
+    k.kind = 1;
+    return k;
+  }
+  const DevMesh& M = S.mean_mesh;
+  if (M.n_nodes == 0 || !(M.xf.s > 0.0f)) return k;
+  for (int a = 0; a < 3; ++a) {
+    k.wmin[a] = fmaf(M.xf.s, M.lbmin[a], M.xf.pos[a]);
+    k.wmax[a] = fmaf(M.xf.s, M.lbmax[a], M.xf.pos[a]);
+  }
+  k.cb = mk(0.5f * (k.wmin[0] + k.wmax[0]), 0.5f * (k.wmin[1] + k.wmax[1]),
+            0.5f * (k.wmin[2] + k.wmax[2]));
+  k.rb = 0.5f * norm(mk(k.wmax[0] - k.wmin[0], k.wmax[1] - k.wmin[1], k.wmax[2] - k.wmin[2])) * 1.0001f;
+  k.Bm = B + 0.01f + 2e-6f * (norm(sub(o, k.cb)) + ft + k.rb + B);
+  k.kind = 2;
+  return k;
+}
+
+// Returns +1 / -1 when the sign of f(p) is certain, 0 otherwise.  *final: no later
+// grid point of the ray can have another certain-or-uncertain sign (the march can
+// stop once this sign equals the origin's).
+__device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo& k, F3 p, F3 d,
+                                              bool* final) {
+  *final = false;
+  if (k.kind == 1) {
+    const F3 r = sub(p, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
+    const float q2 = sq3(r);
+    if (q2 > k.outer2) {
+      *final = dot(d, r) >= 0.0f;
+      return 1;
+    }
+    return q2 < k.inner2 ? -1 : 0;
+  }
+  if (k.kind == 2) {
+    const DevMesh& M = S.mean_mesh;
+    const F3 pl = xf_to_local(M.xf, p);
+    const float dx = fmaxf(fmaxf(M.lbmin[0] - pl.x, 0.0f), pl.x - M.lbmax[0]);
+    const float dy = fmaxf(fmaxf(M.lbmin[1] - pl.y, 0.0f), pl.y - M.lbmax[1]);
+    const float dz = fmaxf(fmaxf(M.lbmin[2] - pl.z, 0.0f), pl.z - M.lbmax[2]);
+    const float dw = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) * M.xf.s;
+    if (!(dw > k.Bm)) return 0;
+    // the root-box test of the +x parity ray (AABB::intersect with d = (1, 0, 0))
+    const bool in_slab = pl.y >= M.lbmin[1] && pl.y <= M.lbmax[1] && pl.z >= M.lbmin[2] &&
+                         pl.z <= M.lbmax[2] && M.lbmax[0] - pl.x >= 0.0f;
+    if (in_slab) return mesh_inside(M, p) ? -1 : 1;
+    const F3 rc = sub(p, k.cb);
+    const float e = 1e-3f * (1.0f + fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
+    const bool leaves = (p.x > k.wmax[0] + e && d.x >= 0.0f) || (p.y > k.wmax[1] + e && d.y >= 0.0f) ||
+                        (p.y < k.wmin[1] - e && d.y <= 0.0f) || (p.z > k.wmax[2] + e && d.z >= 0.0f) ||
+                        (p.z < k.wmin[2] - e && d.z <= 0.0f);
+    *final = leaves && dot(d, rc) >= 0.0f && norm(rc) > k.rb + k.Bm;
+    return 1;
+  }
+  return 0;
 }
 
 // Serial march of one ray (used for NEE shadow rays in k_shade): the same
@@ -409,14 +477,15 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
   int rid = 0;
   F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f;
   bool sp = false;
   uint32_t seed = 0;
   Cond c;
   c.active = false;
+  SkipInfo sk;
+  sk.kind = 0;
   int nsteps = 0;
   unsigned long long evals = 0, refsteps = 0, rays = 0;
-  const F3 mc = mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]);
 
   auto finish = [&](bool hit, F3 p) {
     a.hit[rid] = hit ? 1 : 0;
@@ -460,7 +529,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sign_shell(S, c, o, ft, outer2, inner2);
+            sk = make_skip(S, c, o, ft);
             nsteps = 0;
             ++rays;
             phase = 1;
@@ -479,9 +548,8 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
             break;
           }
           const F3 p = madd(d, t, o);
-          const F3 r = sub(p, mc);
-          const float q2 = sq3(r);
-          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          bool fin;
+          const int cs = classify_point(S, sk, p, d, &fin);
           if (cs == 0) {
             q = p;
             do_eval = true;
@@ -492,11 +560,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
             finish(true, p);
             break;
           }
-          if (cs > 0 && dot(d, r) >= 0.0f) {
-            finish(false, p);
-            break;
-          }
-          if (!(t + dt > t)) {  // dt below half an ulp of t: the grid cannot advance
+          if (fin || !(t + dt > t)) {  // no later sign change / the grid cannot advance
             finish(false, p);
             break;
           }
@@ -613,7 +677,11 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sign_shell(S, c, o, ft, outer2, inner2);
+            {
+              const SkipInfo sk = make_skip(S, c, o, ft);
+              outer2 = sk.outer2;
+              inner2 = sk.inner2;
+            }
             nsteps = 0;
             wvalid = false;  // new realization seed
             if (gl == 0) ++rays;
@@ -1218,6 +1286,7 @@ struct mupsi_ctx {
   int cus = 0;
   int march_blocks = 0;         // persistent grid of k_march
   int march_cached_blocks = 0;  // persistent grid of k_march_cached
+  std::vector<mupsi_scene*> scenes;  // live scenes, freed by mupsi_close
 };
 
 extern "C" const char* mupsi_status_string(mupsi_status s) {
@@ -1261,10 +1330,16 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   return MUPSI_OK;
 }
 
+static void destroy_scene(mupsi_scene* scene);
+
+// Frees the context and every scene still open on it (their handles become invalid).
 extern "C" void mupsi_close(mupsi_ctx* ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  (void)hipSetDevice(ctx->device);
+  std::vector<mupsi_scene*> live;
+  live.swap(ctx->scenes);
+  for (mupsi_scene* s : live) destroy_scene(s);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
@@ -1402,15 +1477,23 @@ extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_des
   const size_t npix = (size_t)D.w * D.h;
   HIP_TRY(sc->mem.alloc(npix * 3, &sc->accum));
   HIP_TRY(hipMemset(sc->accum, 0, npix * 3 * sizeof(double)));
+  ctx->scenes.push_back(sc.get());
   *out = sc.release();
   return MUPSI_OK;
 }
 
+static void destroy_scene(mupsi_scene* scene) {
+  (void)hipStreamSynchronize(scene->ctx->stream);
+  delete scene;
+}
+
 extern "C" void mupsi_scene_free(mupsi_scene* scene) {
   if (!scene) return;
-  hipSetDevice(scene->ctx->device);
-  hipStreamSynchronize(scene->ctx->stream);
-  delete scene;
+  mupsi_ctx* ctx = scene->ctx;
+  auto it = std::find(ctx->scenes.begin(), ctx->scenes.end(), scene);
+  if (it != ctx->scenes.end()) ctx->scenes.erase(it);
+  (void)hipSetDevice(ctx->device);
+  destroy_scene(scene);
 }
 
 extern "C" mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int32_t* height) {

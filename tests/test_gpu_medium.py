@@ -97,15 +97,16 @@ def test_march_hits_bit_exact(ctx, dev, cfg):
     t = got["t"].cpu().numpy()
     nrm = got["n"].cpu().numpy()
     # a ray is ambiguous when some visited grid point has |f| within a few FP32 roundings of
-    # zero (relative to |mu| + L1 mass) or lies within 1e-6 of a cell boundary
-    unamb = (ref64["amb"] > 2e-6) & (ref64["cell"] > 1e-6)
+    # zero (relative to |mu| + L1 mass).  Cell indices are FP32 decisions on both sides
+    # (the FP64 oracle shares the FP32 grid points), so cell boundaries are not ambiguous.
+    unamb = ref64["amb"] > 2e-6
     print(f"cfg{cfg}: unambiguous {unamb.mean():.3f}, hits {int(hit.sum())}")
     # decisions: identical to the FP64 oracle on every unambiguous ray
     same_hit = hit == ref64["hit"]
     same_t = (~hit.astype(bool)) | (t.view(np.uint32) == ref64["t"].view(np.uint32))
     ok = same_hit & same_t
     assert ok[unamb].all(), f"{(~ok[unamb]).sum()} unambiguous rays differ"
-    assert unamb.mean() > 0.85
+    assert unamb.mean() > 0.95
     # and with the FP32-faithful oracle on nearly all rays
     agree32 = (hit == ref32["hit"]) & ((~hit.astype(bool)) | (t == ref32["t"]))
     assert agree32.mean() > 0.98

@@ -109,3 +109,21 @@ def test_scene_json_matches_dict(ctx, tmp_path):
     a.render_passes(0, 2)
     b.render_passes(0, 2)
     np.testing.assert_array_equal(a.read_accum()[0], b.read_accum()[0])
+
+
+def test_context_close_with_live_scene_exits_cleanly():
+    """Closing a context frees its scenes; dropping the scene afterwards must not touch it."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from mupsi_amd import Context, Scene, scenes\n"
+            "c = Context(0)\n"
+            "s = Scene(c, scenes.config_scene(2, width=8, height=8))\n"
+            "s.render_passes(0, 1)\n"
+            "c.close()\n"
+            "del s\n"
+            "print('ok')\n") % str(Path(__file__).resolve().parents[1])
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().endswith("ok")
