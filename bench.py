@@ -59,6 +59,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="copy a tensor of this many MiB before the timed region, so that a "
+                         "profiler's FETCH_SIZE / WRITE_SIZE can be scaled by a known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the GPIS path tracer shards by pixel tiles or passes "
@@ -74,6 +77,14 @@ def main() -> int:
     ctx = Context(0)
     s = Scene(ctx, scene)
     print(f"bench: {ctx.info()} scene {W}x{H} spp {spp}", file=sys.stderr, flush=True)
+
+    if args.copy_probe_mb:
+        src = torch.empty(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copy probe of {args.copy_probe_mb} MiB done", file=sys.stderr, flush=True)
+        del src, dst
 
     for i in range(args.warmup):
         s.reset()

@@ -609,16 +609,17 @@ struct CachedConsts {
   float psi_k;    // -log2(e) / L^2: h = exp2(|p - s|^2 * psi_k)
 };
 
+template <int PPC>
 __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, CachedConsts cc) {
-  __shared__ float4 win[2][27 * kMaxPpcCached];
+  __shared__ float4 win[2][27 * PPC];
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
   const int grp = lane >> 5;
   const int gl = lane & (kGroup - 1);
   float4* W = win[grp];
   const DevGP& G = S.gp;
-  const int ppc = G.ppc;
-  const int nslots = 27 * ppc;
+  constexpr int ppc = PPC;
+  constexpr int nslots = 27 * PPC;
   const F3 mc = mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]);
   // ray state: identical in the 32 lanes of a group
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
@@ -745,12 +746,17 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
     __syncthreads();  // the previous evaluation's reads of the window are complete
     if (newcells >= 0) {
       const int items = kNewCells.n[newcells] * ppc;
+      const int mx = mod3(cx), my = mod3(cy), mz = mod3(cz);  // residues of the window centre
       for (int it = gl; it < items; it += kGroup) {
         const int k = it / ppc;
         const int i = it - k * ppc;
         const int ni = kNewCells.idx[newcells][k];
-        const int nx = cx + ni / 9 - 1, ny = cy + (ni / 3) % 3 - 1, nz = cz + ni % 3 - 1;
-        W[(mod3(nx) * 9 + mod3(ny) * 3 + mod3(nz)) * ppc + i] = make_impulse(G, nx, ny, nz, seed, i);
+        const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
+        int rx = mx + ox, ry = my + oy, rz = mz + oz;
+        rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
+        ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
+        rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
+        W[(rx * 9 + ry * 3 + rz) * ppc + i] = make_impulse(G, cx + ox, cy + oy, cz + oz, seed, i);
       }
       wx = cx;
       wy = cy;
@@ -761,6 +767,7 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
 
     float acc = 0.0f;
     if (do_eval) {
+#pragma unroll
       for (int sl = gl; sl < nslots; sl += kGroup) {
         const float4 im = W[sl];
         const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
@@ -1324,7 +1331,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_cached, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_cached<3>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   *out = ctx.release();
   return MUPSI_OK;
@@ -1537,7 +1544,17 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
     int e = 0;
     cc.kr_pow2 = std::frexp(S.gp.kr, &e) == 0.5f ? 1 : 0;
     cc.psi_k = (float)(-1.4426950408889634 / (double)S.gp.L2);
-    hipLaunchKernelGGL(k_march_cached, dim3(ctx->march_cached_blocks), dim3(64), 0, st, S, ma, cc);
+    const dim3 grid(ctx->march_cached_blocks), block(64);
+    switch (S.gp.ppc) {
+      case 1: hipLaunchKernelGGL(k_march_cached<1>, grid, block, 0, st, S, ma, cc); break;
+      case 2: hipLaunchKernelGGL(k_march_cached<2>, grid, block, 0, st, S, ma, cc); break;
+      case 3: hipLaunchKernelGGL(k_march_cached<3>, grid, block, 0, st, S, ma, cc); break;
+      case 4: hipLaunchKernelGGL(k_march_cached<4>, grid, block, 0, st, S, ma, cc); break;
+      case 5: hipLaunchKernelGGL(k_march_cached<5>, grid, block, 0, st, S, ma, cc); break;
+      case 6: hipLaunchKernelGGL(k_march_cached<6>, grid, block, 0, st, S, ma, cc); break;
+      case 7: hipLaunchKernelGGL(k_march_cached<7>, grid, block, 0, st, S, ma, cc); break;
+      default: hipLaunchKernelGGL(k_march_cached<8>, grid, block, 0, st, S, ma, cc); break;
+    }
   } else {
     hipLaunchKernelGGL(k_march, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
   }
