@@ -7,9 +7,11 @@ Workload (one step): the sphere-mean GPIS scene in renewal_plus mode at
 every pixel, i.e. 4,194,304 paths, rendered into the FP64 accumulator.
 
 Reported next to it:
-  roofline      the GP-medium march kernel (k_march): algorithmic FP32 flops per
-                launch (field evaluations x (27*ppc*16 + 9)) over its HIP-event
-                time, against the MI355X FP32 vector peak;
+  roofline      the GP-medium march kernel (k_march_cached<ppc>): algorithmic FP32
+                flops per launch (field evaluations x (27*ppc*16 + 9)) over its
+                HIP-event time, against the MI355X FP32 vector peak; its HBM traffic
+                per launch is the rocprofv3 FETCH_SIZE + WRITE_SIZE figure measured for
+                this same workload, scaled by a device copy of known size;
   cpu_baseline  the CPU restatement of the reference trace loop (OpenMP over
                 pixels, static schedule) on a bounded sample of the same
                 workload, timed on this host's cores.
@@ -27,6 +29,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per k_march_cached<3> launch on this workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`, FETCH_SIZE scaled by
+# 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
+# profiles/k_march_cached_sphere_renewal_256x256x64.md holds the table.
+MARCH_TRAFFIC_BYTES = 101_084_731
 CFG = 2
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "
@@ -140,12 +147,12 @@ def main() -> int:
                    "paths_per_step": W * H * spp, "tracer": "plain wavefront"},
         "roofline": {
             "bound": "valu",
-            "kernel": "k_march",
+            "kernel": f"k_march_cached<{ppc}>",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": MARCH_TRAFFIC_BYTES,
             "flops_per_launch": evals * flops_per_eval / max(1, launches),
             "avg_launch_ms": march_ms / max(1, launches),
         },
