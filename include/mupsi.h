@@ -263,6 +263,90 @@ mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices, const in
                              int32_t n_faces, float* nodes_out, int32_t* n_nodes_out,
                              uint32_t* tri_index_out, uint32_t* root_out);
 
+/* ---- learned path guiding ----
+ * No counterpart in the reference: an addition to its trace loop
+ * (trace.cpp:77-116, the Lambertian sample of handleVolume).  A guide is a
+ * 32 -> 64 -> 64 -> 64 -> 41 MLP (BF16 MFMA, FP32 accumulate, FP32 master
+ * parameters, Adam) whose output is an 8-lobe von Mises-Fisher mixture over
+ * directions plus the probability alpha of sampling it at this vertex.
+ * Attached to a scene, it proposes the direction at every GP-medium hit; the
+ * proposal is one-sample MIS-combined with the Lambertian cosine sample, so the
+ * image keeps the plain tracer's expectation.  The first train_passes
+ * passes rendered with the guide attached train it online on the radiance the
+ * tracer itself measures. */
+typedef struct mupsi_guide mupsi_guide;
+
+typedef struct {
+  uint64_t seed;           /* parameter initialisation and minibatch draws */
+  float learning_rate;     /* Adam step size (1e-3) */
+  float alpha;             /* largest guide selection probability (0.9; 0 = never) */
+  int32_t train_passes;    /* passes of online training (32) */
+  int32_t batch_size;      /* training minibatch, multiple of 64 (4096) */
+  int32_t min_steps_per_pass; /* Adam steps after each training pass: about four */
+  int32_t max_steps_per_pass; /*   epochs of the pass's new records, within [min, max] */
+  int32_t max_records;     /* guided vertices recorded per path for training (1..4) */
+  float box_center[3];     /* position normalisation of the encoding: */
+  float box_radius;        /*   p_hat = clamp((p - center) / radius, -1, 1) */
+  float indirect_clamp;    /* training radiance: direct + min(indirect, clamp) (luminance) */
+  float rr_boost;          /* guided roulette survives with min(1, rr_boost max(T)) (>= 1) */
+} mupsi_guide_config;
+
+typedef struct {
+  int64_t train_passes_done;
+  int64_t train_steps;
+  int64_t train_samples;     /* training records collected */
+  double last_loss;          /* mean minibatch loss of the last step */
+  double train_ms;           /* time in training kernels (HIP events) */
+  double infer_ms;           /* time in inference kernels (HIP events) */
+} mupsi_guide_stats;
+
+enum {
+  MUPSI_GUIDE_NUM_PARAMS = 13097,
+  MUPSI_GUIDE_NUM_INPUTS = 32,
+  MUPSI_GUIDE_NUM_OUTPUTS = 41,
+  MUPSI_GUIDE_MIXTURE_FLOATS = 44  /* 8 x (mu.x, mu.y, mu.z, kappa, weight), alpha, 3 unused */
+};
+
+mupsi_status mupsi_guide_create(mupsi_ctx* ctx, const mupsi_guide_config* cfg, mupsi_guide** out);
+void mupsi_guide_free(mupsi_guide* guide);
+/* FP32 parameters, layout W1[64][32] b1[64] W2[64][64] b2 W3[64][64] b3 W4[41][64] b4[41]. */
+mupsi_status mupsi_guide_get_params(mupsi_guide* guide, float* host_params);
+/* Replaces the parameters and resets Adam's moments and step count. */
+mupsi_status mupsi_guide_set_params(mupsi_guide* guide, const float* host_params);
+mupsi_status mupsi_guide_stats_get(mupsi_guide* guide, mupsi_guide_stats* out);
+/* Attach (or, with guide = NULL, detach) a guide; resets its training pass count. */
+mupsi_status mupsi_scene_set_guide(mupsi_scene* scene, mupsi_guide* guide);
+
+/* Training samples the tracer recorded for the attached guide (the first
+ * min(capacity, stored) records of its ring): features n x 32, sampled
+ * directions n x 3, weights lum(Li) max(cos, 0) / q; host arrays. */
+mupsi_status mupsi_scene_guide_records(mupsi_scene* scene, float* feats, float* dirs,
+                                       float* weights, int64_t capacity, int64_t* count);
+
+/* Batch entry points on device pointers (inference, gradients, one Adam step). */
+/* feats n x 32 f32 -> raw n x 41 outputs and/or mixture n x 44. */
+mupsi_status mupsi_guide_forward(mupsi_guide* guide, const float* feats, int64_t n, float* raw,
+                                 float* mixture);
+/* Loss sum_i -w_i log q(dir_i | feats_i) / n of the combined density
+ * q = alpha p_guide + (1 - alpha) max(cos, 0)/pi (cos against the normal in
+ * feats[24..26]) and its gradient (host, 13097 floats). */
+mupsi_status mupsi_guide_gradient(mupsi_guide* guide, const float* feats, const float* dirs,
+                                  const float* weights, int64_t n, float* grad_host,
+                                  double* loss_host);
+/* One Adam step on the minibatch (all n samples, n <= 65536). */
+mupsi_status mupsi_guide_train_step(mupsi_guide* guide, const float* feats, const float* dirs,
+                                    const float* weights, int64_t n);
+/* Encoding of (p, n, wo), each n x 3, into n x 32 features. */
+mupsi_status mupsi_guide_encode(mupsi_guide* guide, const float* p, const float* nrm,
+                                const float* wo, int64_t n, float* feats);
+/* The tracer's guided sample at a Lambertian vertex: u n x 4 uniforms (technique,
+ * lobe, cosine, azimuth; the cosine branch uses u[1], u[2] as its r1, r2),
+ * mixture n x 44, normals n x 3 -> dir n x 3, combined pdf q n, weight n
+ * (albedo-free f cos / q). */
+mupsi_status mupsi_guide_sample(mupsi_guide* guide, const float* mixture, const float* nrm,
+                                const float* u, int64_t n, float* dir, float* pdf,
+                                float* weight);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,8 +27,8 @@ from . import _abi
 from ._abi import MupsiError, check
 from .scene import build_desc, write_scene_files
 
-__all__ = ["Context", "Scene", "Renderer", "MupsiError", "build_desc", "write_scene_files",
-           "library_path"]
+__all__ = ["Context", "Scene", "Renderer", "Guide", "MupsiError", "build_desc",
+           "write_scene_files", "library_path"]
 
 
 def library_path() -> Path:
@@ -45,6 +45,7 @@ class Context:
         self.handle = h
         self.device = device
         self._scenes: "weakref.WeakSet[Scene]" = weakref.WeakSet()
+        self._guides: "weakref.WeakSet[Guide]" = weakref.WeakSet()
 
     def info(self) -> tuple[str, int]:
         name = C.create_string_buffer(256)
@@ -57,6 +58,8 @@ class Context:
         if self.handle:
             for s in list(self._scenes):
                 s.close()
+            for g in list(self._guides):
+                g.close()
             self.lib.mupsi_close(self.handle)
             self.handle = None
 
@@ -128,6 +131,24 @@ class Scene:
                                          out.ctypes.data_as(C.c_void_p)), "mupsi_trace_paths")
         return out
 
+    def set_guide(self, guide: "Optional[Guide]") -> None:
+        """Attach a learned guide (None: the plain tracer); resets its training count."""
+        check(self.lib.mupsi_scene_set_guide(self.handle, guide.handle if guide else None),
+              "mupsi_scene_set_guide")
+        self._guide = guide
+
+    def guide_records(self, capacity: int = 1 << 22):
+        """Training samples recorded for the attached guide: (feats, dirs, weights)."""
+        f = np.zeros((capacity, 32), np.float32)
+        d = np.zeros((capacity, 3), np.float32)
+        w = np.zeros(capacity, np.float32)
+        n = C.c_int64()
+        check(self.lib.mupsi_scene_guide_records(self.handle, f.ctypes.data_as(C.c_void_p),
+                                                 d.ctypes.data_as(C.c_void_p),
+                                                 w.ctypes.data_as(C.c_void_p), capacity,
+                                                 C.byref(n)), "mupsi_scene_guide_records")
+        return f[:n.value], d[:n.value], w[:n.value]
+
     def stats(self) -> dict:
         s = _abi.Stats()
         check(self.lib.mupsi_last_stats(self.handle, C.byref(s)), "mupsi_last_stats")
@@ -137,6 +158,88 @@ class Scene:
         if getattr(self, "handle", None):
             if self.ctx.handle:           # mupsi_close already freed it otherwise
                 self.lib.mupsi_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Guide:
+    """Learned path-guiding network (mupsi_guide_*): an MLP on MFMA proposing
+    directions at GP-medium hits, trained online during the first train_passes
+    passes of a render it is attached to (Scene.set_guide)."""
+
+    DEFAULTS = dict(seed=1, learning_rate=1e-3, alpha=0.9, train_passes=32, batch_size=4096,
+                    min_steps_per_pass=16, max_steps_per_pass=64, max_records=3,
+                    box_center=(0.0, 0.0, 0.0), box_radius=1.0, indirect_clamp=1.0,
+                    rr_boost=4.0)
+
+    def __init__(self, ctx: Context, **cfg):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        c = dict(self.DEFAULTS)
+        c.update(cfg)
+        self.config = c
+        k = _abi.GuideConfig()
+        k.seed = int(c["seed"])
+        k.learning_rate = float(c["learning_rate"])
+        k.alpha = float(c["alpha"])
+        k.train_passes = int(c["train_passes"])
+        k.batch_size = int(c["batch_size"])
+        k.min_steps_per_pass = int(c["min_steps_per_pass"])
+        k.max_steps_per_pass = int(c["max_steps_per_pass"])
+        k.max_records = int(c["max_records"])
+        k.box_center[:] = [float(x) for x in c["box_center"]]
+        k.box_radius = float(c["box_radius"])
+        k.indirect_clamp = float(c["indirect_clamp"])
+        k.rr_boost = float(c["rr_boost"])
+        h = C.c_void_p()
+        check(self.lib.mupsi_guide_create(ctx.handle, C.byref(k), C.byref(h)), "mupsi_guide_create")
+        self.handle = h
+        ctx._guides.add(self)
+
+    @classmethod
+    def for_scene(cls, ctx: Context, scene: dict, **cfg) -> "Guide":
+        """Encoding box around the GP medium of a scene dict (mean sphere or mesh)."""
+        gm = scene.get("gp_medium", {})
+        if gm.get("mean_type", "sphere") == "sphere":
+            c = [float(x) for x in gm.get("mean_center", [0, 0, 0])]
+            r = 2.0 * float(gm.get("mean_radius", 1.0))
+        else:
+            v = np.asarray(gm["mesh"]["vertices"], np.float64)
+            tr = gm.get("transform", {})
+            v = v * float(tr.get("scale", 1.0)) + np.asarray(tr.get("position", [0, 0, 0]))
+            lo, hi = v.min(0), v.max(0)
+            c = list((lo + hi) / 2)
+            r = 1.5 * float(np.linalg.norm(hi - lo) / 2)
+        cfg.setdefault("box_center", c)
+        cfg.setdefault("box_radius", r)
+        return cls(ctx, **cfg)
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(_abi.GUIDE_NUM_PARAMS, np.float32)
+        check(self.lib.mupsi_guide_get_params(self.handle, out.ctypes.data_as(C.c_void_p)),
+              "mupsi_guide_get_params")
+        return out
+
+    def set_params(self, theta: np.ndarray) -> None:
+        theta = np.ascontiguousarray(theta, np.float32)
+        assert theta.size == _abi.GUIDE_NUM_PARAMS
+        check(self.lib.mupsi_guide_set_params(self.handle, theta.ctypes.data_as(C.c_void_p)),
+              "mupsi_guide_set_params")
+
+    def stats(self) -> dict:
+        s = _abi.GuideStats()
+        check(self.lib.mupsi_guide_stats_get(self.handle, C.byref(s)), "mupsi_guide_stats_get")
+        return {name: getattr(s, name) for name, _ in s._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            if self.ctx.handle:
+                self.lib.mupsi_guide_free(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -70,6 +70,23 @@ class Stats(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("learning_rate", C.c_float), ("alpha", C.c_float),
+                ("train_passes", C.c_int32), ("batch_size", C.c_int32),
+                ("min_steps_per_pass", C.c_int32), ("max_steps_per_pass", C.c_int32),
+                ("max_records", C.c_int32),
+                ("box_center", C.c_float * 3), ("box_radius", C.c_float),
+                ("indirect_clamp", C.c_float), ("rr_boost", C.c_float)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("train_passes_done", C.c_int64), ("train_steps", C.c_int64),
+                ("train_samples", C.c_int64), ("last_loss", C.c_double),
+                ("train_ms", C.c_double), ("infer_ms", C.c_double)]
+
+
+GUIDE_NUM_PARAMS = 13097
+
 VP = C.c_void_p
 I32, I64 = C.c_int32, C.c_int64
 
@@ -101,6 +118,18 @@ SIGNATURES = {
     "mupsi_mesh_closest": (I32, [VP, I32, VP, VP, VP, I64, VP, VP, VP]),
     "mupsi_mesh_inside": (I32, [VP, I32, VP, VP, VP, I64, VP]),
     "mupsi_bvh_build": (I32, [VP, I32, VP, I32, VP, C.POINTER(I32), VP, C.POINTER(C.c_uint32)]),
+    "mupsi_guide_create": (I32, [VP, C.POINTER(GuideConfig), C.POINTER(VP)]),
+    "mupsi_guide_free": (None, [VP]),
+    "mupsi_guide_get_params": (I32, [VP, VP]),
+    "mupsi_guide_set_params": (I32, [VP, VP]),
+    "mupsi_guide_stats_get": (I32, [VP, C.POINTER(GuideStats)]),
+    "mupsi_scene_set_guide": (I32, [VP, VP]),
+    "mupsi_scene_guide_records": (I32, [VP, VP, VP, VP, I64, C.POINTER(I64)]),
+    "mupsi_guide_forward": (I32, [VP, VP, I64, VP, VP]),
+    "mupsi_guide_gradient": (I32, [VP, VP, VP, VP, I64, VP, C.POINTER(C.c_double)]),
+    "mupsi_guide_train_step": (I32, [VP, VP, VP, VP, I64]),
+    "mupsi_guide_encode": (I32, [VP, VP, VP, VP, I64, VP]),
+    "mupsi_guide_sample": (I32, [VP, VP, VP, VP, I64, VP, VP, VP]),
 }
 
 _lib = None

@@ -156,3 +156,52 @@ def mesh_inside(scene, prim: int, p):
     check(scene.lib.mupsi_mesh_inside(scene.handle, prim, _dev(px), _dev(py), _dev(pz), n, _dev(out)),
           "mupsi_mesh_inside")
     return out
+
+
+# ---------------------------------------------------------------- guiding network
+
+def guide_forward(guide, feats):
+    """feats (n, 32) f32 -> (raw (n, 41), mixture (n, 44)) from the MFMA forward kernel."""
+    torch = _torch()
+    n = feats.shape[0]
+    raw = torch.empty((n, 41), dtype=torch.float32, device=feats.device)
+    mix = torch.empty((n, 44), dtype=torch.float32, device=feats.device)
+    check(guide.lib.mupsi_guide_forward(guide.handle, _dev(feats), n, _dev(raw), _dev(mix)),
+          "mupsi_guide_forward")
+    return raw, mix
+
+
+def guide_gradient(guide, feats, dirs, weights):
+    """Mean loss and its gradient (numpy, 13032) over the batch."""
+    import numpy as np
+    grad = np.zeros(A.GUIDE_NUM_PARAMS, np.float32)
+    loss = C.c_double()
+    check(guide.lib.mupsi_guide_gradient(guide.handle, _dev(feats), _dev(dirs), _dev(weights),
+                                         feats.shape[0], grad.ctypes.data_as(C.c_void_p),
+                                         C.byref(loss)), "mupsi_guide_gradient")
+    return loss.value, grad
+
+
+def guide_train_step(guide, feats, dirs, weights):
+    check(guide.lib.mupsi_guide_train_step(guide.handle, _dev(feats), _dev(dirs), _dev(weights),
+                                           feats.shape[0]), "mupsi_guide_train_step")
+
+
+def guide_encode(guide, p, nrm, wo):
+    torch = _torch()
+    out = torch.empty((p.shape[0], 32), dtype=torch.float32, device=p.device)
+    check(guide.lib.mupsi_guide_encode(guide.handle, _dev(p), _dev(nrm), _dev(wo), p.shape[0],
+                                       _dev(out)), "mupsi_guide_encode")
+    return out
+
+
+def guide_sample(guide, mixture, nrm, u):
+    """The tracer's guided Lambertian sample on explicit uniforms u (n, 4)."""
+    torch = _torch()
+    n = mixture.shape[0]
+    d = torch.empty((n, 3), dtype=torch.float32, device=mixture.device)
+    q = torch.empty(n, dtype=torch.float32, device=mixture.device)
+    w = torch.empty(n, dtype=torch.float32, device=mixture.device)
+    check(guide.lib.mupsi_guide_sample(guide.handle, _dev(mixture), _dev(nrm), _dev(u), n,
+                                       _dev(d), _dev(q), _dev(w)), "mupsi_guide_sample")
+    return d, q, w

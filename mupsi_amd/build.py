@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libmupsi_hip.so"
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "scene_host.cpp", "host_bvh.cpp", "scene_json.cpp"]
+SOURCES = ["kernels.hip", "guide.hip", "scene_host.cpp", "host_bvh.cpp", "scene_json.cpp"]
 
 # -ffp-contract=off: the only fused multiply-adds are explicit fmaf() calls, which
 # fix the float operation order the bit-exact paths rely on.

@@ -114,6 +114,9 @@ struct DevMesh {
   const float* fnormals;
   const float* texcoords;    // or null
   const int32_t* face_tex;   // or null
+  const float* verts;        // local vertex positions (light sampling)
+  const float* face_cdf;     // normalised cumulative face areas (light sampling)
+  float total_area;          // local-space area (Mesh::prepareForRender)
   int n_nodes, n_faces;
   uint32_t root;
   DevXf xf;

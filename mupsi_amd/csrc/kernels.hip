@@ -25,8 +25,10 @@
 
 #include "../../include/mupsi.h"
 #include "device.h"
+#include "guide_host.h"
 #include "host_bvh.h"
 #include "march_cached.h"
+#include "runtime.h"
 #include "scene_host.h"
 
 using namespace mupsi;
@@ -37,11 +39,15 @@ using namespace mupsi::dev;
 constexpr uint32_t kBounceMask = 0xffffu;
 constexpr uint32_t kFlagMedium = 1u << 16;
 constexpr uint32_t kFlagCond = 1u << 17;
+constexpr uint32_t kRecShift = 18;  // guided vertices recorded for training (3 bits)
+constexpr uint32_t kRecMask = 7u << kRecShift;
+constexpr uint32_t kFlagOpen = 1u << 21;  // the last record still collects its direct radiance
 
 struct PathSoA {
   uint64_t* rng;
   float *ox, *oy, *oz, *dx, *dy, *dz, *nt;
   float *Tx, *Ty, *Tz, *Lx, *Ly, *Lz;
+  float* mis;  // guided tracer: product of the MIS factors (cos/pi)/q of the path
   uint32_t* flags;
   float *cx, *cy, *cz, *cu, *cgx, *cgy, *cgz;
   uint32_t* mseed;
@@ -65,6 +71,19 @@ __device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int&
     y = pix / s.W;
     k = s.k_begin + pid / s.npix;
   }
+}
+
+// Slot of this lane among the wave's lanes with pred, after one atomic per wave
+// on count (every lane of the wave must call it); -1 when pred is false.
+__device__ __forceinline__ int wave_slot(bool pred, int* count) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
 // Append value to queue if pred; one atomic per wave.
@@ -161,17 +180,20 @@ __device__ F3 bsdf_weight(int bsdf, F3 alb, F3 wo, F3 n, F3 wi) {
 }
 // Bsdf::sample (bsdf.cpp:23-37,51-55,75-79).  The Lambertian sample takes
 // phi, sqrt, cos and sin in double as the reference's unqualified calls do.
+__device__ __forceinline__ F3 lambert_dir(F3 n, float r1, float r2) {
+  const double phi = (double)(float)(2 * M_PI * (double)r1);
+  const float cosT = (float)sqrt((double)(1 - r2));
+  const float sinT = (float)sqrt((double)r2);
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  const F3 lw = mk((float)((double)sinT * cp), (float)((double)sinT * sp), cosT);
+  return to_global(make_frame(n), lw);
+}
 __device__ void bsdf_sample(int bsdf, F3 alb, F3 wo, F3 n, Pcg& rng, F3& wi, F3& weight) {
   if (bsdf == MUPSI_BSDF_LAMBERTIAN) {
     const float r1 = rng.next1D();
     const float r2 = rng.next1D();
-    const double phi = (double)(float)(2 * M_PI * (double)r1);
-    const float cosT = (float)sqrt((double)(1 - r2));
-    const float sinT = (float)sqrt((double)r2);
-    double sp, cp;
-    sincos(phi, &sp, &cp);
-    const F3 lw = mk((float)((double)sinT * cp), (float)((double)sinT * sp), cosT);
-    wi = to_global(make_frame(n), lw);
+    wi = lambert_dir(n, r1, r2);
     weight = alb;
   } else if (bsdf == MUPSI_BSDF_NULL) {
     wi = neg(wo);
@@ -180,6 +202,41 @@ __device__ void bsdf_sample(int bsdf, F3 alb, F3 wo, F3 n, Pcg& rng, F3& wi, F3&
     const float k = 2.0f * dot(wo, n);
     wi = mk(fmaf(k, n.x, -wo.x), fmaf(k, n.y, -wo.y), fmaf(k, n.z, -wo.z));
     weight = mk(1, 1, 1);
+  }
+}
+
+// Guided Lambertian sample (one-sample MIS of the guide mixture m and the cosine
+// lobe, with the vertex's learned selection probability alpha = m[40]):
+// q = alpha p_guide + (1 - alpha) cos/pi, and the albedo-free weight
+// f cos / q = (cos/pi) / q (0 below the surface).  u0 picks the technique; the
+// guide branch takes u1..u3, the cosine branch takes (u1, u2) as its (r1, r2).
+__device__ __forceinline__ void guided_eval(const float* m, F3 n, F3 wi, float& q, float& wfac) {
+  const float alpha = m[guide::kAlphaIdx];
+  const float cosv = dot(wi, n);
+  const float pg = guide::mixture_pdf(m, wi.x, wi.y, wi.z);
+  const float kInvPi = 0.318309886183790671538f;
+  const float pb = fmaxf(cosv, 0.0f) * kInvPi;
+  q = alpha * pg + (1.0f - alpha) * pb;
+  wfac = (cosv > 0.0f && q > 0.0f) ? (cosv * kInvPi) / q : 0.0f;
+}
+__device__ __forceinline__ F3 guided_dir(const float* m, F3 n, float u0, float u1, float u2,
+                                         float u3) {
+  if (u0 < m[guide::kAlphaIdx]) {
+    F3 w;
+    guide::mixture_sample(m, u1, u2, u3, w.x, w.y, w.z);
+    return w;
+  }
+  return lambert_dir(n, u1, u2);
+}
+__device__ __forceinline__ void load_mix(const float* src, float* m) {
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+  for (int r = 0; r < guide::kMix / 4; ++r) {
+    const float4 v = s4[r];
+    m[4 * r] = v.x;
+    m[4 * r + 1] = v.y;
+    m[4 * r + 2] = v.z;
+    m[4 * r + 3] = v.w;
   }
 }
 
@@ -330,13 +387,46 @@ __device__ bool sphere_sample_direct(const DevPrim& pr, F3 pw, Pcg& rng, LightSa
   return true;
 }
 
+// Mesh::sampleDirect (mesh.cpp:252-296): face by std::lower_bound on the area
+// CDF, a point by sqrt(r1) barycentrics, pdf 1/total local area (area measure).
+__device__ bool mesh_sample_direct(const DevPrim& pr, const DevMesh& m, F3 p, Pcg& rng,
+                                   LightSample& ls) {
+  if (m.n_faces <= 0 || !(m.total_area > 0.0f) || !m.face_cdf) return false;
+  const float r = rng.next1D();
+  int lo = 0, hi = m.n_faces;
+  while (lo < hi) {  // first face with cdf >= r
+    const int mid = (lo + hi) >> 1;
+    if (m.face_cdf[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  const int fi = lo >= m.n_faces ? m.n_faces - 1 : lo;
+  const float r1 = rng.next1D();
+  const float r2 = rng.next1D();
+  const float sr1 = sqrtf(r1);
+  const float bu = 1.0f - sr1, bv = r2 * sr1;
+  const int32_t* f = m.faces + 3 * fi;
+  const F3 v0 = ld3(m.verts, f[0]), v1 = ld3(m.verts, f[1]), v2 = ld3(m.verts, f[2]);
+  const F3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+  const F3 pl = mk(fmaf(e2.x, bv, fmaf(e1.x, bu, v0.x)), fmaf(e2.y, bv, fmaf(e1.y, bu, v0.y)),
+                   fmaf(e2.z, bv, fmaf(e1.z, bu, v0.z)));
+  const F3 pw = m.xf.identity ? pl : xf_to_world(m.xf, pl);
+  const F3 dir = sub(pw, p);
+  const float dist = sqrtf(sq3(dir));
+  ls.d = divs(dir, dist);
+  ls.dist = dist;
+  ls.pdf = 1.0f / m.total_area;
+  const F3 em = pr.has_em ? mk(pr.em[0], pr.em[1], pr.em[2]) : mk(0, 0, 0);
+  ls.w = divs(em, fmaxf(ls.pdf, 1e-6f));
+  return true;
+}
+
 // Scene::chooseLight (scene.cpp:50-61): no draw when the scene has no emitter.
 __device__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls) {
   if (S.n_prims == 0 || S.n_lights == 0) return false;
   const int idx = (int)(rng.nextI() % (uint32_t)S.n_lights);
   const DevPrim& pr = S.prims[S.lights[idx]];
-  if (pr.type != MUPSI_PRIM_SPHERE) return false;
-  return sphere_sample_direct(pr, p, rng, ls);
+  if (pr.type == MUPSI_PRIM_SPHERE) return sphere_sample_direct(pr, p, rng, ls);
+  return mesh_sample_direct(pr, S.meshes[pr.mesh], p, rng, ls);
 }
 
 // ====================================================================== device: march
@@ -886,6 +976,7 @@ __global__ __launch_bounds__(256) void k_init(DevScene S, InitArgs a) {
   P.dy[pid] = d.y;
   P.dz[pid] = d.z;
   P.nt[pid] = 0.0f;
+  P.mis[pid] = 1.0f;
   P.Tx[pid] = 1.0f;
   P.Ty[pid] = 1.0f;
   P.Tz[pid] = 1.0f;
@@ -896,6 +987,24 @@ __global__ __launch_bounds__(256) void k_init(DevScene S, InitArgs a) {
   a.qA[pid] = pid;
 }
 
+// Guided sampling at GP-medium hits, and the training records of a guided vertex:
+// rec[(pid * K + j) * 13 ..] = sampled direction (3), its pdf q, the path
+// radiance after the vertex's light sample (3), the throughput after the vertex's
+// weight and Russian roulette (3), and the path radiance when the next
+// scattering vertex is reached (3; null and mirror surfaces pass light through,
+// an emitter's own emission there still counts).  The difference of the two
+// radiances over the throughput is the radiance that arrived directly along the
+// sampled direction.
+constexpr int kRecFloats = 13;
+struct GuideShade {
+  int on;
+  float rr_boost;    // guided roulette: survive with min(1, rr_boost max(T))
+  const float* mix;  // kMix floats per path, written by the inference of this bounce
+  int record;
+  int K;
+  float* rec;
+};
+
 struct ShadeArgs {
   PathSoA P;
   PathSrc src;
@@ -905,8 +1014,21 @@ struct ShadeArgs {
   int* nA2;
   int* qM2;
   int* nM2;
+  GuideShade gs;
 };
 
+// The guided tracer keeps T as the plain tracer's throughput (albedo weights and
+// the reference's Russian roulette on max(T), trace.cpp:68-72,106-111) and the
+// product of its MIS factors (cos/pi)/q in `mis`; radiance is added with T * mis.
+// Roulette on T * mis would kill a path in proportion to 1/q -- exactly undoing
+// the higher probability with which the guide drew it.  Roulette on T alone
+// does not depend on the sampled direction, so the estimator stays unbiased.
+// Any survival probability that does not depend on the sample keeps it so; the
+// guided tracer may raise it to min(1, rr_boost max(T)) to trade time for noise.
+__device__ __forceinline__ float rr_survival(const GuideShade& gs, F3 T) {
+  const float m = max3(T);
+  return gs.on ? fminf(1.0f, gs.rr_boost * m) : m;
+}
 // One iteration of the bounce loop of PathTracer::trace (trace.cpp:138-210) after
 // the march: handleVolume + sampleCondition for medium hits, else
 // Scene::intersect + handleSurface + medium switch, or the skydrome.
@@ -931,13 +1053,32 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
   Pcg rng{P.rng[pid]};
   F3 T = mk(P.Tx[pid], P.Ty[pid], P.Tz[pid]);
   F3 L = mk(P.Lx[pid], P.Ly[pid], P.Lz[pid]);
+  float mis = a.gs.on ? P.mis[pid] : 1.0f;
+  auto Te = [&]() { return a.gs.on ? mul(T, mis) : T; };  // throughput radiance is added with
   Ray ray{mk(P.ox[pid], P.oy[pid], P.oz[pid]), mk(P.dx[pid], P.dy[pid], P.dz[pid]), P.nt[pid],
           3.402823466e+38f};
   const bool single = S.gp_mode != MUPSI_GP_RENEWAL_PLUS;
   bool alive = true, hasHit = false, exited = true;
+  bool rec_now = false;  // this event is a guided vertex to record
+  F3 rec_wi = mk(0, 0, 0);
+  float rec_q = 0.0f;
+  F3 rec_L = mk(0, 0, 0);
+
+  uint32_t nrec = (fl & kRecMask) >> kRecShift;
+  bool open = (fl & kFlagOpen) != 0;
+  auto close_record = [&](F3 Lc) {
+    if (a.gs.record && open && nrec > 0) {
+      float* r = a.gs.rec + ((size_t)pid * a.gs.K + (nrec - 1)) * kRecFloats;
+      r[10] = Lc.x;
+      r[11] = Lc.y;
+      r[12] = Lc.z;
+    }
+    open = false;
+  };
 
   if (medium && !S.skip_medium && P.mhit[pid]) {
     // handleVolume (trace.cpp:77-116)
+    close_record(L);
     const uint32_t iseed = P.mseed[pid];
     const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
     F3 gr;
@@ -953,18 +1094,41 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
       const float trans = march_serial(S, sr, iseed, cur) ? 0.0f : 1.0f;
       const F3 f = bsdf_weight(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, ls.d);
       const float ac = fabsf(dot(ls.d, nrm));
-      L = mk(fmaf(trans * (T.x * (f.x * ls.w.x)), ac, L.x),
-             fmaf(trans * (T.y * (f.y * ls.w.y)), ac, L.y),
-             fmaf(trans * (T.z * (f.z * ls.w.z)), ac, L.z));
+      const F3 Tt = Te();
+      L = mk(fmaf(trans * (Tt.x * (f.x * ls.w.x)), ac, L.x),
+             fmaf(trans * (Tt.y * (f.y * ls.w.y)), ac, L.y),
+             fmaf(trans * (Tt.z * (f.z * ls.w.z)), ac, L.z));
     }
     F3 wi, wt;
-    bsdf_sample(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, rng, wi, wt);
+    float q = 0.0f;
+    if (a.gs.on) {
+      float m[guide::kMix];
+      load_mix(a.gs.mix + (size_t)pid * guide::kMix, m);
+      const float u0 = rng.next1D();
+      float u1, u2, u3 = 0.0f;
+      u1 = rng.next1D();
+      u2 = rng.next1D();
+      if (u0 < m[guide::kAlphaIdx]) u3 = rng.next1D();
+      wi = guided_dir(m, nrm, u0, u1, u2, u3);
+      float wfac;
+      guided_eval(m, nrm, wi, q, wfac);
+      wt = alb;
+      mis *= wfac;
+    } else {
+      bsdf_sample(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, rng, wi, wt);
+    }
     T = cmul(T, wt);
-    const float surv = max3(T);
-    if (rng.next1D() > surv) {
+    const float surv = rr_survival(a.gs, T);
+    if ((a.gs.on && !(mis > 0.0f)) || rng.next1D() > surv) {  // mis 0: below the surface
       alive = false;
     } else {
       T = divs(T, surv);
+      if (a.gs.record) {
+        rec_now = true;
+        rec_wi = wi;
+        rec_q = q;
+        rec_L = L;
+      }
       if (!single) {
         const uint32_t ns =
             constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u, (uint32_t)(bounce + 1)));
@@ -993,6 +1157,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
       // handleSurface (trace.cpp:32-75)
       const DevPrim& pr = S.prims[hp];
       const F3 wo = neg(ray.d), nrm = Ng;
+      const F3 L0 = L;
       F3 alb = mk(1, 1, 1);
       if (pr.bsdf == MUPSI_BSDF_LAMBERTIAN)
         alb = pr.albedo.rgb ? tex_at(pr.albedo, uu, vv, nullptr) : mk(0.8f, 0.8f, 0.8f);
@@ -1004,19 +1169,25 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
         if (!scene_occluded(S, sr)) {
           const F3 w = bsdf_weight(pr.bsdf, alb, wo, nrm, ls.d);
           const float ac = fabsf(dot(ls.d, nrm));
-          L = mk(fmaf(T.x * (w.x * ls.w.x), ac, L.x), fmaf(T.y * (w.y * ls.w.y), ac, L.y),
-                 fmaf(T.z * (w.z * ls.w.z), ac, L.z));
+          const F3 Tt = Te();
+          L = mk(fmaf(Tt.x * (w.x * ls.w.x), ac, L.x), fmaf(Tt.y * (w.y * ls.w.y), ac, L.y),
+                 fmaf(Tt.z * (w.z * ls.w.z), ac, L.z));
         }
       }
       F3 wi, wt;
       bsdf_sample(pr.bsdf, alb, wo, nrm, rng, wi, wt);
+      F3 Lc = L0;
       if (pr.has_em) {
         const float mm = fmaxf(0.0f, dot(nrm, wi));
-        L = mk(fmaf(T.x, pr.em[0] * mm, L.x), fmaf(T.y, pr.em[1] * mm, L.y),
-               fmaf(T.z, pr.em[2] * mm, L.z));
+        const F3 Tt = Te();
+        L = mk(fmaf(Tt.x, pr.em[0] * mm, L.x), fmaf(Tt.y, pr.em[1] * mm, L.y),
+               fmaf(Tt.z, pr.em[2] * mm, L.z));
+        Lc = mk(fmaf(Tt.x, pr.em[0] * mm, L0.x), fmaf(Tt.y, pr.em[1] * mm, L0.y),
+                fmaf(Tt.z, pr.em[2] * mm, L0.z));
       }
+      if (pr.bsdf == MUPSI_BSDF_LAMBERTIAN) close_record(Lc);
       T = cmul(T, wt);
-      const float surv = max3(T);
+      const float surv = rr_survival(a.gs, T);
       if (rng.next1D() > surv) {
         alive = false;
       } else {
@@ -1030,7 +1201,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
       }
     } else if (S.sky.rgb) {
       const F3 e = sky_lookup(S, ray.d, nullptr);
-      L = mk(fmaf(T.x, e.x, L.x), fmaf(T.y, e.y, L.y), fmaf(T.z, e.z, L.z));
+      const F3 Tt = Te();
+      L = mk(fmaf(Tt.x, e.x, L.x), fmaf(Tt.y, e.y, L.y), fmaf(Tt.z, e.z, L.z));
     }
   }
 
@@ -1038,11 +1210,28 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
   P.Lx[pid] = L.x;
   P.Ly[pid] = L.y;
   P.Lz[pid] = L.z;
+  if (cont && rec_now && (int)nrec < a.gs.K) {
+    float* r = a.gs.rec + ((size_t)pid * a.gs.K + nrec) * kRecFloats;
+    r[0] = rec_wi.x;
+    r[1] = rec_wi.y;
+    r[2] = rec_wi.z;
+    r[3] = rec_q;
+    r[4] = rec_L.x;
+    r[5] = rec_L.y;
+    r[6] = rec_L.z;
+    const F3 Tt = Te();  // the throughput radiance after this vertex is added with
+    r[7] = Tt.x;
+    r[8] = Tt.y;
+    r[9] = Tt.z;
+    ++nrec;
+    open = true;
+  }
   if (cont) {
     P.rng[pid] = rng.s;
     P.Tx[pid] = T.x;
     P.Ty[pid] = T.y;
     P.Tz[pid] = T.z;
+    if (a.gs.on) P.mis[pid] = mis;
     P.ox[pid] = ray.o.x;
     P.oy[pid] = ray.o.y;
     P.oz[pid] = ray.o.z;
@@ -1050,7 +1239,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
     P.dy[pid] = ray.d.y;
     P.dz[pid] = ray.d.z;
     P.nt[pid] = ray.nearT;
-    P.flags[pid] = (uint32_t)bounce | (medium ? kFlagMedium : 0u) | (cur.active ? kFlagCond : 0u);
+    P.flags[pid] = (uint32_t)bounce | (medium ? kFlagMedium : 0u) | (cur.active ? kFlagCond : 0u) |
+                   (nrec << kRecShift) | (open ? kFlagOpen : 0u);
     if (cur.active) {
       P.cx[pid] = cur.C.x;
       P.cy[pid] = cur.C.y;
@@ -1068,6 +1258,150 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
   }
   wave_push(cont, pid, a.qA2, a.nA2);
   wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
+}
+
+// Guided vertices of this bounce: every path of queue M whose march hit the
+// medium gets its hit normal (the same evaluation k_shade makes), its network
+// input features (row gi of feat, gi = its slot in queue G) and, when recording,
+// a copy of the features in its training slot (pid * K + j).
+struct GuidePrepArgs {
+  PathSoA P;
+  const int* qM;
+  const int* nM;
+  int* qG;
+  int* nG;
+  float* feat;
+  int record;
+  int K;
+  float* rec_feat;
+  guide::EncodeBox box;
+};
+__global__ __launch_bounds__(256) void k_guide_prep(DevScene S, GuidePrepArgs a) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *a.nM;
+  const PathSoA& P = a.P;
+  int pid = 0;
+  bool act = false;
+  if (qi < n) {
+    pid = a.qM[qi];
+    act = P.mhit[pid] != 0;
+  }
+  const int gi = wave_slot(act, a.nG);
+  if (!act) return;
+  const uint32_t fl = P.flags[pid];
+  Cond cur;
+  cur.active = (fl & kFlagCond) != 0;
+  if (cur.active) {
+    cur.C = mk(P.cx[pid], P.cy[pid], P.cz[pid]);
+    cur.u = P.cu[pid];
+    cur.g = mk(P.cgx[pid], P.cgy[pid], P.cgz[pid]);
+  }
+  const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
+  F3 gr;
+  field_eval<true>(S, Pp, P.mseed[pid], cur, &gr);
+  const F3 nrm = normalized(gr);
+  float f[guide::kIn];
+  guide::encode_features(a.box, Pp.x, Pp.y, Pp.z, nrm.x, nrm.y, nrm.z, -P.dx[pid], -P.dy[pid],
+                         -P.dz[pid], f);
+  float4* dst = reinterpret_cast<float4*>(a.feat + (size_t)gi * guide::kIn);
+#pragma unroll
+  for (int r = 0; r < guide::kIn / 4; ++r) dst[r] = make_float4(f[4 * r], f[4 * r + 1], f[4 * r + 2], f[4 * r + 3]);
+  a.qG[gi] = pid;
+  if (a.record) {
+    const int j = (int)((fl & kRecMask) >> kRecShift);
+    if (j < a.K) {
+      float4* rd = reinterpret_cast<float4*>(a.rec_feat + ((size_t)pid * a.K + j) * guide::kIn);
+#pragma unroll
+      for (int r = 0; r < guide::kIn / 4; ++r) rd[r] = make_float4(f[4 * r], f[4 * r + 1], f[4 * r + 2], f[4 * r + 3]);
+    }
+  }
+}
+
+// Training samples of a finished pass: for every recorded guided vertex, the
+// radiance that arrived along its sampled direction is the path radiance added
+// after the vertex divided by the throughput after it.  It splits into the part
+// collected before the next scattering vertex (direct: sky, emitters, through
+// null and mirror surfaces) and the rest (indirect, which carries the noise of
+// every later vertex); the training radiance is lum(direct) + min(lum(indirect),
+// clamp), and the sample's weight is that times max(cos, 0) / q (the product of
+// incident radiance and cosine, over the density the direction was drawn from).
+struct CollectArgs {
+  PathSoA P;
+  int n;
+  int K;
+  const float* rec;
+  const float* rec_feat;
+  float* tr_feat;  // ring of tr_cap records; this pass writes from tr_head on
+  float* tr_dir;
+  float* tr_w;
+  int* tr_count;
+  int64_t tr_head, tr_cap;
+  float indirect_clamp;
+};
+__global__ __launch_bounds__(256) void k_collect(CollectArgs a) {
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inr = pid < a.n;
+  const uint32_t pfl = inr ? a.P.flags[pid] : 0u;
+  const int nrec = (int)((pfl & kRecMask) >> kRecShift);
+  const bool last_open = (pfl & kFlagOpen) != 0;
+  const F3 Le = inr ? mk(a.P.Lx[pid], a.P.Ly[pid], a.P.Lz[pid]) : mk(0, 0, 0);
+  for (int j = 0; j < a.K; ++j) {  // uniform trip count: wave_slot needs every lane
+    bool ok = inr && j < nrec;
+    float w = 0.0f;
+    F3 wi = mk(0, 0, 0);
+    const size_t slot = (size_t)pid * a.K + j;
+    if (ok) {
+      const float* r = a.rec + slot * kRecFloats;
+      wi = mk(r[0], r[1], r[2]);
+      const float q = r[3];
+      // radiance collected before the next scattering vertex (the path's end if
+      // the record is still open)
+      const F3 Lc = (j == nrec - 1 && last_open) ? Le : mk(r[10], r[11], r[12]);
+      const F3 Li = mk(r[7] > 0.0f ? (Lc.x - r[4]) / r[7] : 0.0f,
+                       r[8] > 0.0f ? (Lc.y - r[5]) / r[8] : 0.0f,
+                       r[9] > 0.0f ? (Lc.z - r[6]) / r[9] : 0.0f);
+      const F3 Lind = mk(r[7] > 0.0f ? (Le.x - Lc.x) / r[7] : 0.0f,
+                         r[8] > 0.0f ? (Le.y - Lc.y) / r[8] : 0.0f,
+                         r[9] > 0.0f ? (Le.z - Lc.z) / r[9] : 0.0f);
+      const float lind = fmaxf((Lind.x + Lind.y + Lind.z) * (1.0f / 3.0f), 0.0f);
+      const float lum = (Li.x + Li.y + Li.z) * (1.0f / 3.0f) + fminf(lind, a.indirect_clamp);
+      const float* fe = a.rec_feat + slot * guide::kIn;
+      const float cosv = dot(wi, mk(fe[24], fe[25], fe[26]));
+      w = q > 0.0f ? lum * fmaxf(cosv, 0.0f) / q : 0.0f;
+      ok = w > 0.0f && isfinite(w);
+    }
+    const int tslot = wave_slot(ok, a.tr_count);
+    if (ok) {
+      const int64_t t = (a.tr_head + tslot) % a.tr_cap;
+      const float4* src = reinterpret_cast<const float4*>(a.rec_feat + slot * guide::kIn);
+      float4* dst = reinterpret_cast<float4*>(a.tr_feat + (size_t)t * guide::kIn);
+#pragma unroll
+      for (int r = 0; r < guide::kIn / 4; ++r) dst[r] = src[r];
+      a.tr_dir[3 * (size_t)t] = wi.x;
+      a.tr_dir[3 * (size_t)t + 1] = wi.y;
+      a.tr_dir[3 * (size_t)t + 2] = wi.z;
+      a.tr_w[t] = w;
+    }
+  }
+}
+
+// mupsi_guide_sample: the guided sample of k_shade on explicit uniforms.
+__global__ __launch_bounds__(256) void k_guide_sample(const float* mix, const float* nrm,
+                                                      const float* u, int64_t n, float* dir,
+                                                      float* pdf, float* weight) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float m[guide::kMix];
+  load_mix(mix + i * guide::kMix, m);
+  const F3 nn = ld3(nrm, i);
+  const F3 wi = guided_dir(m, nn, u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3]);
+  float q, wfac;
+  guided_eval(m, nn, wi, q, wfac);
+  dir[3 * i] = wi.x;
+  dir[3 * i + 1] = wi.y;
+  dir[3 * i + 2] = wi.z;
+  pdf[i] = q;
+  weight[i] = wfac;
 }
 
 // Per-pixel FP64 accumulation over the passes of one launch, in pass order.
@@ -1264,37 +1598,7 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
   in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
 }
 
-// ====================================================================== host: errors
-
-
-static mupsi_status fail(mupsi_status s, const std::string& msg) {
-  g_last_error = msg;
-  return s;
-}
-
-#define HIP_TRY(expr)                                                                        \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
-      return fail(e_ == hipErrorOutOfMemory ? MUPSI_ERR_NOMEM : MUPSI_ERR_HIP,               \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                        \
-  } while (0)
-
-static inline unsigned grid_for(int64_t n, int block = 256) {
-  return (unsigned)std::max<int64_t>(1, (n + block - 1) / block);
-}
-
 // ====================================================================== host: context
-
-struct mupsi_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string name;
-  int cus = 0;
-  int march_blocks = 0;         // persistent grid of k_march
-  int march_cached_blocks = 0;  // persistent grid of k_march_cached
-  std::vector<mupsi_scene*> scenes;  // live scenes, freed by mupsi_close
-};
 
 extern "C" const char* mupsi_status_string(mupsi_status s) {
   switch (s) {
@@ -1346,6 +1650,9 @@ extern "C" void mupsi_close(mupsi_ctx* ctx) {
   std::vector<mupsi_scene*> live;
   live.swap(ctx->scenes);
   for (mupsi_scene* s : live) destroy_scene(s);
+  std::vector<mupsi_guide*> guides;
+  guides.swap(ctx->guides);
+  for (mupsi_guide* g : guides) destroy_guide(g);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1363,42 +1670,33 @@ extern "C" mupsi_status mupsi_device_info(mupsi_ctx* ctx, char* name, int32_t na
 
 // ====================================================================== host: scene
 
-struct DeviceBuffers {
-  std::vector<void*> ptrs;
-  ~DeviceBuffers() {
-    for (void* p : ptrs) hipFree(p);
-  }
-  template <typename T>
-  hipError_t upload(const T* src, size_t count, const T** dst) {
-    *dst = nullptr;
-    if (count == 0 || !src) return hipSuccess;
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess) return e;
-    ptrs.push_back(p);
-    e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
-    *dst = static_cast<const T*>(p);
-    return e;
-  }
-  template <typename T>
-  hipError_t alloc(size_t count, T** dst) {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
-    if (e != hipSuccess) return e;
-    ptrs.push_back(p);
-    *dst = static_cast<T*>(p);
-    return hipSuccess;
-  }
-};
-
 struct PathBuffers {
   DeviceBuffers mem;
   int64_t capacity = 0;
   PathSoA P{};
   int* qA[2]{};
   int* qM[2]{};
-  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work]
+  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work, nG, training records]
   unsigned long long* stats = nullptr;
+};
+
+// Buffers of guided rendering: per-path mixture and features of the current
+// bounce, queue G, and the training records of one pass.
+struct GuideBuffers {
+  DeviceBuffers mem;
+  int64_t capacity = 0;      // paths
+  int64_t rec_capacity = 0;  // record slots (paths of a training pass x K)
+  float* mix = nullptr;      // capacity x 44
+  float* feat = nullptr;     // capacity x 32
+  int* qG = nullptr;
+  float* rec = nullptr;      // rec_capacity x 10
+  float* rec_feat = nullptr; // rec_capacity x 32
+  // training samples: a ring of tr_capacity records kept across the training
+  // passes (minibatches are drawn from all of it)
+  int64_t tr_capacity = 0, tr_head = 0, tr_size = 0;
+  float* tr_feat = nullptr;  // tr_capacity x 32
+  float* tr_dir = nullptr;   // tr_capacity x 3
+  float* tr_w = nullptr;     // tr_capacity
 };
 
 struct mupsi_scene {
@@ -1411,6 +1709,8 @@ struct mupsi_scene {
   double* accum = nullptr;
   int64_t sample_count = 0;
   std::unique_ptr<PathBuffers> pb;
+  std::unique_ptr<GuideBuffers> gb;
+  mupsi_guide* guide = nullptr;  // attached guide, or null (plain tracer)
   mupsi_stats last{};
 };
 
@@ -1426,6 +1726,9 @@ static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh*
   HIP_TRY(mem.upload(hm.fnormals.data(), hm.fnormals.size(), &m.fnormals));
   HIP_TRY(mem.upload(hm.texcoords.data(), hm.texcoords.size(), &m.texcoords));
   HIP_TRY(mem.upload(hm.face_tex.data(), hm.face_tex.size(), &m.face_tex));
+  HIP_TRY(mem.upload(hm.verts.data(), hm.verts.size(), &m.verts));
+  HIP_TRY(mem.upload(hm.face_cdf.data(), hm.face_cdf.size(), &m.face_cdf));
+  m.total_area = hm.total_area;
   m.n_nodes = (int)hm.bvh.nodes.size();
   m.n_faces = (int)(hm.faces.size() / 3);
   m.root = hm.bvh.root;
@@ -1517,7 +1820,7 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   DeviceBuffers& m = pb->mem;
   PathSoA& P = pb->P;
   HIP_TRY(m.alloc(n, &P.rng));
-  float** f32[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.nt, &P.Tx, &P.Ty, &P.Tz,
+  float** f32[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.nt, &P.Tx, &P.Ty, &P.Tz, &P.mis,
                    &P.Lx, &P.Ly, &P.Lz, &P.cx, &P.cy, &P.cz, &P.cu, &P.cgx, &P.cgy, &P.cgz,
                    &P.mt, &P.mpx, &P.mpy, &P.mpz};
   for (float** p : f32) HIP_TRY(m.alloc(n, p));
@@ -1532,6 +1835,61 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   HIP_TRY(m.alloc(8, &pb->stats));
   pb->capacity = n;
   sc->pb = std::move(pb);
+  return MUPSI_OK;
+}
+
+static mupsi_status ensure_guide_buffers(mupsi_scene* sc, int64_t n, int64_t rec_slots,
+                                         int64_t ring) {
+  GuideBuffers* gb = sc->gb.get();
+  if (gb && gb->capacity >= n && gb->rec_capacity >= rec_slots && gb->tr_capacity >= ring)
+    return MUPSI_OK;
+  const int64_t cap = std::max(n, gb ? gb->capacity : 0);
+  const int64_t rcap = std::max(rec_slots, gb ? gb->rec_capacity : 0);
+  const int64_t tcap = std::max(ring, gb ? gb->tr_capacity : 0);
+  sc->gb.reset();
+  auto g = std::make_unique<GuideBuffers>();
+  HIP_TRY(g->mem.alloc(cap * guide::kMix, &g->mix));
+  HIP_TRY(g->mem.alloc(cap * guide::kIn, &g->feat));
+  HIP_TRY(g->mem.alloc(cap, &g->qG));
+  HIP_TRY(g->mem.alloc(rcap * kRecFloats, &g->rec));
+  HIP_TRY(g->mem.alloc(rcap * guide::kIn, &g->rec_feat));
+  HIP_TRY(g->mem.alloc(tcap * guide::kIn, &g->tr_feat));
+  HIP_TRY(g->mem.alloc(tcap * 3, &g->tr_dir));
+  HIP_TRY(g->mem.alloc(tcap, &g->tr_w));
+  g->capacity = cap;
+  g->rec_capacity = rcap;
+  g->tr_capacity = tcap;
+  sc->gb = std::move(g);
+  return MUPSI_OK;
+}
+
+void detach_guide(mupsi_scene* s, const mupsi_guide* g) {
+  if (s && s->guide == g) s->guide = nullptr;
+}
+
+extern "C" mupsi_status mupsi_scene_set_guide(mupsi_scene* sc, mupsi_guide* g) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_guide: null scene");
+  if (g && g->ctx != sc->ctx) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_guide: guide of another context");
+  sc->guide = g;
+  if (g) g->stats.train_passes_done = 0;
+  if (sc->gb) sc->gb->tr_head = sc->gb->tr_size = 0;  // a fresh training history
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_scene_guide_records(mupsi_scene* sc, float* feats, float* dirs,
+                                                  float* weights, int64_t capacity, int64_t* count) {
+  if (!sc || !count) return fail(MUPSI_ERR_INVALID, "mupsi_scene_guide_records: null argument");
+  *count = 0;
+  if (!sc->gb || sc->gb->tr_size == 0) return MUPSI_OK;
+  const int64_t n = std::min<int64_t>(capacity, sc->gb->tr_size);
+  if (n <= 0) return MUPSI_OK;
+  if (!feats || !dirs || !weights) return fail(MUPSI_ERR_INVALID, "mupsi_scene_guide_records: null array");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  HIP_TRY(hipMemcpy(feats, sc->gb->tr_feat, n * guide::kIn * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(dirs, sc->gb->tr_dir, n * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(weights, sc->gb->tr_w, n * sizeof(float), hipMemcpyDeviceToHost));
+  *count = n;
   return MUPSI_OK;
 }
 
@@ -1561,9 +1919,15 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
 }
 
 // Run the wavefront over n paths described by src; leaves per-path radiance in P.L*.
+struct GuideRun {
+  mupsi_guide* g;
+  int record;
+  double ms;  // guide prep + inference time (HIP events)
+};
+
 static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
                                   const volatile int32_t* abort_flag, double* march_ms,
-                                  int* launches) {
+                                  int* launches, GuideRun* gr = nullptr) {
   mupsi_ctx* ctx = sc->ctx;
   hipStream_t st = ctx->stream;
   PathBuffers& pb = *sc->pb;
@@ -1573,8 +1937,10 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
   InitArgs ia{pb.P, src, ni, pb.qA[0]};
   hipLaunchKernelGGL(k_init, dim3(grid_for(n)), dim3(256), 0, st, S, ia);
   HIP_TRY(hipGetLastError());
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs, gevs;
   mupsi_status status = MUPSI_OK;
+  GuideBuffers* gb = gr ? sc->gb.get() : nullptr;
+  const int K = gr ? gr->g->cfg.max_records : 0;
   const int iters = std::max(1, S.max_bounce);
   for (int it = 0; it < iters; ++it) {
     if (abort_flag && *abort_flag) {
@@ -1614,10 +1980,27 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
       launch_march(ctx, S, ma, st);
       HIP_TRY(hipEventRecord(e1, st));
       HIP_TRY(hipGetLastError());
+      if (gr) {
+        hipEvent_t g0, g1;
+        HIP_TRY(hipEventCreate(&g0));
+        HIP_TRY(hipEventCreate(&g1));
+        gevs.emplace_back(g0, g1);
+        HIP_TRY(hipEventRecord(g0, st));
+        int* nG = pb.counters + 5;
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nG, 1, 0, -1);
+        GuidePrepArgs ga{pb.P, pb.qM[cur], nM, gb->qG, nG, gb->feat, gr->record, K, gb->rec_feat,
+                         gr->g->box()};
+        hipLaunchKernelGGL(k_guide_prep, dim3(grid_for(n)), dim3(256), 0, st, S, ga);
+        guide::launch_infer(gr->g, gb->feat, nG, 0, gb->qG, nullptr, gb->mix, st);
+        HIP_TRY(hipEventRecord(g1, st));
+        HIP_TRY(hipGetLastError());
+      }
     }
     // zero next queue counters
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nA2, 2, 0, -1);
-    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2};
+    GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
+    if (gr) gs = GuideShade{1, gr->g->cfg.rr_boost, gb->mix, gr->record, K, gb->rec};
+    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs};
     hipLaunchKernelGGL(k_shade, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
   }
@@ -1625,13 +2008,20 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
   double ms = 0;
   for (auto& e : evs) {
     float t = 0;
-    hipEventElapsedTime(&t, e.first, e.second);
+    (void)hipEventElapsedTime(&t, e.first, e.second);
     ms += t;
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
   }
   if (march_ms) *march_ms += ms;
   if (launches) *launches += (int)evs.size();
+  for (auto& e : gevs) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e.first, e.second);
+    if (gr) gr->ms += t;
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
   return status;
 }
 
@@ -1649,24 +2039,89 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
   if ((int64_t)ppl * npix > INT32_MAX) ppl = (int)std::max<int64_t>(1, INT32_MAX / npix);
   mupsi_status st = ensure_paths(sc, (int64_t)ppl * npix);
   if (st != MUPSI_OK) return st;
+  mupsi_guide* guide = sc->guide;
+  if (guide) {
+    const bool will_train = guide->stats.train_passes_done < guide->cfg.train_passes;
+    const int64_t per_pass = npix * guide->cfg.max_records;
+    const int64_t ring = std::min<int64_t>(std::max<int64_t>(per_pass, 8 << 20),
+                                           (int64_t)std::max(1, guide->cfg.train_passes) * per_pass);
+    st = ensure_guide_buffers(sc, (int64_t)ppl * npix, will_train ? per_pass : 1,
+                              will_train ? ring : 1);
+    if (st != MUPSI_OK) return st;
+  }
   PathBuffers& pb = *sc->pb;
-  HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), sc->ctx->stream));
+  hipStream_t strm = sc->ctx->stream;
+  HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), strm));
   mupsi_stats stats{};
   double march_ms = 0;
   int launches = 0;
-  for (int kb = k_begin; kb < k_begin + k_count; kb += ppl) {
+  for (int kb = k_begin; kb < k_begin + k_count;) {
     if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
-    const int kc = std::min(ppl, k_begin + k_count - kb);
+    // online training: each of the guide's first train_passes passes is traced
+    // alone, then its recorded vertices train the network before the next pass
+    const bool training = guide && guide->stats.train_passes_done < guide->cfg.train_passes;
+    const int kc = training ? 1 : std::min(ppl, k_begin + k_count - kb);
     PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr};
-    st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches);
+    GuideRun gr{guide, training ? 1 : 0, 0.0};
+    st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches,
+                       guide ? &gr : nullptr);
     if (st == MUPSI_ERR_ABORTED) return fail(st, "render aborted");
     if (st != MUPSI_OK) return st;
-    hipLaunchKernelGGL(k_accum, dim3(grid_for(npix)), dim3(256), 0, sc->ctx->stream, pb.P.Lx,
-                       pb.P.Ly, pb.P.Lz, (int)npix, kc, sc->accum);
+    hipLaunchKernelGGL(k_accum, dim3(grid_for(npix)), dim3(256), 0, strm, pb.P.Lx, pb.P.Ly, pb.P.Lz,
+                       (int)npix, kc, sc->accum);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+    if (guide) guide->stats.infer_ms += gr.ms;
+    if (training) {
+      GuideBuffers& gb = *sc->gb;
+      int* tr_count = pb.counters + 6;
+      hipEvent_t t0e, t1e;
+      HIP_TRY(hipEventCreate(&t0e));
+      HIP_TRY(hipEventCreate(&t1e));
+      HIP_TRY(hipEventRecord(t0e, strm));
+      hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, strm, tr_count, 1, 0, -1);
+      CollectArgs ca{pb.P, (int)npix, guide->cfg.max_records, gb.rec, gb.rec_feat, gb.tr_feat,
+                     gb.tr_dir, gb.tr_w, tr_count, gb.tr_head, gb.tr_capacity,
+                     guide->cfg.indirect_clamp};
+      hipLaunchKernelGGL(k_collect, dim3(grid_for(npix)), dim3(256), 0, strm, ca);
+      HIP_TRY(hipGetLastError());
+      int count = 0;
+      HIP_TRY(hipMemcpyAsync(&count, tr_count, sizeof(int), hipMemcpyDeviceToHost, strm));
+      HIP_TRY(hipStreamSynchronize(strm));
+      gb.tr_head = (gb.tr_head + count) % gb.tr_capacity;
+      gb.tr_size = std::min(gb.tr_size + count, gb.tr_capacity);
+      const int batch = guide->cfg.batch_size;
+      // about four passes over this pass's new records, drawn from the whole ring,
+      // and at least min_steps_per_pass steps
+      const int steps = (int)std::min<int64_t>(
+          guide->cfg.max_steps_per_pass,
+          std::max<int64_t>(guide->cfg.min_steps_per_pass, (4LL * count + batch - 1) / batch));
+      if (gb.tr_size > 0 && steps > 0) {
+        st = guide::train_random(guide, gb.tr_feat, gb.tr_dir, gb.tr_w, (int)gb.tr_size, steps,
+                                 batch, strm);
+        if (st != MUPSI_OK) return st;
+      }
+      HIP_TRY(hipEventRecord(t1e, strm));
+      HIP_TRY(hipEventSynchronize(t1e));
+      float tms = 0;
+      (void)hipEventElapsedTime(&tms, t0e, t1e);
+      (void)hipEventDestroy(t0e);
+      (void)hipEventDestroy(t1e);
+      guide->stats.train_ms += tms;
+      guide->stats.train_samples += count;
+      guide->stats.train_passes_done += 1;
+      if (gb.tr_size > 0 && steps > 0) {
+        float lp[256];
+        const int nb = std::max(1, std::min(batch / 64, 256));
+        HIP_TRY(hipMemcpy(lp, guide->loss_part, nb * sizeof(float), hipMemcpyDeviceToHost));
+        double ls = 0;
+        for (int i = 0; i < nb; ++i) ls += lp[i];
+        guide->stats.last_loss = ls / batch;
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(strm));
     sc->sample_count += kc;
     stats.paths += (int64_t)kc * npix;
+    kb += kc;
   }
   unsigned long long hs[8];
   HIP_TRY(hipMemcpy(hs, pb.stats, sizeof(hs), hipMemcpyDeviceToHost));
@@ -1750,6 +2205,21 @@ extern "C" mupsi_status mupsi_trace_paths(mupsi_scene* sc, const int32_t* x, con
 // ====================================================================== host: batch API
 
 static DevGP gp_from_params(const mupsi_gp_params& p) { return make_dev_gp(p); }
+
+extern "C" mupsi_status mupsi_guide_sample(mupsi_guide* g, const float* mixture, const float* nrm,
+                                           const float* u, int64_t n, float* dir, float* pdf,
+                                           float* weight) {
+  if (!g) return fail(MUPSI_ERR_INVALID, "mupsi_guide_sample: null guide");
+  if (n <= 0) return MUPSI_OK;
+  if (!mixture || !nrm || !u || !dir || !pdf || !weight)
+    return fail(MUPSI_ERR_INVALID, "mupsi_guide_sample: null array");
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  hipLaunchKernelGGL(k_guide_sample, dim3(grid_for(n)), dim3(256), 0, g->ctx->stream, mixture, nrm,
+                     u, n, dir, pdf, weight);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  return MUPSI_OK;
+}
 
 extern "C" mupsi_status mupsi_psi_eval(mupsi_ctx* ctx, const mupsi_gp_params* gp, const float* px,
                                        const float* py, const float* pz, const uint32_t* seed,

@@ -97,6 +97,21 @@ bool lower_mesh(const mupsi_mesh_desc& d, HostMesh* m, std::string* err) {
     m->fnormals[3 * i + 1] = ok ? n.y / len : 0.0f;
     m->fnormals[3 * i + 2] = ok ? n.z / len : 0.0f;
   }
+  // Mesh::prepareForRender (mesh.cpp:298-319): face areas 0.5 |e1 x e2|, the
+  // float running total, then every entry divided by the total.
+  m->verts.assign(d.vertices, d.vertices + 3LL * d.n_vertices);
+  m->face_cdf.resize(d.n_faces);
+  float total = 0.0f;
+  for (int i = 0; i < d.n_faces; ++i) {
+    const V v0 = vld(d.vertices + 3LL * d.faces[3 * i]);
+    const V v1 = vld(d.vertices + 3LL * d.faces[3 * i + 1]);
+    const V v2 = vld(d.vertices + 3LL * d.faces[3 * i + 2]);
+    total += 0.5f * std::sqrt(vsq(vcross(vsub(v1, v0), vsub(v2, v0))));
+    m->face_cdf[i] = total;
+  }
+  if (total > 0.0f)
+    for (float& c : m->face_cdf) c /= total;
+  m->total_area = total;
   m->bvh = build_bvh(d.vertices, d.faces, (uint32_t)d.n_faces);
   const std::vector<BvhNode>& nodes = m->bvh.nodes;
   m->order.assign(nodes.size(), 0);
@@ -259,6 +274,18 @@ mupsi_status lower_scene(const mupsi_scene_desc& d, HostScene* out, std::string*
       if (!lower_mesh(pd.mesh, &hm, err)) return MUPSI_ERR_INVALID;
       H.meshes.push_back(std::move(hm));
       pr.mesh = (int)H.meshes.size() - 1;
+      // Mesh emission: the reference parses "emission" for spheres only
+      // (controller.cpp:225-228); a mesh given one is lit and sampled exactly as
+      // its Mesh::getEmission / Mesh::sampleDirect would be (mesh.cpp:252-296).
+      pr.has_em = pd.has_emission ? 1 : 0;
+      for (int a = 0; a < 3; ++a) pr.em[a] = pd.emission[a];
+      if (pr.has_em) {
+        if (S.n_lights >= 16) {
+          *err = "more than 16 emissive primitives";
+          return MUPSI_ERR_UNSUPPORTED;
+        }
+        S.lights[S.n_lights++] = i;
+      }
       if (pd.bsdf == MUPSI_BSDF_LAMBERTIAN && texture_ok(pd.albedo)) {
         albedo.assign(pd.albedo.rgb, pd.albedo.rgb + 3LL * pd.albedo.width * pd.albedo.height);
         pr.albedo.w = pd.albedo.width;

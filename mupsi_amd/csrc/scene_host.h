@@ -23,6 +23,9 @@ struct HostMesh {
   std::vector<float> fnormals;     // n_faces x 3
   std::vector<float> texcoords;    // n_texcoords x 2 (may be empty)
   std::vector<int32_t> face_tex;   // n_faces x 3 (may be empty)
+  std::vector<float> verts;        // n_vertices x 3, local space
+  std::vector<float> face_cdf;     // n_faces, cumulative area / total
+  float total_area = 0.0f;
   dev::DevXf xf;
 };
 

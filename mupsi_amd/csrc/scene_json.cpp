@@ -604,6 +604,14 @@ extern "C" mupsi_status mupsi_scene_load_json(mupsi_ctx* ctx, const char* path, 
         fill_mesh(*m, pd.mesh);
         read_transform(pj.get("transform"), pd.mesh);
         store.meshes.push_back(std::move(m));
+        // "emission" on a mesh is ignored by the reference (controller.cpp:225-228);
+        // the scene key "extensions": {"mesh_emission": true} honours it
+        const Json* ext = j.get("extensions");
+        const Json* me = ext ? ext->get("mesh_emission") : nullptr;
+        if (pj.has("emission") && me && me->kind == Json::Bool && me->b) {
+          pd.has_emission = 1;
+          vec3(pj.get("emission"), pd.emission);
+        }
         if (pj.has("texture") && bs == "lambertian") {
           auto img = std::make_unique<Image>();
           if (!load_image(resolve(dir, str_v(pj.get("texture"), "")), *img, err)) {

@@ -216,6 +216,11 @@ def build_desc(scene: dict) -> tuple[A.SceneDesc, SceneHolder]:
         elif pj["type"] == "mesh":
             p.type = A.PRIM_MESH
             p.mesh = _mesh_desc(h, pj.get("mesh", pj.get("file")), pj.get("transform"))
+            # the reference ignores "emission" on meshes (controller.cpp:225-228); the
+            # "extensions": {"mesh_emission": true} scene key turns it on
+            if "emission" in pj and scene.get("extensions", {}).get("mesh_emission"):
+                p.has_emission = 1
+                p.emission[:] = [float(x) for x in pj["emission"]]
             if "texture" in pj and bsdf_name == "lambertian":
                 p.albedo = _texture(h, pj["texture"])
         else:

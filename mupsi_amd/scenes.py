@@ -4,8 +4,10 @@
   sun lobe of peak 50.
 - `icosphere`: a radius-1 icosphere subdivided n times, optionally displaced
   along the vertex normals by seeded 3-octave value noise.
-- `config_scene(i)`: the scene dicts of configurations 1-3 (sphere mean,
-  single realization / renewal_plus; displaced-icosphere mesh mean).
+- `config_scene(i)`: the scene dicts of configurations 1-3 and 5 (sphere mean,
+  single realization / renewal_plus; displaced-icosphere mesh mean; the
+  renewal_plus sphere scene with a mirror sphere and a small emissive quad,
+  the learned-guiding workload).
 """
 from __future__ import annotations
 
@@ -162,4 +164,28 @@ def config_scene(cfg: int, *, width: int | None = None, height: int | None = Non
                  "texture": np.full((1, 1, 3), 0.7, np.float32)},
             ],
         }
+    if cfg == 5:
+        sc = config_scene(2, width=width or 1024, height=height or 1024)
+        sc["spp"] = 256
+        # mesh emission is an extension of the reference schema (it ignores the key)
+        sc["extensions"] = {"mesh_emission": True}
+        sc["primitives"] = sc["primitives"] + [
+            # mirror sphere of radius 0.5 in view, 5 units in front of the camera
+            {"type": "sphere", "center": [0.9, -0.6, -5.0], "radius": 0.5, "bsdf": "specular"},
+            # emissive quad of area 0.01 (0.1 x 0.1), radiance 200, facing down (-y),
+            # 25 units above the top of the mean sphere, inside the medium sphere
+            {"type": "mesh", "mesh": emissive_quad([0.0, 95.0, -500.0], 0.1), "bsdf": "lambertian",
+             "emission": [200.0, 200.0, 200.0]},
+        ]
+        return sc
     raise ValueError(f"configuration {cfg} has no scene generator here")
+
+
+def emissive_quad(center, side: float) -> dict:
+    """Horizontal square of the given side centred at `center`, face normals -y."""
+    cx, cy, cz = (float(c) for c in center)
+    h = side / 2
+    v = np.array([[cx - h, cy, cz - h], [cx + h, cy, cz - h], [cx + h, cy, cz + h],
+                  [cx - h, cy, cz + h]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return {"vertices": v, "faces": f, "normals": None, "texcoords": None, "face_tex": None}

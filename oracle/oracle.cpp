@@ -1125,6 +1125,34 @@ static bool sphere_sample_direct(const Prim& pr, V3 pw, Pcg& rng, LightSample& l
   return true;
 }
 
+// Mesh::sampleDirect (mesh.cpp:252-296): std::lower_bound on the normalised area
+// CDF picks the face, (1 - sqrt(r1), r2 sqrt(r1)) the barycentrics; the pdf is
+// the area measure 1/totalArea (local space) with no cos/dist^2 conversion.
+static bool mesh_sample_direct(const Prim& pr, const Mesh& m, V3 p, Pcg& rng, LightSample& ls) {
+  if (m.cdf.empty() || !(m.total_area > 0.0f) || m.faces.empty()) return false;
+  const float r = rng.next1D();
+  size_t fi = (size_t)(std::lower_bound(m.cdf.begin(), m.cdf.end(), r) - m.cdf.begin());
+  if (fi >= m.faces.size()) fi = m.faces.size() - 1;
+  const float r1 = rng.next1D();
+  const float r2 = rng.next1D();
+  const float sr1 = sqrtf(r1);
+  const float bu = 1.0f - sr1, bv = r2 * sr1;
+  const Tri& f = m.faces[fi];
+  const V3 v0 = m.verts[f.a];
+  const V3 e1 = sub(m.verts[f.b], v0), e2 = sub(m.verts[f.c], v0);
+  const V3 pl = mk(fmaf(e2.x, bv, fmaf(e1.x, bu, v0.x)), fmaf(e2.y, bv, fmaf(e1.y, bu, v0.y)),
+                   fmaf(e2.z, bv, fmaf(e1.z, bu, v0.z)));
+  const V3 pw = m.xf.identity ? pl : xf_to_world(m.xf, pl);
+  const V3 dir = sub(pw, p);
+  const float dist = sqrtf(sq3(dir));
+  ls.d = divs(dir, dist);
+  ls.dist = dist;
+  ls.pdf = 1.0f / m.total_area;
+  const V3 em = pr.has_em ? pr.em : mk(0, 0, 0);
+  ls.w = divs(em, std::max(ls.pdf, 1e-6f));
+  return true;
+}
+
 // Scene::chooseLight (scene.cpp:50-61)
 static bool choose_light(const Scene& s, V3 p, Pcg& rng, LightSample& ls) {
   if (s.prims.empty()) return false;
@@ -1135,8 +1163,8 @@ static bool choose_light(const Scene& s, V3 p, Pcg& rng, LightSample& ls) {
   if (nl == 0) return false;
   int idx = (int)(rng.nextI() % (uint32_t)nl);
   const Prim& pr = s.prims[lights[idx]];
-  if (pr.type != MUPSI_PRIM_SPHERE) return false;  // mesh emission is not parsed
-  return sphere_sample_direct(pr, p, rng, ls);
+  if (pr.type == MUPSI_PRIM_SPHERE) return sphere_sample_direct(pr, p, rng, ls);
+  return mesh_sample_direct(pr, s.meshes[pr.mesh], p, rng, ls);
 }
 
 static const float kDefaultAlbedo = 0.8f;  // bsdf.cpp:11
@@ -1309,7 +1337,9 @@ static Scene* build_scene(const mupsi_scene_desc& d) {
     Prim pr;
     pr.type = pd.type;
     pr.bsdf = pd.bsdf;
-    pr.has_em = pd.type == MUPSI_PRIM_SPHERE && pd.has_emission;
+    // emission: the reference parses it for spheres only (controller.cpp:225-228);
+    // a mesh given one behaves as Mesh::getEmission / sampleDirect would
+    pr.has_em = pd.has_emission != 0;
     pr.em = mk(pd.emission[0], pd.emission[1], pd.emission[2]);
     pr.int_medium = pd.type == MUPSI_PRIM_SPHERE && pd.int_medium;
     if (pd.type == MUPSI_PRIM_SPHERE) {
