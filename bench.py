@@ -15,6 +15,13 @@ Reported next to it:
   cpu_baseline  the CPU restatement of the reference trace loop (OpenMP over
                 pixels, static schedule) on a bounded sample of the same
                 workload, timed on this host's cores.
+  guiding       the learned-guiding workload (the same sphere scene with a
+                mirror sphere and an emissive quad, 1024x1024): wall time for
+                the plain and the guided tracer to reach 1e-3 relative MSE
+                (mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)).
+                Both tracers are unbiased, so relMSE is the per-pixel variance of
+                the progressive average, measured from independent renders; the
+                guided time includes its 32 online-training passes.
 """
 from __future__ import annotations
 
@@ -60,12 +67,141 @@ def cpu_baseline(scene: dict, target_s: float = 15.0) -> dict:
     }
 
 
+def _chunk_stats(s, k0: int, chunks: int, spp: int):
+    """Per-pass relative variance v (mean over pixels and channels) and seconds
+    per pass, from `chunks` independent renders of `spp` passes."""
+    import numpy as np
+    import torch
+    ims = []
+    secs = 0.0
+    for c in range(chunks):
+        s.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.render_passes(k0 + c * spp, spp)
+        torch.cuda.synchronize()
+        secs += time.perf_counter() - t0
+        ims.append(s.image())
+    ims = np.array(ims)
+    R = ims.mean(0)
+    v = float(np.mean(ims.var(0, ddof=1) * spp / (R ** 2 + 0.01)))
+    return v, secs / (chunks * spp)
+
+
+def _passes_to_target(v_t: float, n_t: int, v_g: float, target: float) -> int:
+    """Smallest N >= n_t with (n_t v_t + (N - n_t) v_g) / N^2 <= target: the relMSE
+    of the progressive average whose first n_t passes have per-pass variance v_t."""
+    n = n_t
+    while (n_t * v_t + (n - n_t) * v_g) / (n * n) > target:
+        n += 1
+    return n
+
+
+def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs: int = 4) -> dict:
+    """Time to `target` relMSE, plain vs guided, on the learned-guiding workload.
+
+    Plain: per-pass relative variance v_p from independent chunks -> v_p / target
+    passes.  Guided (progressive, as render_passes accumulates it): `runs`
+    independent online trainings of train_passes passes give the training-phase
+    image's variance (network randomness included) and the training time; the
+    last trained network's per-pass variance v_g comes from chunks; the time to
+    target counts every pass of the progressive image.  The figure that discards
+    the training passes' samples is reported too."""
+    import numpy as np
+    import torch
+    from mupsi_amd import Guide, Scene, scenes
+    sc = scenes.config_scene(5)
+    W, H = sc["width"], sc["height"]
+    plain = Scene(ctx, sc)
+    plain.render_passes(0, 1)                         # allocations outside the timing
+    v_p, t_p = _chunk_stats(plain, 1000, chunks, spp)
+    plain.reset()
+    plain.render_passes(20000, 256)
+    R = plain.image()                                 # 256-pass mean, for the denominators
+    plain.close()
+    guided = Scene(ctx, sc)
+    warm = Guide.for_scene(ctx, sc)
+    guided.set_guide(warm)
+    # allocate the path, guide and training-ring buffers at their final sizes
+    guided.render_passes(0, warm.config["train_passes"] + spp)
+    warm.close()
+    n_t = 0
+    imgs, t_trains = [], []
+    g = None
+    for r in range(runs):
+        if g is not None:
+            g.close()
+        g = Guide.for_scene(ctx, sc, seed=1 + r)
+        n_t = g.config["train_passes"]
+        guided.set_guide(g)                           # fresh network and training history
+        guided.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        guided.render_passes(2000 + 100 * r, n_t)
+        torch.cuda.synchronize()
+        t_trains.append(time.perf_counter() - t0)
+        imgs.append(guided.image())
+    imgs = np.array(imgs)
+    v_t = float(np.mean(imgs.var(0, ddof=1) * n_t / (R ** 2 + 0.01)))
+    t_train = float(np.median(t_trains))
+    v_g, t_g = _chunk_stats(guided, 3000, chunks, spp)
+    st = g.stats()
+    # the same guided tracer with its roulette but without the network (alpha = 0)
+    ctl = Guide.for_scene(ctx, sc, alpha=0.0, train_passes=0)
+    guided.set_guide(ctl)
+    v_c, _ = _chunk_stats(guided, 4000, chunks // 2, spp)
+    # the network with the reference's roulette
+    g1 = Guide.for_scene(ctx, sc, rr_boost=1.0)
+    guided.set_guide(g1)
+    guided.render_passes(5000, g1.config["train_passes"])
+    v_1, _ = _chunk_stats(guided, 6000, chunks // 2, spp)
+    guided.close()
+    time_plain = t_p * v_p / target
+    n_star = _passes_to_target(v_t, n_t, v_g, target)
+    time_guided = t_train + (n_star - n_t) * t_g
+    time_guided_discard = t_train + t_g * v_g / target
+    paths = W * H
+    # algorithmic GEMM flops per training sample: forward 2(64*32 + 2*64*64 + 41*64),
+    # input gradients of layers 4..2 2(41*64 + 2*64*64), weight gradients as forward
+    mflop_step = 73088 * g.config["batch_size"] / 1e6
+    out = {
+        "workload": f"sphere-mean GPIS renewal_plus + mirror sphere r0.5 + emissive quad "
+                    f"(area 0.01, radiance 200), {W}x{H}, guide trained online on the first "
+                    f"{n_t} passes",
+        "target_relmse": target,
+        "plain": {"v_per_pass": v_p, "s_per_pass": t_p, "Mpaths_s": paths / t_p / 1e6,
+                  "passes_to_target": v_p / target, "time_to_target_s": time_plain},
+        "guided": {"v_per_pass_trained": v_g, "v_per_pass_training_phase": v_t,
+                   "s_per_pass": t_g, "Mpaths_s": paths / t_g / 1e6,
+                   "train_s": t_train, "train_s_runs": t_trains,
+                   "passes_to_target": n_star, "time_to_target_s": time_guided,
+                   "time_to_target_discarding_training_passes_s": time_guided_discard,
+                   "train_steps": st["train_steps"], "train_records": st["train_samples"],
+                   "rr_boost": g.config["rr_boost"]},
+        "guided_time_fraction": time_guided / time_plain,
+        "guided_time_fraction_discarding_training_passes": time_guided_discard / time_plain,
+        "decomposition_v_per_pass": {
+            "plain": v_p, "network_with_reference_roulette": v_1,
+            "roulette_only_alpha0": v_c, "guided": v_g},
+        "mlp_train": {"steps": st["train_steps"], "ms": st["train_ms"],
+                      "TFLOP_s": (st["train_steps"] * mflop_step / 1e6) / (st["train_ms"] * 1e-3)
+                      if st["train_ms"] > 0 else 0.0,
+                      "peak_bf16_TFLOP_s": 2500.0},
+        "estimates": f"{chunks} x {spp} passes per chunk estimate, {runs} independent trainings",
+    }
+    g.close()
+    g1.close()
+    ctl.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy a tensor of this many MiB before the timed region, so that a "
                          "profiler's FETCH_SIZE / WRITE_SIZE can be scaled by a known traffic")
@@ -125,6 +261,11 @@ def main() -> int:
           f"{launches} launches; {evals} field evals (reference grid points {steps_ref}); "
           f"image mean {img.mean():.5f}", file=sys.stderr, flush=True)
 
+    guiding = None
+    if not args.no_guiding:
+        guiding = guiding_bench(ctx)
+        print(f"bench: guiding {json.dumps(guiding)}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene)
@@ -157,6 +298,7 @@ def main() -> int:
             "avg_launch_ms": march_ms / max(1, launches),
         },
         "cpu_baseline": cpu,
+        "guiding": guiding,
     }
     print(json.dumps(line), flush=True)
     s.close()

@@ -76,7 +76,8 @@ class GuideConfig(C.Structure):
                 ("min_steps_per_pass", C.c_int32), ("max_steps_per_pass", C.c_int32),
                 ("max_records", C.c_int32),
                 ("box_center", C.c_float * 3), ("box_radius", C.c_float),
-                ("indirect_clamp", C.c_float), ("rr_boost", C.c_float)]
+                ("indirect_clamp", C.c_float), ("rr_boost", C.c_float),
+                ("passes_per_update", C.c_int32)]
 
 
 class GuideStats(C.Structure):

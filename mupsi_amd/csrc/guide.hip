@@ -454,8 +454,10 @@ std::vector<float> init_params(uint64_t seed) {
   fill(kOffW3, kHid, kHid, 1.0f);
   fill(kOffW4, kOut, kHid, 0.1f);
   // log kappa = -3 + 11 sigmoid(s): s = -0.56 starts every lobe near kappa = e;
-  // the selection output starts at alpha = alpha_max / 2
+  // the selection output starts low (alpha = alpha_max sigmoid(-2) = 0.12 alpha_max),
+  // so the first passes sample close to the cosine lobe until the guide has learnt
   for (int k = 0; k < kLobes; ++k) th[kOffB4 + 5 * k + 3] = -0.56f;
+  th[kOffB4 + kAlphaIdx] = -2.0f;
   return th;
 }
 
@@ -510,9 +512,11 @@ extern "C" mupsi_status mupsi_guide_create(mupsi_ctx* ctx, const mupsi_guide_con
       cfg->train_passes < 0 || cfg->batch_size < 64 || cfg->batch_size % 64 != 0 ||
       cfg->min_steps_per_pass < 0 || cfg->max_steps_per_pass < cfg->min_steps_per_pass ||
       cfg->max_records < 1 || cfg->max_records > 4 ||
-      !(cfg->box_radius > 0.0f) || !(cfg->indirect_clamp >= 0.0f) || !(cfg->rr_boost >= 1.0f))
+      !(cfg->box_radius > 0.0f) || !(cfg->indirect_clamp >= 0.0f) || !(cfg->rr_boost >= 1.0f) ||
+      cfg->passes_per_update < 1)
     return fail(MUPSI_ERR_INVALID, "mupsi_guide_create: bad configuration (lr > 0, alpha in [0,1], "
-                                   "batch a positive multiple of 64, max_records 1..4, radius > 0, rr_boost >= 1)");
+                                   "batch a positive multiple of 64, max_records 1..4, radius > 0, rr_boost >= 1, "
+                                   "passes_per_update >= 1)");
   HIP_TRY(hipSetDevice(ctx->device));
   auto g = std::make_unique<mupsi_guide>();
   g->ctx = ctx;

@@ -42,6 +42,7 @@ constexpr uint32_t kFlagCond = 1u << 17;
 constexpr uint32_t kRecShift = 18;  // guided vertices recorded for training (3 bits)
 constexpr uint32_t kRecMask = 7u << kRecShift;
 constexpr uint32_t kFlagOpen = 1u << 21;  // the last record still collects its direct radiance
+constexpr uint32_t kFlagNeeRec = 1u << 22;  // the record of this event waits for its light sample
 
 struct PathSoA {
   uint64_t* rng;
@@ -53,6 +54,20 @@ struct PathSoA {
   uint32_t* mseed;
   uint8_t* mhit;
   float *mt, *mpx, *mpy, *mpz;
+};
+
+// Deferred light samples of GP-medium hits (handleVolume's NEE, trace.cpp:84-98):
+// the shadow ray, the realization and conditioning it is marched in, its march
+// result, and the contribution waiting for it: L = fmaf(pend, ac, L) if the
+// march finds no sign change (transmittance 1), as the reference adds it.
+struct ShadowSoA {
+  float *ox, *oy, *oz, *dx, *dy, *dz, *nt, *ft;
+  uint32_t* seed;
+  uint8_t* active;
+  float *cx, *cy, *cz, *cu, *cgx, *cgy, *cgz;
+  uint8_t* hit;
+  float *t, *px, *py, *pz;
+  float *pend_x, *pend_y, *pend_z, *ac;
 };
 
 // Path id -> (x, y, k): explicit lists (replay) or the pixel grid of a launch.
@@ -1015,6 +1030,10 @@ struct ShadeArgs {
   int* qM2;
   int* nM2;
   GuideShade gs;
+  int defer_nee;  // medium light samples go to queue S instead of a serial march
+  ShadowSoA SH;
+  int* qS;
+  int* nS;
 };
 
 // The guided tracer keeps T as the plain tracer's throughput (albedo weights and
@@ -1059,6 +1078,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
           3.402823466e+38f};
   const bool single = S.gp_mode != MUPSI_GP_RENEWAL_PLUS;
   bool alive = true, hasHit = false, exited = true;
+  bool shadow = false;   // a deferred light sample waits in queue S
   bool rec_now = false;  // this event is a guided vertex to record
   F3 rec_wi = mk(0, 0, 0);
   float rec_q = 0.0f;
@@ -1091,13 +1111,41 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
       Ray sr = make_ray(madd(nrm, S.eps, Pp), ls.d);
       const float bias = ls.dist - S.eps * fabsf(dot(ls.d, nrm));
       sr.farT = bias * 0.99f;
-      const float trans = march_serial(S, sr, iseed, cur) ? 0.0f : 1.0f;
       const F3 f = bsdf_weight(MUPSI_BSDF_LAMBERTIAN, alb, wo, nrm, ls.d);
       const float ac = fabsf(dot(ls.d, nrm));
       const F3 Tt = Te();
-      L = mk(fmaf(trans * (Tt.x * (f.x * ls.w.x)), ac, L.x),
-             fmaf(trans * (Tt.y * (f.y * ls.w.y)), ac, L.y),
-             fmaf(trans * (Tt.z * (f.z * ls.w.z)), ac, L.z));
+      if (a.defer_nee) {
+        const ShadowSoA& H = a.SH;
+        H.ox[pid] = sr.o.x;
+        H.oy[pid] = sr.o.y;
+        H.oz[pid] = sr.o.z;
+        H.dx[pid] = sr.d.x;
+        H.dy[pid] = sr.d.y;
+        H.dz[pid] = sr.d.z;
+        H.nt[pid] = sr.nearT;
+        H.ft[pid] = sr.farT;
+        H.seed[pid] = iseed;
+        H.active[pid] = cur.active ? 1 : 0;
+        if (cur.active) {
+          H.cx[pid] = cur.C.x;
+          H.cy[pid] = cur.C.y;
+          H.cz[pid] = cur.C.z;
+          H.cu[pid] = cur.u;
+          H.cgx[pid] = cur.g.x;
+          H.cgy[pid] = cur.g.y;
+          H.cgz[pid] = cur.g.z;
+        }
+        H.pend_x[pid] = Tt.x * (f.x * ls.w.x);
+        H.pend_y[pid] = Tt.y * (f.y * ls.w.y);
+        H.pend_z[pid] = Tt.z * (f.z * ls.w.z);
+        H.ac[pid] = ac;
+        shadow = true;
+      } else {
+        const float trans = march_serial(S, sr, iseed, cur) ? 0.0f : 1.0f;
+        L = mk(fmaf(trans * (Tt.x * (f.x * ls.w.x)), ac, L.x),
+               fmaf(trans * (Tt.y * (f.y * ls.w.y)), ac, L.y),
+               fmaf(trans * (Tt.z * (f.z * ls.w.z)), ac, L.z));
+      }
     }
     F3 wi, wt;
     float q = 0.0f;
@@ -1207,6 +1255,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
   }
 
   const bool cont = alive && hasHit && bounce < S.max_bounce;
+  bool rec_wait = false;
   P.Lx[pid] = L.x;
   P.Ly[pid] = L.y;
   P.Lz[pid] = L.z;
@@ -1225,6 +1274,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
     r[9] = Tt.z;
     ++nrec;
     open = true;
+    rec_wait = shadow;  // r[4..6] still lacks this vertex's light sample
   }
   if (cont) {
     P.rng[pid] = rng.s;
@@ -1240,7 +1290,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
     P.dz[pid] = ray.d.z;
     P.nt[pid] = ray.nearT;
     P.flags[pid] = (uint32_t)bounce | (medium ? kFlagMedium : 0u) | (cur.active ? kFlagCond : 0u) |
-                   (nrec << kRecShift) | (open ? kFlagOpen : 0u);
+                   (nrec << kRecShift) | (open ? kFlagOpen : 0u) | (rec_wait ? kFlagNeeRec : 0u);
     if (cur.active) {
       P.cx[pid] = cur.C.x;
       P.cy[pid] = cur.C.y;
@@ -1258,6 +1308,40 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
   }
   wave_push(cont, pid, a.qA2, a.nA2);
   wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
+  if (a.defer_nee) wave_push(shadow, pid, a.qS, a.nS);
+}
+
+// The deferred light samples after their march (k_march / k_march_cached over
+// queue S): add the contribution where the shadow ray found no sign change, in
+// the reference's operation order, before any later event of the path adds to L.
+struct NeeArgs {
+  PathSoA P;
+  ShadowSoA SH;
+  const int* qS;
+  const int* nS;
+  GuideShade gs;
+};
+__global__ __launch_bounds__(256) void k_nee_finish(NeeArgs a) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= *a.nS) return;
+  const int pid = a.qS[qi];
+  const ShadowSoA& H = a.SH;
+  const PathSoA& P = a.P;
+  const float trans = H.hit[pid] ? 0.0f : 1.0f;
+  const float ac = H.ac[pid];
+  const float px = trans * H.pend_x[pid], py = trans * H.pend_y[pid], pz = trans * H.pend_z[pid];
+  P.Lx[pid] = fmaf(px, ac, P.Lx[pid]);
+  P.Ly[pid] = fmaf(py, ac, P.Ly[pid]);
+  P.Lz[pid] = fmaf(pz, ac, P.Lz[pid]);
+  const uint32_t fl = P.flags[pid];
+  if (a.gs.record && (fl & kFlagNeeRec)) {
+    const uint32_t nrec = (fl & kRecMask) >> kRecShift;
+    float* r = a.gs.rec + ((size_t)pid * a.gs.K + (nrec - 1)) * kRecFloats;
+    r[4] = fmaf(px, ac, r[4]);
+    r[5] = fmaf(py, ac, r[5]);
+    r[6] = fmaf(pz, ac, r[6]);
+    P.flags[pid] = fl & ~kFlagNeeRec;
+  }
 }
 
 // Guided vertices of this bounce: every path of queue M whose march hit the
@@ -1676,8 +1760,10 @@ struct PathBuffers {
   PathSoA P{};
   int* qA[2]{};
   int* qM[2]{};
-  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work, nG, training records]
+  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work, nG, training records, nS]
   unsigned long long* stats = nullptr;
+  ShadowSoA SH{};
+  int* qS = nullptr;
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -1827,6 +1913,17 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   HIP_TRY(m.alloc(n, &P.flags));
   HIP_TRY(m.alloc(n, &P.mseed));
   HIP_TRY(m.alloc(n, &P.mhit));
+  if (sc->ds.n_lights > 0 && sc->ds.has_gp) {  // deferred medium light samples
+    ShadowSoA& H = pb->SH;
+    float** sf[] = {&H.ox, &H.oy, &H.oz, &H.dx, &H.dy, &H.dz, &H.nt, &H.ft, &H.cx, &H.cy, &H.cz,
+                    &H.cu, &H.cgx, &H.cgy, &H.cgz, &H.t, &H.px, &H.py, &H.pz, &H.pend_x,
+                    &H.pend_y, &H.pend_z, &H.ac};
+    for (float** p : sf) HIP_TRY(m.alloc(n, p));
+    HIP_TRY(m.alloc(n, &H.seed));
+    HIP_TRY(m.alloc(n, &H.active));
+    HIP_TRY(m.alloc(n, &H.hit));
+    HIP_TRY(m.alloc(n, &pb->qS));
+  }
   for (int i = 0; i < 2; ++i) {
     HIP_TRY(m.alloc(n, &pb->qA[i]));
     HIP_TRY(m.alloc(n, &pb->qM[i]));
@@ -2000,9 +2097,46 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nA2, 2, 0, -1);
     GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
     if (gr) gs = GuideShade{1, gr->g->cfg.rr_boost, gb->mix, gr->record, K, gb->rec};
-    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs};
+    const bool defer = pb.qS != nullptr;
+    int* nS = pb.counters + 7;
+    if (defer) hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nS, 1, 0, -1);
+    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
+                 defer ? 1 : 0, pb.SH, pb.qS, nS};
     hipLaunchKernelGGL(k_shade, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
+    if (defer) {
+      // transmittance of this iteration's medium light samples (gpmedium.cpp:105-129)
+      int* work = pb.counters + 4;
+      hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work, 1, 0, -1);
+      const ShadowSoA& H = pb.SH;
+      MarchArgs ma{};
+      ma.queue = pb.qS;
+      ma.count_dev = nS;
+      ma.ox = H.ox; ma.oy = H.oy; ma.oz = H.oz;
+      ma.dx = H.dx; ma.dy = H.dy; ma.dz = H.dz;
+      ma.nt = H.nt;
+      ma.ft = H.ft;
+      ma.seed = H.seed;
+      ma.flags = nullptr;
+      ma.active = H.active;
+      ma.cx = H.cx; ma.cy = H.cy; ma.cz = H.cz; ma.cu = H.cu;
+      ma.cgx = H.cgx; ma.cgy = H.cgy; ma.cgz = H.cgz;
+      ma.hit = H.hit;
+      ma.t = H.t; ma.px = H.px; ma.py = H.py; ma.pz = H.pz;
+      ma.steps = nullptr;
+      ma.work = work;
+      ma.stats = pb.stats;
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      evs.emplace_back(e0, e1);
+      HIP_TRY(hipEventRecord(e0, st));
+      launch_march(ctx, S, ma, st);
+      HIP_TRY(hipEventRecord(e1, st));
+      NeeArgs na{pb.P, H, pb.qS, nS, gs};
+      hipLaunchKernelGGL(k_nee_finish, dim3(grid_for(n)), dim3(256), 0, st, na);
+      HIP_TRY(hipGetLastError());
+    }
   }
   HIP_TRY(hipStreamSynchronize(st));
   double ms = 0;
@@ -2042,7 +2176,7 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
   mupsi_guide* guide = sc->guide;
   if (guide) {
     const bool will_train = guide->stats.train_passes_done < guide->cfg.train_passes;
-    const int64_t per_pass = npix * guide->cfg.max_records;
+    const int64_t per_pass = npix * guide->cfg.max_records * guide->cfg.passes_per_update;
     const int64_t ring = std::min<int64_t>(std::max<int64_t>(per_pass, 8 << 20),
                                            (int64_t)std::max(1, guide->cfg.train_passes) * per_pass);
     st = ensure_guide_buffers(sc, (int64_t)ppl * npix, will_train ? per_pass : 1,
@@ -2057,10 +2191,15 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
   int launches = 0;
   for (int kb = k_begin; kb < k_begin + k_count;) {
     if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
-    // online training: each of the guide's first train_passes passes is traced
-    // alone, then its recorded vertices train the network before the next pass
+    // online training: the guide's first train_passes passes are traced
+    // passes_per_update at a time, and their recorded vertices train the network
+    // before the next ones
     const bool training = guide && guide->stats.train_passes_done < guide->cfg.train_passes;
-    const int kc = training ? 1 : std::min(ppl, k_begin + k_count - kb);
+    const int kc =
+        training ? (int)std::min<int64_t>({(int64_t)guide->cfg.passes_per_update,
+                                           guide->cfg.train_passes - guide->stats.train_passes_done,
+                                           (int64_t)(k_begin + k_count - kb), (int64_t)ppl})
+                 : std::min(ppl, k_begin + k_count - kb);
     PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr};
     GuideRun gr{guide, training ? 1 : 0, 0.0};
     st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches,
@@ -2079,10 +2218,11 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
       HIP_TRY(hipEventCreate(&t1e));
       HIP_TRY(hipEventRecord(t0e, strm));
       hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, strm, tr_count, 1, 0, -1);
-      CollectArgs ca{pb.P, (int)npix, guide->cfg.max_records, gb.rec, gb.rec_feat, gb.tr_feat,
+      const int64_t np = (int64_t)kc * npix;
+      CollectArgs ca{pb.P, (int)np, guide->cfg.max_records, gb.rec, gb.rec_feat, gb.tr_feat,
                      gb.tr_dir, gb.tr_w, tr_count, gb.tr_head, gb.tr_capacity,
                      guide->cfg.indirect_clamp};
-      hipLaunchKernelGGL(k_collect, dim3(grid_for(npix)), dim3(256), 0, strm, ca);
+      hipLaunchKernelGGL(k_collect, dim3(grid_for(np)), dim3(256), 0, strm, ca);
       HIP_TRY(hipGetLastError());
       int count = 0;
       HIP_TRY(hipMemcpyAsync(&count, tr_count, sizeof(int), hipMemcpyDeviceToHost, strm));
@@ -2093,8 +2233,9 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
       // about four passes over this pass's new records, drawn from the whole ring,
       // and at least min_steps_per_pass steps
       const int steps = (int)std::min<int64_t>(
-          guide->cfg.max_steps_per_pass,
-          std::max<int64_t>(guide->cfg.min_steps_per_pass, (4LL * count + batch - 1) / batch));
+          (int64_t)guide->cfg.max_steps_per_pass * kc,
+          std::max<int64_t>((int64_t)guide->cfg.min_steps_per_pass * kc,
+                            (4LL * count + batch - 1) / batch));
       if (gb.tr_size > 0 && steps > 0) {
         st = guide::train_random(guide, gb.tr_feat, gb.tr_dir, gb.tr_w, (int)gb.tr_size, steps,
                                  batch, strm);
@@ -2108,7 +2249,7 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
       (void)hipEventDestroy(t1e);
       guide->stats.train_ms += tms;
       guide->stats.train_samples += count;
-      guide->stats.train_passes_done += 1;
+      guide->stats.train_passes_done += kc;
       if (gb.tr_size > 0 && steps > 0) {
         float lp[256];
         const int nb = std::max(1, std::min(batch / 64, 256));

@@ -18,8 +18,11 @@ def _medium_pixels(W, H, rng, n):
     return x, y
 
 
-@pytest.mark.parametrize("cfg", [1, 2])
+@pytest.mark.parametrize("cfg", [1, 2, 5])
 def test_path_replay_matches_oracle(ctx, cfg):
+    """Whole paths against the oracle; configuration 5 adds the emissive quad, so
+    every medium hit takes a light sample whose shadow ray the GPU marches in a
+    separate queue (the oracle marches it serially)."""
     from mupsi_amd import Scene
     sc = scenes.config_scene(cfg)
     s = Scene(ctx, sc)
