@@ -263,6 +263,42 @@ mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices, const in
                              int32_t n_faces, float* nodes_out, int32_t* n_nodes_out,
                              uint32_t* tri_index_out, uint32_t* root_out);
 
+/* ---- batched scenes ----
+ * Several uploaded scenes of one context traced in shared wavefront launches
+ * (the pass loop of Renderer::startRenderProgressive, renderer.cpp:22-49, for
+ * many scenes at once): every launch mixes the paths of several scenes, each
+ * path carrying its scene id, so rays of uneven march lengths from all scenes
+ * share the persistent marchers.  Each scene accumulates into its own
+ * accumulator (mupsi_read_accum), with the same per-path results as
+ * mupsi_render_passes.  The scenes must outlive the batch; light samples of
+ * medium hits are marched serially in batched launches and attached guides
+ * are not used. */
+typedef struct mupsi_batch mupsi_batch;
+
+typedef struct {
+  int64_t paths_per_launch;    /* 0: chosen by the autotuner on the first launches */
+  int32_t march_blocks_per_cu; /* persistent marcher blocks per CU; 0: autotuned */
+  int32_t use_graph;           /* 1: each launch's bounce loop replays a captured HIP graph */
+} mupsi_batch_tuning;
+
+typedef struct {
+  int64_t paths;               /* paths traced by mupsi_batch_render calls */
+  int64_t launches;
+  double total_ms;             /* wall time of the render calls */
+  double tune_ms;              /* part of it spent on autotuner probe launches (real passes) */
+  int64_t graph_builds;
+} mupsi_batch_stats;
+
+mupsi_status mupsi_batch_create(mupsi_ctx* ctx, mupsi_scene* const* scenes, int32_t n,
+                                mupsi_batch** out);
+void mupsi_batch_free(mupsi_batch* batch);
+mupsi_status mupsi_batch_set_tuning(mupsi_batch* batch, const mupsi_batch_tuning* tuning);
+mupsi_status mupsi_batch_get_tuning(mupsi_batch* batch, mupsi_batch_tuning* tuning);
+/* Passes k_begin[s] .. k_begin[s] + k_count[s] - 1 of every scene s. */
+mupsi_status mupsi_batch_render(mupsi_batch* batch, const int32_t* k_begin, const int32_t* k_count,
+                                const volatile int32_t* abort_flag);
+mupsi_status mupsi_batch_stats_get(mupsi_batch* batch, mupsi_batch_stats* out);
+
 /* ---- learned path guiding ----
  * No counterpart in the reference: an addition to its trace loop
  * (trace.cpp:77-116, the Lambertian sample of handleVolume).  A guide is a

@@ -27,7 +27,7 @@ from . import _abi
 from ._abi import MupsiError, check
 from .scene import build_desc, write_scene_files
 
-__all__ = ["Context", "Scene", "Renderer", "Guide", "MupsiError", "build_desc",
+__all__ = ["Context", "Scene", "Renderer", "Guide", "Batch", "MupsiError", "build_desc",
            "write_scene_files", "library_path"]
 
 
@@ -46,6 +46,7 @@ class Context:
         self.device = device
         self._scenes: "weakref.WeakSet[Scene]" = weakref.WeakSet()
         self._guides: "weakref.WeakSet[Guide]" = weakref.WeakSet()
+        self._batches: "weakref.WeakSet[Batch]" = weakref.WeakSet()
 
     def info(self) -> tuple[str, int]:
         name = C.create_string_buffer(256)
@@ -56,6 +57,8 @@ class Context:
     def close(self) -> None:
         """Free every scene of this context, then the context (mupsi_close)."""
         if self.handle:
+            for b in list(self._batches):
+                b.close()
             for s in list(self._scenes):
                 s.close()
             for g in list(self._guides):
@@ -241,6 +244,61 @@ class Guide:
         if getattr(self, "handle", None):
             if self.ctx.handle:
                 self.lib.mupsi_guide_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several scenes of one context traced in shared wavefront launches
+    (mupsi_batch_*): render(k_begin, k_count) renders passes
+    [k_begin[s], k_begin[s] + k_count[s]) of every scene s into its own
+    accumulator (read with Scene.read_accum / Scene.image)."""
+
+    def __init__(self, ctx: Context, scenes: list, tuning: dict | None = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[sc.handle.value for sc in self.scenes])
+        h = C.c_void_p()
+        check(self.lib.mupsi_batch_create(ctx.handle, arr, len(self.scenes), C.byref(h)),
+              "mupsi_batch_create")
+        self.handle = h
+        ctx._batches.add(self)
+        if tuning:
+            self.set_tuning(**tuning)
+
+    def set_tuning(self, paths_per_launch: int = 0, march_blocks_per_cu: int = 0,
+                   use_graph: bool = True) -> None:
+        t = _abi.BatchTuning(paths_per_launch, march_blocks_per_cu, 1 if use_graph else 0)
+        check(self.lib.mupsi_batch_set_tuning(self.handle, C.byref(t)), "mupsi_batch_set_tuning")
+
+    def tuning(self) -> dict:
+        t = _abi.BatchTuning()
+        check(self.lib.mupsi_batch_get_tuning(self.handle, C.byref(t)), "mupsi_batch_get_tuning")
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def render(self, k_begin, k_count, abort_flag: Optional[C.c_int32] = None) -> None:
+        n = len(self.scenes)
+        kb = np.ascontiguousarray(np.broadcast_to(np.asarray(k_begin, np.int32), (n,)))
+        kc = np.ascontiguousarray(np.broadcast_to(np.asarray(k_count, np.int32), (n,)))
+        ptr = C.cast(C.byref(abort_flag), C.c_void_p) if abort_flag is not None else None
+        check(self.lib.mupsi_batch_render(self.handle, kb.ctypes.data_as(C.c_void_p),
+                                          kc.ctypes.data_as(C.c_void_p), ptr), "mupsi_batch_render")
+
+    def stats(self) -> dict:
+        s = _abi.BatchStats()
+        check(self.lib.mupsi_batch_stats_get(self.handle, C.byref(s)), "mupsi_batch_stats_get")
+        return {name: getattr(s, name) for name, _ in s._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            if self.ctx.handle:
+                self.lib.mupsi_batch_free(self.handle)
             self.handle = None
 
     def __del__(self):

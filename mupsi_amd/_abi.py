@@ -86,6 +86,16 @@ class GuideStats(C.Structure):
                 ("train_ms", C.c_double), ("infer_ms", C.c_double)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("paths_per_launch", C.c_int64), ("march_blocks_per_cu", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("tune_ms", C.c_double), ("graph_builds", C.c_int64)]
+
+
 GUIDE_NUM_PARAMS = 13097
 
 VP = C.c_void_p
@@ -119,6 +129,12 @@ SIGNATURES = {
     "mupsi_mesh_closest": (I32, [VP, I32, VP, VP, VP, I64, VP, VP, VP]),
     "mupsi_mesh_inside": (I32, [VP, I32, VP, VP, VP, I64, VP]),
     "mupsi_bvh_build": (I32, [VP, I32, VP, I32, VP, C.POINTER(I32), VP, C.POINTER(C.c_uint32)]),
+    "mupsi_batch_create": (I32, [VP, VP, I32, C.POINTER(VP)]),
+    "mupsi_batch_free": (None, [VP]),
+    "mupsi_batch_set_tuning": (I32, [VP, C.POINTER(BatchTuning)]),
+    "mupsi_batch_get_tuning": (I32, [VP, C.POINTER(BatchTuning)]),
+    "mupsi_batch_render": (I32, [VP, VP, VP, VP]),
+    "mupsi_batch_stats_get": (I32, [VP, C.POINTER(BatchStats)]),
     "mupsi_guide_create": (I32, [VP, C.POINTER(GuideConfig), C.POINTER(VP)]),
     "mupsi_guide_free": (None, [VP]),
     "mupsi_guide_get_params": (I32, [VP, VP]),

@@ -54,6 +54,7 @@ struct PathSoA {
   uint32_t* mseed;
   uint8_t* mhit;
   float *mt, *mpx, *mpy, *mpz;
+  uint16_t* sid;  // batched launches: the path's scene
 };
 
 // Deferred light samples of GP-medium hits (handleVolume's NEE, trace.cpp:84-98):
@@ -70,13 +71,36 @@ struct ShadowSoA {
   float *pend_x, *pend_y, *pend_z, *ac;
 };
 
-// Path id -> (x, y, k): explicit lists (replay) or the pixel grid of a launch.
+// A batched launch (several scenes in one wavefront): scene s owns the path ids
+// [off, off + kc * npix), passes k_begin .. k_begin + kc - 1 of its W-wide image,
+// pass-major.  The plan lives in device memory so that one captured graph can
+// be replayed with the next launch's plan.
+struct LaunchPlan {
+  int64_t off;
+  int npix, W, k_begin, kc;
+};
+
+// Path id -> (x, y, k): explicit lists (replay), the pixel grid of a launch, or a
+// batched launch's plan (with the path's scene id).
 struct PathSrc {
   int W, npix, k_begin;
   const int32_t *xs, *ys, *ks;
+  const LaunchPlan* plan;   // batched launch, or null
+  int nplan;
+  const int64_t* total;     // batched launch: paths in this launch (device)
 };
-__device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int& y, int& k) {
-  if (s.xs) {
+__device__ __forceinline__ void plan_xyk(const LaunchPlan& L, int64_t pid, int& x, int& y, int& k) {
+  const int64_t local = pid - L.off;
+  const int pix = (int)(local % L.npix);
+  x = pix % L.W;
+  y = pix / L.W;
+  k = L.k_begin + (int)(local / L.npix);
+}
+__device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int& y, int& k,
+                                         const uint16_t* sid = nullptr) {
+  if (s.plan) {
+    plan_xyk(s.plan[sid[pid]], pid, x, y, k);
+  } else if (s.xs) {
     x = s.xs[pid];
     y = s.ys[pid];
     k = s.ks[pid];
@@ -86,6 +110,15 @@ __device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int&
     y = pix / s.W;
     k = s.k_begin + pid / s.npix;
   }
+}
+
+// The scene of a path: the kernel argument in a single-scene launch, the
+// per-scene table in a batched one.
+template <bool kMulti>
+__device__ __forceinline__ const DevScene& pick_scene(const DevScene& S0, const DevScene* scenes,
+                                                      const uint16_t* sid, int pid) {
+  if constexpr (kMulti) return scenes[sid[pid]];
+  else return S0;
 }
 
 // Slot of this lane among the wave's lanes with pred, after one atomic per wave
@@ -562,6 +595,8 @@ struct MarchArgs {
   int32_t* steps;  // t-grid points visited up to and including the hit, or null
   int* work;
   unsigned long long* stats;  // [0] field evals, [1] t-grid points (reference work), [2] rays
+  const DevScene* scenes;     // batched launch: per-scene constants, indexed by sid[ray]
+  const uint16_t* sid;
 };
 
 // GPMedium::sampleDistance / transmittance (gpmedium.cpp:46-129), lane per ray.
@@ -576,7 +611,8 @@ struct MarchArgs {
 // with | |p-c| - R | > B + margin has a known sign.  Such points are not
 // evaluated: same sign as f_prev -> no hit; opposite sign -> the hit is there.
 // Past the outer shell and moving away, no later point can change sign.
-__global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = __lane_id();
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
@@ -591,6 +627,11 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
   sk.kind = 0;
   int nsteps = 0;
   unsigned long long evals = 0, refsteps = 0, rays = 0;
+  int sidx = 0;  // batched launch: scene of this lane's ray
+  auto scene = [&]() -> const DevScene& {
+    if constexpr (kMulti) return a.scenes[sidx];
+    else return S0;
+  };
 
   auto finish = [&](bool hit, F3 p) {
     a.hit[rid] = hit ? 1 : 0;
@@ -622,6 +663,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
           const int qi = base + __popcll(m & ((1ull << lane) - 1ull));
           if (qi < n) {
             rid = a.queue ? a.queue[qi] : qi;
+            if constexpr (kMulti) sidx = a.sid[rid];
             o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
             d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
             nt = a.nt[rid];
@@ -634,7 +676,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sk = make_skip(S, c, o, ft);
+            sk = make_skip(scene(), c, o, ft);
             nsteps = 0;
             ++rays;
             phase = 1;
@@ -654,7 +696,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
           }
           const F3 p = madd(d, t, o);
           bool fin;
-          const int cs = classify_point(S, sk, p, d, &fin);
+          const int cs = classify_point(scene(), sk, p, d, &fin);
           if (cs == 0) {
             q = p;
             do_eval = true;
@@ -676,7 +718,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S, MarchArgs a) {
     }
     if (__ballot(phase != 3) == 0) break;
     if (do_eval) {
-      const float f = field_eval<false>(S, q, seed, c, nullptr);
+      const float f = field_eval<false>(scene(), q, seed, c, nullptr);
       ++evals;
       if (phase == 1) {
         fprev = f;
@@ -917,6 +959,216 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
   }
 }
 
+// Batched launch: the same marcher for rays of sphere-mean scenes with any
+// points_per_cell <= 8, each 32-lane group taking its ray's scene constants.
+__global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedConsts* ccs) {
+  __shared__ float4 win[2][27 * kMaxPpcCached];
+  const int n = a.count_dev ? *a.count_dev : a.count_host;
+  const int lane = threadIdx.x;
+  const int grp = lane >> 5;
+  const int gl = lane & (kGroup - 1);
+  float4* W = win[grp];
+  // the group's ray and its scene
+  DevGP G{};
+  int ppc = 1, nslots = 27, sidx = 0;
+  F3 mc = mk(0, 0, 0);
+  CachedConsts cc{};
+  // ray state: identical in the 32 lanes of a group
+  int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
+  int rid = 0;
+  F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  bool sp = false;
+  uint32_t seed = 0;
+  Cond c;
+  c.active = false;
+  int nsteps = 0;
+  int wx = 0, wy = 0, wz = 0;
+  bool wvalid = false;
+  unsigned long long evals = 0, refsteps = 0, rays = 0;
+
+  auto finish = [&](bool hit, F3 p) {
+    if (gl == 0) {
+      a.hit[rid] = hit ? 1 : 0;
+      if (hit) {
+        a.t[rid] = t;
+        a.px[rid] = p.x;
+        a.py[rid] = p.y;
+        a.pz[rid] = p.z;
+      }
+      if (a.steps) a.steps[rid] = hit ? nsteps : -1;
+      refsteps += hit ? (unsigned long long)nsteps
+                      : nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
+    }
+    phase = 0;
+  };
+
+  while (true) {
+    bool do_eval = false;
+    F3 q = mk(0, 0, 0);
+    for (int round = 0; round < 2; ++round) {
+      const bool need = phase == 0;
+      const uint64_t m = __ballot(need && gl == 0);
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.work, __popcll(m));
+        base = __shfl(base, leader);
+        if (need) {
+          const int qi = base + __popcll(m & ((1ull << (grp * kGroup)) - 1ull));
+          if (qi < n) {
+            rid = a.queue ? a.queue[qi] : qi;
+            sidx = a.sid[rid];
+            G = a.scenes[sidx].gp;
+            ppc = G.ppc;
+            nslots = 27 * ppc;
+            mc = mk(a.scenes[sidx].mean_c[0], a.scenes[sidx].mean_c[1], a.scenes[sidx].mean_c[2]);
+            cc = ccs[sidx];
+            o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
+            d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
+            nt = a.nt[rid];
+            ft = fminf(a.ft ? a.ft[rid] : 3.402823466e+38f, 2000.0f);
+            dt = fminf(0.5f, (ft - nt) / 100);
+            seed = a.seed[rid];
+            c.active = a.flags ? (a.flags[rid] & kFlagCond) != 0 : (a.active && a.active[rid]);
+            if (c.active) {
+              c.C = mk(a.cx[rid], a.cy[rid], a.cz[rid]);
+              c.u = a.cu[rid];
+              c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
+            }
+            {
+              const SkipInfo sk = make_skip(a.scenes[sidx], c, o, ft);
+              outer2 = sk.outer2;
+              inner2 = sk.inner2;
+            }
+            nsteps = 0;
+            wvalid = false;  // new realization seed
+            if (gl == 0) ++rays;
+            phase = 1;
+          } else {
+            phase = 3;
+          }
+        }
+      }
+      if (phase == 1) {
+        q = o;
+        do_eval = true;
+      } else if (phase == 2 && !do_eval) {
+        while (true) {
+          if (!(t < ft)) {
+            finish(false, q);
+            break;
+          }
+          const F3 p = madd(d, t, o);
+          const F3 r = sub(p, mc);
+          const float q2 = sq3(r);
+          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          if (cs == 0) {
+            q = p;
+            do_eval = true;
+            break;
+          }
+          ++nsteps;
+          if ((cs > 0) != sp) {
+            finish(true, p);
+            break;
+          }
+          if ((cs > 0 && dot(d, r) >= 0.0f) || !(t + dt > t)) {
+            finish(false, p);
+            break;
+          }
+          t += dt;
+        }
+      }
+      if (__ballot(phase == 0) == 0) break;
+    }
+    if (__ballot(phase != 3) == 0) break;
+
+    // Window: cells entering the 27-cell neighbourhood of the evaluation point.
+    int newcells = -1;
+    int cx = 0, cy = 0, cz = 0;
+    if (do_eval) {
+      cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
+      cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
+      cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
+      if (!wvalid) {
+        newcells = 27;
+      } else {
+        const int ex = cx - wx, ey = cy - wy, ez = cz - wz;
+        if (ex | ey | ez) {
+          newcells = (abs(ex) <= 1 && abs(ey) <= 1 && abs(ez) <= 1) ? (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)
+                                                                   : 27;
+        }
+      }
+    }
+    __syncthreads();  // the previous evaluation's reads of the window are complete
+    if (newcells >= 0) {
+      const int items = kNewCells.n[newcells] * ppc;
+      const int mx = mod3(cx), my = mod3(cy), mz = mod3(cz);  // residues of the window centre
+      for (int it = gl; it < items; it += kGroup) {
+        const int k = it / ppc;
+        const int i = it - k * ppc;
+        const int ni = kNewCells.idx[newcells][k];
+        const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
+        int rx = mx + ox, ry = my + oy, rz = mz + oz;
+        rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
+        ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
+        rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
+        W[(rx * 9 + ry * 3 + rz) * ppc + i] = make_impulse(G, cx + ox, cy + oy, cz + oz, seed, i);
+      }
+      wx = cx;
+      wy = cy;
+      wz = cz;
+      wvalid = true;
+    }
+    __syncthreads();
+
+    float acc = 0.0f;
+    if (do_eval) {
+      for (int sl = gl; sl < nslots; sl += kGroup) {
+        const float4 im = W[sl];
+        const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
+        const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+        acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * cc.psi_k), acc);
+      }
+    }
+    for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+
+    if (do_eval) {
+      float f = mean_eval<false>(a.scenes[sidx], q, nullptr) + G.sigma * acc / G.sqrt_var;
+      if (c.active) {
+        const F3 r = sub(q, c.C);
+        const float h = kern_h(G, c.C, q);
+        f = f + fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
+      }
+      if (gl == 0) ++evals;
+      if (phase == 1) {
+        fprev = f;
+        sp = f > 0.0f;
+        t = nt;
+        phase = 2;
+      } else {
+        ++nsteps;
+        if ((f > 0.0f) != sp && fabsf(fprev - f) > 1e-6f) {
+          finish(true, q);
+        } else if (!(t + dt > t)) {
+          finish(false, q);
+        } else {
+          t += dt;
+        }
+      }
+    }
+  }
+  evals = wave_sum(evals);
+  refsteps = wave_sum(refsteps);
+  rays = wave_sum(rays);
+  if (lane == 0 && a.stats) {
+    atomicAdd(&a.stats[0], evals);
+    atomicAdd(&a.stats[1], refsteps);
+    atomicAdd(&a.stats[2], rays);
+  }
+}
+
 // ====================================================================== device: Renewal+
 
 // GPMedium::sampleCondition (gpmedium.cpp:133-162) as a 4x4 FP64 solve of
@@ -975,13 +1227,28 @@ struct InitArgs {
   PathSrc src;
   int n;
   int* qA;
+  const DevScene* scenes;  // batched launch
 };
-__global__ __launch_bounds__(256) void k_init(DevScene S, InitArgs a) {
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_init(DevScene S0, InitArgs a) {
   const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= a.n) return;
-  int x, y, k;
-  path_xyk(a.src, pid, x, y, k);
   const PathSoA& P = a.P;
+  int x, y, k;
+  if constexpr (kMulti) {
+    if (pid >= *a.src.total) return;
+    int lo = 0, hi = a.src.nplan - 1;  // the last plan entry with off <= pid
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.src.plan[mid].off <= pid) lo = mid;
+      else hi = mid - 1;
+    }
+    P.sid[pid] = (uint16_t)lo;
+    plan_xyk(a.src.plan[lo], pid, x, y, k);
+  } else {
+    if (pid >= a.n) return;
+    path_xyk(a.src, pid, x, y, k);
+  }
+  const DevScene& S = pick_scene<kMulti>(S0, a.scenes, P.sid, pid);
   P.rng[pid] = pcg_ctor_state(make_seed4((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u));
   const F3 d = normalized(camera_dir(S, x, y));
   P.ox[pid] = S.cam_pos[0];
@@ -1034,6 +1301,9 @@ struct ShadeArgs {
   ShadowSoA SH;
   int* qS;
   int* nS;
+  const DevScene* scenes;  // batched launch: per-scene constants; medium paths of
+  int* qMm2;               // mesh-mean scenes (or ppc > 8) go to queue Mm
+  int* nMm2;
 };
 
 // The guided tracer keeps T as the plain tracer's throughput (albedo weights and
@@ -1051,14 +1321,16 @@ __device__ __forceinline__ float rr_survival(const GuideShade& gs, F3 T) {
 // One iteration of the bounce loop of PathTracer::trace (trace.cpp:138-210) after
 // the march: handleVolume + sampleCondition for medium hits, else
 // Scene::intersect + handleSurface + medium switch, or the skydrome.
-__global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = *a.nA;
   if (qi >= n) return;
   const int pid = a.qA[qi];
   const PathSoA& P = a.P;
+  const DevScene& S = pick_scene<kMulti>(S0, a.scenes, P.sid, pid);
   int x, y, k;
-  path_xyk(a.src, pid, x, y, k);
+  path_xyk(a.src, pid, x, y, k, P.sid);
   uint32_t fl = P.flags[pid];
   int bounce = (int)(fl & kBounceMask);
   bool medium = (fl & kFlagMedium) != 0;
@@ -1307,7 +1579,13 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S, ShadeArgs a) {
     }
   }
   wave_push(cont, pid, a.qA2, a.nA2);
-  wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
+  if constexpr (kMulti) {
+    const bool cached = S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached;
+    wave_push(cont && medium && !S.skip_medium && cached, pid, a.qM2, a.nM2);
+    wave_push(cont && medium && !S.skip_medium && !cached, pid, a.qMm2, a.nMm2);
+  } else {
+    wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
+  }
   if (a.defer_nee) wave_push(shadow, pid, a.qS, a.nS);
 }
 
@@ -1496,6 +1774,25 @@ __global__ __launch_bounds__(256) void k_accum(const float* Lx, const float* Ly,
   double sx = acc[3 * pix], sy = acc[3 * pix + 1], sz = acc[3 * pix + 2];
   for (int kk = 0; kk < kcount; ++kk) {
     const int64_t i = (int64_t)kk * npix + pix;
+    sx += (double)Lx[i];
+    sy += (double)Ly[i];
+    sz += (double)Lz[i];
+  }
+  acc[3 * pix] = sx;
+  acc[3 * pix + 1] = sy;
+  acc[3 * pix + 2] = sz;
+}
+
+// Batched launch: scene s's share of the launch into its own accumulator.
+__global__ __launch_bounds__(256) void k_accum_plan(const float* Lx, const float* Ly,
+                                                    const float* Lz, const LaunchPlan* plan, int s,
+                                                    double* acc) {
+  const LaunchPlan L = plan[s];
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= L.npix || L.kc <= 0) return;
+  double sx = acc[3 * pix], sy = acc[3 * pix + 1], sz = acc[3 * pix + 2];
+  for (int kk = 0; kk < L.kc; ++kk) {
+    const int64_t i = L.off + (int64_t)kk * L.npix + pix;
     sx += (double)Lx[i];
     sy += (double)Ly[i];
     sz += (double)Lz[i];
@@ -1717,7 +2014,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   ctx->cus = prop.multiProcessorCount;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   int per_cu = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 256, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_cached<3>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
@@ -1726,11 +2023,16 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
 }
 
 static void destroy_scene(mupsi_scene* scene);
+static void destroy_batch(mupsi_batch* b);
+static void batch_forget_scene(mupsi_ctx* ctx, const mupsi_scene* scene);
 
 // Frees the context and every scene still open on it (their handles become invalid).
 extern "C" void mupsi_close(mupsi_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  std::vector<mupsi_batch*> batches;
+  batches.swap(ctx->batches);
+  for (mupsi_batch* b : batches) destroy_batch(b);
   std::vector<mupsi_scene*> live;
   live.swap(ctx->scenes);
   for (mupsi_scene* s : live) destroy_scene(s);
@@ -1760,10 +2062,12 @@ struct PathBuffers {
   PathSoA P{};
   int* qA[2]{};
   int* qM[2]{};
-  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work, nG, training records, nS]
+  int* counters = nullptr;  // [nA0, nM0, nA1, nM1, work, nG, training records, nS,
+                            //  batched: nMm0, nMm1, work of queue Mm]
   unsigned long long* stats = nullptr;
   ShadowSoA SH{};
   int* qS = nullptr;
+  int* qMm[2]{};  // batched launches: medium paths for the lane-per-ray marcher
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -1888,6 +2192,7 @@ extern "C" void mupsi_scene_free(mupsi_scene* scene) {
   mupsi_ctx* ctx = scene->ctx;
   auto it = std::find(ctx->scenes.begin(), ctx->scenes.end(), scene);
   if (it != ctx->scenes.end()) ctx->scenes.erase(it);
+  batch_forget_scene(ctx, scene);
   (void)hipSetDevice(ctx->device);
   destroy_scene(scene);
 }
@@ -1899,9 +2204,9 @@ extern "C" mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int
   return MUPSI_OK;
 }
 
-static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
-  if (sc->pb && sc->pb->capacity >= n) return MUPSI_OK;
-  sc->pb.reset();
+static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_t n, bool shadow,
+                                       bool mesh_queue) {
+  out.reset();
   auto pb = std::make_unique<PathBuffers>();
   DeviceBuffers& m = pb->mem;
   PathSoA& P = pb->P;
@@ -1913,7 +2218,8 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   HIP_TRY(m.alloc(n, &P.flags));
   HIP_TRY(m.alloc(n, &P.mseed));
   HIP_TRY(m.alloc(n, &P.mhit));
-  if (sc->ds.n_lights > 0 && sc->ds.has_gp) {  // deferred medium light samples
+  HIP_TRY(m.alloc(n, &P.sid));
+  if (shadow) {  // deferred medium light samples
     ShadowSoA& H = pb->SH;
     float** sf[] = {&H.ox, &H.oy, &H.oz, &H.dx, &H.dy, &H.dz, &H.nt, &H.ft, &H.cx, &H.cy, &H.cz,
                     &H.cu, &H.cgx, &H.cgy, &H.cgz, &H.t, &H.px, &H.py, &H.pz, &H.pend_x,
@@ -1927,12 +2233,18 @@ static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   for (int i = 0; i < 2; ++i) {
     HIP_TRY(m.alloc(n, &pb->qA[i]));
     HIP_TRY(m.alloc(n, &pb->qM[i]));
+    if (mesh_queue) HIP_TRY(m.alloc(n, &pb->qMm[i]));
   }
-  HIP_TRY(m.alloc(8, &pb->counters));
+  HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
   pb->capacity = n;
-  sc->pb = std::move(pb);
+  out = std::move(pb);
   return MUPSI_OK;
+}
+
+static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
+  if (sc->pb && sc->pb->capacity >= n) return MUPSI_OK;
+  return alloc_path_buffers(sc->pb, n, sc->ds.n_lights > 0 && sc->ds.has_gp, false);
 }
 
 static mupsi_status ensure_guide_buffers(mupsi_scene* sc, int64_t n, int64_t rec_slots,
@@ -1992,13 +2304,18 @@ extern "C" mupsi_status mupsi_scene_guide_records(mupsi_scene* sc, float* feats,
 
 // The cached-window marcher serves sphere means with points_per_cell <= 8; mesh
 // means (BVH queries per evaluation) and larger ppc use the lane-per-ray k_march.
+static CachedConsts cached_consts(const DevGP& g) {
+  CachedConsts cc;
+  cc.inv_kr = 1.0f / g.kr;
+  int e = 0;
+  cc.kr_pow2 = std::frexp(g.kr, &e) == 0.5f ? 1 : 0;
+  cc.psi_k = (float)(-1.4426950408889634 / (double)g.L2);
+  return cc;
+}
+
 static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma, hipStream_t st) {
   if (S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached) {
-    CachedConsts cc;
-    cc.inv_kr = 1.0f / S.gp.kr;
-    int e = 0;
-    cc.kr_pow2 = std::frexp(S.gp.kr, &e) == 0.5f ? 1 : 0;
-    cc.psi_k = (float)(-1.4426950408889634 / (double)S.gp.L2);
+    const CachedConsts cc = cached_consts(S.gp);
     const dim3 grid(ctx->march_cached_blocks), block(64);
     switch (S.gp.ppc) {
       case 1: hipLaunchKernelGGL(k_march_cached<1>, grid, block, 0, st, S, ma, cc); break;
@@ -2011,7 +2328,7 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
       default: hipLaunchKernelGGL(k_march_cached<8>, grid, block, 0, st, S, ma, cc); break;
     }
   } else {
-    hipLaunchKernelGGL(k_march, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
+    hipLaunchKernelGGL(k_march<false>, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
   }
 }
 
@@ -2031,8 +2348,8 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
   const DevScene& S = sc->ds;
   const int ni = (int)n;
   hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, pb.counters, 8, ni, 0);
-  InitArgs ia{pb.P, src, ni, pb.qA[0]};
-  hipLaunchKernelGGL(k_init, dim3(grid_for(n)), dim3(256), 0, st, S, ia);
+  InitArgs ia{pb.P, src, ni, pb.qA[0], nullptr};
+  hipLaunchKernelGGL(k_init<false>, dim3(grid_for(n)), dim3(256), 0, st, S, ia);
   HIP_TRY(hipGetLastError());
   std::vector<std::pair<hipEvent_t, hipEvent_t>> evs, gevs;
   mupsi_status status = MUPSI_OK;
@@ -2101,8 +2418,8 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
     int* nS = pb.counters + 7;
     if (defer) hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nS, 1, 0, -1);
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
-                 defer ? 1 : 0, pb.SH, pb.qS, nS};
-    hipLaunchKernelGGL(k_shade, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
+                 defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
     if (defer) {
       // transmittance of this iteration's medium light samples (gpmedium.cpp:105-129)
@@ -2200,7 +2517,7 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
                                            guide->cfg.train_passes - guide->stats.train_passes_done,
                                            (int64_t)(k_begin + k_count - kb), (int64_t)ppl})
                  : std::min(ppl, k_begin + k_count - kb);
-    PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr};
+    PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     GuideRun gr{guide, training ? 1 : 0, 0.0};
     st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches,
                        guide ? &gr : nullptr);
@@ -2330,7 +2647,7 @@ extern "C" mupsi_status mupsi_trace_paths(mupsi_scene* sc, const int32_t* x, con
   HIP_TRY(tmp.upload(k, n, &dk));
   PathBuffers& pb = *sc->pb;
   HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), sc->ctx->stream));
-  PathSrc src{sc->ds.w, sc->ds.w * sc->ds.h, 0, dx, dy, dk};
+  PathSrc src{sc->ds.w, sc->ds.w * sc->ds.h, 0, dx, dy, dk, nullptr, 0, nullptr};
   st = run_wavefront(sc, src, n, nullptr, nullptr, nullptr);
   if (st != MUPSI_OK) return st;
   float* drgb;
@@ -2340,6 +2657,362 @@ extern "C" mupsi_status mupsi_trace_paths(mupsi_scene* sc, const int32_t* x, con
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
   HIP_TRY(hipMemcpy(rgb_out, drgb, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+  return MUPSI_OK;
+}
+
+// ====================================================================== host: batched scenes
+
+struct mupsi_batch {
+  mupsi_ctx* ctx = nullptr;
+  std::vector<mupsi_scene*> scenes;
+  bool valid = true;  // false once a member scene was freed
+  DeviceBuffers mem;
+  DevScene* d_scenes = nullptr;
+  CachedConsts* d_cc = nullptr;
+  LaunchPlan* d_plan = nullptr;
+  int64_t* d_total = nullptr;
+  std::vector<LaunchPlan> plan;
+  std::unique_ptr<PathBuffers> pb;
+  int max_bounce = 1;
+  bool any_cached = false, any_plain = false;
+  mupsi_batch_tuning tuning{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_cap = 0;
+  int graph_blocks = 0;
+  mupsi_batch_stats stats{};
+  ~mupsi_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+// nA0 = the launch's path count (all paths start in queue A); every other counter 0.
+__global__ void k_batch_counters(int* c, const int64_t* total) {
+  const int i = threadIdx.x;
+  if (i < 16) c[i] = i == 0 ? (int)*total : 0;
+}
+
+static bool scene_uses_cached(const DevScene& S) {
+  return S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached;
+}
+
+// The kernels of one batched launch (paths of the device plan), on stream st.
+// Eager or under graph capture: nothing here synchronises or touches the host.
+static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
+  PathBuffers& pb = *b->pb;
+  const int64_t n = pb.capacity;
+  mupsi_ctx* ctx = b->ctx;
+  PathSrc src{0, 1, 0, nullptr, nullptr, nullptr, b->d_plan, (int)b->scenes.size(), b->d_total};
+  hipLaunchKernelGGL(k_batch_counters, dim3(1), dim3(64), 0, st, pb.counters, b->d_total);
+  InitArgs ia{pb.P, src, (int)n, pb.qA[0], b->d_scenes};
+  hipLaunchKernelGGL(k_init<true>, dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, ia);
+  const int per_cu = std::max(1, b->tuning.march_blocks_per_cu);
+  for (int it = 0; it < b->max_bounce; ++it) {
+    const int cur = it & 1, nxt = cur ^ 1;
+    int* nA = pb.counters + 2 * cur;
+    int* nM = pb.counters + 2 * cur + 1;
+    int* nMm = pb.counters + 8 + cur;
+    int* work = pb.counters + 4;
+    int* work2 = pb.counters + 10;
+    if (it > 0) {
+      MarchArgs ma{};
+      ma.ox = pb.P.ox; ma.oy = pb.P.oy; ma.oz = pb.P.oz;
+      ma.dx = pb.P.dx; ma.dy = pb.P.dy; ma.dz = pb.P.dz;
+      ma.nt = pb.P.nt;
+      ma.ft = nullptr;
+      ma.seed = pb.P.mseed;
+      ma.flags = pb.P.flags;
+      ma.cx = pb.P.cx; ma.cy = pb.P.cy; ma.cz = pb.P.cz; ma.cu = pb.P.cu;
+      ma.cgx = pb.P.cgx; ma.cgy = pb.P.cgy; ma.cgz = pb.P.cgz;
+      ma.hit = pb.P.mhit;
+      ma.t = pb.P.mt; ma.px = pb.P.mpx; ma.py = pb.P.mpy; ma.pz = pb.P.mpz;
+      ma.stats = pb.stats;
+      ma.scenes = b->d_scenes;
+      ma.sid = pb.P.sid;
+      if (b->any_cached) {
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work, 1, 0, -1);
+        ma.queue = pb.qM[cur];
+        ma.count_dev = nM;
+        ma.work = work;
+        hipLaunchKernelGGL(k_march_multi, dim3(per_cu * ctx->cus * 2), dim3(64), 0, st, ma, b->d_cc);
+      }
+      if (b->any_plain) {
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work2, 1, 0, -1);
+        ma.queue = pb.qMm[cur];
+        ma.count_dev = nMm;
+        ma.work = work2;
+        hipLaunchKernelGGL(k_march<true>, dim3(per_cu * ctx->cus), dim3(256), 0, st, DevScene{}, ma);
+      }
+    }
+    int* nA2 = pb.counters + 2 * nxt;
+    int* nMm2 = pb.counters + 8 + nxt;
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nA2, 2, 0, -1);
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nMm2, 1, 0, -1);
+    GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
+    ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], pb.counters + 2 * nxt + 1,
+                 gs, 0, ShadowSoA{}, nullptr, nullptr, b->d_scenes,
+                 b->any_plain ? pb.qMm[nxt] : nullptr, nMm2};
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
+  }
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    const mupsi_scene* sc = b->scenes[i];
+    const int npix = sc->ds.w * sc->ds.h;
+    hipLaunchKernelGGL(k_accum_plan, dim3(grid_for(npix)), dim3(256), 0, st, pb.P.Lx, pb.P.Ly,
+                       pb.P.Lz, b->d_plan, (int)i, sc->accum);
+  }
+}
+
+static mupsi_status batch_buffers(mupsi_batch* b, int64_t cap) {
+  if (b->pb && b->pb->capacity == cap) return MUPSI_OK;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return alloc_path_buffers(b->pb, cap, false, b->any_plain);
+}
+
+// Capture the launch's kernels into a graph for the current buffers and tuning.
+static mupsi_status batch_graph(mupsi_batch* b) {
+  const int blocks = std::max(1, b->tuning.march_blocks_per_cu);
+  if (b->exec && b->graph_cap == b->pb->capacity && b->graph_blocks == blocks) return MUPSI_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  hipStream_t st = b->ctx->stream;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  enqueue_batch_launch(b, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess) return fail(MUPSI_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_cap = b->pb->capacity;
+  b->graph_blocks = blocks;
+  b->stats.graph_builds += 1;
+  return MUPSI_OK;
+}
+
+// Fill the next launch: every unfinished scene gets passes up to a fair share of
+// `cap` paths (at least one pass while it fits), in a rotating order; offsets
+// are assigned in scene order, as the plan search in k_init expects.
+static int64_t batch_plan(mupsi_batch* b, std::vector<int>& next_k, std::vector<int>& left,
+                          int64_t cap, size_t rotate) {
+  const size_t ns = b->scenes.size();
+  std::vector<int> kc(ns, 0);
+  size_t active = 0;
+  for (size_t s = 0; s < ns; ++s) active += left[s] > 0;
+  if (active == 0) return 0;
+  const int64_t fair = std::max<int64_t>(1, cap / (int64_t)active);
+  int64_t used = 0;
+  for (size_t i = 0; i < ns; ++i) {
+    const size_t s = (rotate + i) % ns;
+    if (left[s] <= 0) continue;
+    const int64_t npix = (int64_t)b->scenes[s]->ds.w * b->scenes[s]->ds.h;
+    int64_t want = std::min<int64_t>(left[s], std::max<int64_t>(1, fair / npix));
+    want = std::min<int64_t>(want, (cap - used) / npix);
+    if (want <= 0) continue;
+    kc[s] = (int)want;
+    used += want * npix;
+  }
+  int64_t off = 0;
+  for (size_t s = 0; s < ns; ++s) {
+    const mupsi_scene* sc = b->scenes[s];
+    const int npix = sc->ds.w * sc->ds.h;
+    b->plan[s] = LaunchPlan{off, npix, sc->ds.w, next_k[s], kc[s]};
+    off += (int64_t)kc[s] * npix;
+    next_k[s] += kc[s];
+    left[s] -= kc[s];
+  }
+  return off;
+}
+
+static mupsi_status batch_launch(mupsi_batch* b, int64_t total, double* ms) {
+  hipStream_t st = b->ctx->stream;
+  HIP_TRY(hipMemcpyAsync(b->d_plan, b->plan.data(), b->plan.size() * sizeof(LaunchPlan),
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(b->d_total, &total, sizeof(int64_t), hipMemcpyHostToDevice, st));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, st));
+  if (b->tuning.use_graph) {
+    mupsi_status s2 = batch_graph(b);
+    if (s2 != MUPSI_OK) return s2;
+    HIP_TRY(hipGraphLaunch(b->exec, st));
+  } else {
+    enqueue_batch_launch(b, st);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t;
+  for (size_t s = 0; s < b->scenes.size(); ++s) b->scenes[s]->sample_count += b->plan[s].kc;
+  b->stats.paths += total;
+  b->stats.launches += 1;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_batch_create(mupsi_ctx* ctx, mupsi_scene* const* scenes, int32_t n,
+                                           mupsi_batch** out) {
+  if (!ctx || !scenes || !out || n <= 0) return fail(MUPSI_ERR_INVALID, "mupsi_batch_create: bad argument");
+  if (n > 65535) return fail(MUPSI_ERR_INVALID, "mupsi_batch_create: at most 65535 scenes");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto b = std::make_unique<mupsi_batch>();
+  b->ctx = ctx;
+  std::vector<DevScene> ds;
+  std::vector<CachedConsts> cc;
+  for (int i = 0; i < n; ++i) {
+    mupsi_scene* sc = scenes[i];
+    if (!sc || sc->ctx != ctx) return fail(MUPSI_ERR_INVALID, "mupsi_batch_create: scene of another context");
+    b->scenes.push_back(sc);
+    ds.push_back(sc->ds);
+    cc.push_back(cached_consts(sc->ds.gp));
+    b->max_bounce = std::max(b->max_bounce, sc->ds.max_bounce);
+    if (sc->ds.has_gp && !sc->ds.skip_medium) {
+      if (scene_uses_cached(sc->ds)) b->any_cached = true;
+      else b->any_plain = true;
+    }
+  }
+  const DevScene* dds;
+  HIP_TRY(b->mem.upload(ds.data(), ds.size(), &dds));
+  b->d_scenes = const_cast<DevScene*>(dds);
+  const CachedConsts* dcc;
+  HIP_TRY(b->mem.upload(cc.data(), cc.size(), &dcc));
+  b->d_cc = const_cast<CachedConsts*>(dcc);
+  HIP_TRY(b->mem.alloc(n, &b->d_plan));
+  HIP_TRY(b->mem.alloc(1, &b->d_total));
+  b->plan.assign(n, LaunchPlan{});
+  b->tuning.use_graph = 1;
+  ctx->batches.push_back(b.get());
+  *out = b.release();
+  return MUPSI_OK;
+}
+
+static void destroy_batch(mupsi_batch* b) {
+  (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+}
+
+// A freed scene invalidates every batch that holds it.
+static void batch_forget_scene(mupsi_ctx* ctx, const mupsi_scene* scene) {
+  for (mupsi_batch* b : ctx->batches)
+    if (std::find(b->scenes.begin(), b->scenes.end(), scene) != b->scenes.end()) b->valid = false;
+}
+
+extern "C" void mupsi_batch_free(mupsi_batch* b) {
+  if (!b) return;
+  mupsi_ctx* ctx = b->ctx;
+  auto it = std::find(ctx->batches.begin(), ctx->batches.end(), b);
+  if (it != ctx->batches.end()) ctx->batches.erase(it);
+  (void)hipSetDevice(ctx->device);
+  destroy_batch(b);
+}
+
+extern "C" mupsi_status mupsi_batch_set_tuning(mupsi_batch* b, const mupsi_batch_tuning* t) {
+  if (!b || !t) return fail(MUPSI_ERR_INVALID, "mupsi_batch_set_tuning: null argument");
+  if (t->paths_per_launch < 0 || t->march_blocks_per_cu < 0 || t->march_blocks_per_cu > 64)
+    return fail(MUPSI_ERR_INVALID, "mupsi_batch_set_tuning: bad tuning");
+  b->tuning = *t;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_batch_get_tuning(mupsi_batch* b, mupsi_batch_tuning* t) {
+  if (!b || !t) return fail(MUPSI_ERR_INVALID, "mupsi_batch_get_tuning: null argument");
+  *t = b->tuning;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_batch_stats_get(mupsi_batch* b, mupsi_batch_stats* out) {
+  if (!b || !out) return fail(MUPSI_ERR_INVALID, "mupsi_batch_stats_get: null argument");
+  *out = b->stats;
+  return MUPSI_OK;
+}
+
+// Autotuner candidates: paths per launch x persistent marcher blocks per CU.
+static const int64_t kTuneCaps[] = {1LL << 21, 1LL << 22, 1LL << 23, 1LL << 24};
+static const int kTuneBlocks[] = {2, 4, 8};
+
+extern "C" mupsi_status mupsi_batch_render(mupsi_batch* b, const int32_t* k_begin,
+                                           const int32_t* k_count,
+                                           const volatile int32_t* abort_flag) {
+  if (!b || !k_begin || !k_count) return fail(MUPSI_ERR_INVALID, "mupsi_batch_render: null argument");
+  if (!b->valid) return fail(MUPSI_ERR_INVALID, "mupsi_batch_render: a scene of the batch was freed");
+  const size_t ns = b->scenes.size();
+  std::vector<int> next_k(ns), left(ns);
+  int64_t work = 0;
+  int64_t min_fit = 0;  // the largest single pass: a launch must hold it
+  for (size_t s = 0; s < ns; ++s) {
+    if (k_begin[s] < 0 || k_count[s] < 0) return fail(MUPSI_ERR_INVALID, "mupsi_batch_render: negative pass range");
+    next_k[s] = k_begin[s];
+    left[s] = k_count[s];
+    const int64_t npix = (int64_t)b->scenes[s]->ds.w * b->scenes[s]->ds.h;
+    work += npix * k_count[s];
+    if (k_count[s] > 0) min_fit = std::max(min_fit, npix);
+  }
+  if (work == 0) return MUPSI_OK;
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  size_t rotate = 0;
+  // autotuning: probe the candidates on real passes, keep the fastest per path
+  if (b->tuning.paths_per_launch == 0 || b->tuning.march_blocks_per_cu == 0) {
+    const auto tt0 = std::chrono::steady_clock::now();
+    double best = 1e300;
+    mupsi_batch_tuning best_t = b->tuning;
+    for (int64_t cap : kTuneCaps) {
+      if (cap < min_fit) continue;
+      for (int blk : kTuneBlocks) {
+        if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
+        mupsi_batch_tuning t = b->tuning;
+        t.paths_per_launch = std::min<int64_t>(cap, std::max<int64_t>(work, min_fit));
+        t.march_blocks_per_cu = blk;
+        b->tuning = t;
+        mupsi_status st = batch_buffers(b, t.paths_per_launch);
+        if (st != MUPSI_OK) return st;
+        const int64_t total = batch_plan(b, next_k, left, t.paths_per_launch, rotate++);
+        if (total == 0) break;
+        double ms = 0;
+        // a first launch on a new graph includes its capture: time a second one
+        st = batch_launch(b, total, &ms);
+        if (st != MUPSI_OK) return st;
+        const int64_t total2 = batch_plan(b, next_k, left, t.paths_per_launch, rotate++);
+        if (total2 > 0) {
+          st = batch_launch(b, total2, &ms);
+          if (st != MUPSI_OK) return st;
+          const double per = ms / (double)total2;
+          if (per < best) {
+            best = per;
+            best_t = t;
+          }
+        }
+      }
+    }
+    if (best < 1e300) b->tuning = best_t;
+    else if (b->tuning.paths_per_launch == 0) b->tuning.paths_per_launch = std::max<int64_t>(min_fit, 1 << 22);
+    if (b->tuning.march_blocks_per_cu == 0) b->tuning.march_blocks_per_cu = 4;
+    b->stats.tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
+  }
+  const int64_t cap = std::max(b->tuning.paths_per_launch, min_fit);
+  mupsi_status st = batch_buffers(b, cap);
+  if (st != MUPSI_OK) return st;
+  while (true) {
+    if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
+    const int64_t total = batch_plan(b, next_k, left, cap, rotate++);
+    if (total == 0) break;
+    st = batch_launch(b, total, nullptr);
+    if (st != MUPSI_OK) return st;
+  }
+  b->stats.total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return MUPSI_OK;
 }
 

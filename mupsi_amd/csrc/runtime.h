@@ -20,6 +20,7 @@ struct mupsi_ctx {
   int march_cached_blocks = 0;  // persistent grid of k_march_cached
   std::vector<mupsi_scene*> scenes;  // live scenes, freed by mupsi_close
   std::vector<mupsi_guide*> guides;  // live guides, freed by mupsi_close
+  std::vector<mupsi_batch*> batches; // live batches, freed by mupsi_close
 };
 
 namespace mupsi {

@@ -4,6 +4,7 @@
   sun lobe of peak 50.
 - `icosphere`: a radius-1 icosphere subdivided n times, optionally displaced
   along the vertex normals by seeded 3-octave value noise.
+- `config_batch(seed)`: the 64 uneven scenes of the batched configuration.
 - `config_scene(i)`: the scene dicts of configurations 1-3 and 5 (sphere mean,
   single realization / renewal_plus; displaced-icosphere mesh mean; the
   renewal_plus sphere scene with a mirror sphere and a small emissive quad,
@@ -189,3 +190,51 @@ def emissive_quad(center, side: float) -> dict:
                   [cx - h, cy, cz + h]], np.float32)
     f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
     return {"vertices": v, "faces": f, "normals": None, "texcoords": None, "face_tex": None}
+
+
+BATCH_SEED = 4
+
+
+def config_batch(seed: int = BATCH_SEED, n: int = 64, max_res: int = 512,
+                 max_subdivisions: int = 7) -> list[dict]:
+    """The batched configuration: n seeded renewal_plus scenes, sphere means (the
+    configuration-2 geometry) and displaced-icosphere mesh means (the
+    configuration-3 geometry) alternating at random; square resolution uniform
+    in 64..max_res, spp uniform in 16..128, kernel length_scale (the cell size)
+    log-uniform in [0.05, 5], sigma uniform in [0.1, 3], points_per_cell uniform
+    in 1..8, icosphere subdivisions uniform in 2..max_subdivisions."""
+    rng = np.random.default_rng(seed)
+    sky = env_map()
+    meshes: dict[int, dict] = {}
+    out = []
+    for i in range(n):
+        res = int(rng.integers(min(64, max_res), max_res + 1))
+        spp = int(rng.integers(16, 129))
+        kernel = {"sigma": float(rng.uniform(0.1, 3.0)),
+                  "length_scale": float(np.exp(rng.uniform(np.log(0.05), np.log(5.0)))),
+                  "aniso": [1, 1, 1], "points_per_cell": int(rng.integers(1, 9))}
+        if rng.uniform() < 0.5:
+            sc = config_scene(2, width=res, height=res)
+            sc["kernel"] = kernel
+        else:
+            sub = int(rng.integers(2, max_subdivisions + 1))
+            if sub not in meshes:
+                meshes[sub] = icosphere(sub, displacement=0.05)
+            sc = {
+                "width": res, "height": res, "max_bounce": 10, "max_medium_bounce": 10,
+                "gp_mode": "renewal_plus", "camera": dict(CAMERA), "kernel": kernel,
+                "gp_medium": {"mean_type": "mesh", "mesh": meshes[sub],
+                              "transform": {"scale": 2.0, "position": [0.0, 0.0, -5.0]}},
+                "skydrome": sky,
+                "primitives": [
+                    {"type": "sphere", "center": [0, 0, -5], "radius": 3, "bsdf": "null",
+                     "int_medium": True},
+                    {"type": "mesh", "mesh": ground_quad(-2.2), "bsdf": "lambertian",
+                     "texture": np.full((1, 1, 3), 0.7, np.float32)},
+                ],
+            }
+        sc["spp"] = spp
+        sc["name"] = f"batch{i:02d}"
+        out.append(sc)
+    return out
+
