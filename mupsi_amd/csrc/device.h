@@ -121,6 +121,12 @@ struct DevMesh {
   uint32_t root;
   DevXf xf;
   float lbmin[3], lbmax[3];  // local-space bounds of the whole mesh
+  // distance to the mesh from the centres of a grid of cubes (local space), or
+  // null: dist(p, mesh) lies within dgrid[(iz * dn[1] + iy) * dn[0] + ix] +- dhalf
+  // (the half diagonal plus a rounding margin) for every p of that cube
+  const float* dlb;
+  int dn[3];
+  float dorg[3], dinv, dhalf;
 };
 struct DevPrim {
   int type, bsdf;
@@ -520,6 +526,34 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p) {
   }
   return (hits % 2) == 1;
 }
+// Bounds of the local-space distance from pl to the mesh: lo from the mesh's
+// bounding box and the distance grid, hi from the grid (infinite outside it).
+__device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, float& lo,
+                                                       float& hi) {
+  const float dx = fmaxf(fmaxf(m.lbmin[0] - pl.x, 0.0f), pl.x - m.lbmax[0]);
+  const float dy = fmaxf(fmaxf(m.lbmin[1] - pl.y, 0.0f), pl.y - m.lbmax[1]);
+  const float dz = fmaxf(fmaxf(m.lbmin[2] - pl.z, 0.0f), pl.z - m.lbmax[2]);
+  lo = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+  hi = INFINITY;
+  if (m.dlb) {
+    const float fx = (pl.x - m.dorg[0]) * m.dinv, fy = (pl.y - m.dorg[1]) * m.dinv,
+                fz = (pl.z - m.dorg[2]) * m.dinv;
+    if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)m.dn[0] && fy < (float)m.dn[1] &&
+        fz < (float)m.dn[2]) {
+      const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+      const float dc = m.dlb[((size_t)iz * m.dn[1] + iy) * m.dn[0] + ix];
+      lo = fmaxf(lo, dc - m.dhalf);
+      hi = dc + m.dhalf;
+    }
+  }
+}
+// Whether the +x parity ray of Mesh::inside can reach the mesh's bounding box
+// from pl (otherwise inside() is exactly false).
+__device__ __forceinline__ bool mesh_in_parity_slab(const DevMesh& m, F3 pl) {
+  return pl.y >= m.lbmin[1] && pl.y <= m.lbmax[1] && pl.z >= m.lbmin[2] && pl.z <= m.lbmax[2] &&
+         m.lbmax[0] - pl.x >= 0.0f;
+}
+
 __device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {
   if (m.xf.identity) return n;
   return normalized(mul(n, m.xf.inv_s));

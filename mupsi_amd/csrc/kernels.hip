@@ -550,9 +550,7 @@ __device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo&
     const float dw = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) * M.xf.s;
     if (!(dw > k.Bm)) return 0;
     // the root-box test of the +x parity ray (AABB::intersect with d = (1, 0, 0))
-    const bool in_slab = pl.y >= M.lbmin[1] && pl.y <= M.lbmax[1] && pl.z >= M.lbmin[2] &&
-                         pl.z <= M.lbmax[2] && M.lbmax[0] - pl.x >= 0.0f;
-    if (in_slab) return mesh_inside(M, p) ? -1 : 1;
+    if (mesh_in_parity_slab(M, pl)) return mesh_inside(M, p) ? -1 : 1;
     const F3 rc = sub(p, k.cb);
     const float e = 1e-3f * (1.0f + fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
     const bool leaves = (p.x > k.wmax[0] + e && d.x >= 0.0f) || (p.y > k.wmax[1] + e && d.y >= 0.0f) ||
@@ -750,6 +748,55 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
 // Cached-window marcher (see march_cached.h): same decisions as k_march, two rays
 // per 64-lane workgroup, impulses regenerated once per cell entering the window.
 // Used for sphere means with points_per_cell <= 8.
+// f at q for the cached marchers, given the window's psi sum acc: the mean is
+// evaluated exactly unless, for a mesh mean, bounds of the distance decide the
+// sign: with mu = s d, d in [lo, hi] and s from inside() (called only where the
+// parity ray can reach the mesh), f lies in [s lo, s hi] + (psi term + conditioning
+// term) up to rounding; when that interval keeps clear of 0 by a margin, the
+// returned value carries the sign with a magnitude no larger than |f|.  Every
+// decision the marcher takes from it (sign, and |f_prev - f| > 1e-6 across a
+// sign change) is the one the exact value gives.
+__device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G, F3 q, float acc,
+                                              const Cond& c) {
+  const float psi_t = G.sigma * acc / G.sqrt_var;
+  float cond_t = 0.0f;
+  if (c.active) {
+    const F3 r = sub(q, c.C);
+    const float h = kern_h(G, c.C, q);
+    cond_t = fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
+  }
+  if (S.mean_type == MUPSI_MEAN_MESH && S.mean_mesh.xf.s > 0.0f) {
+    const DevMesh& M = S.mean_mesh;
+    const F3 pl = xf_to_local(M.xf, q);
+    float lo, hi;
+    mesh_dist_bounds_local(M, pl, lo, hi);
+    lo = lo * M.xf.s * (1.0f - 1e-5f);
+    hi = hi * M.xf.s * (1.0f + 1e-5f);
+    const float e = psi_t + cond_t;
+    const float margin = 1e-3f + 1e-5f * (fabsf(psi_t) + fabsf(cond_t) + lo);
+    if (lo > fabsf(e) + margin) {  // |mu| alone decides: no sign query beyond the parity ray
+      const float sg = (mesh_in_parity_slab(M, pl) && mesh_inside(M, q)) ? -1.0f : 1.0f;
+      return sg * (lo - fabsf(e));
+    }
+    if (hi < INFINITY) {
+      const float sg = (mesh_in_parity_slab(M, pl) && mesh_inside(M, q)) ? -1.0f : 1.0f;
+      const float a = sg * lo + e, b = sg * hi + e;   // f lies between a and b
+      const float fmin = fminf(a, b), fmax = fmaxf(a, b);
+      if (fmin > margin) return fmin;
+      if (fmax < -margin) return fmax;
+      // within the interval a zero crossing is possible: evaluate mu exactly
+      uint32_t ti = 0;
+      F3 o;
+      float f = sg * mesh_dist(M, q, ti, o) + psi_t;  // mean_eval's mu = sg * d
+      if (c.active) f = f + cond_t;
+      return f;
+    }
+  }
+  float f = mean_eval<false>(S, q, nullptr) + psi_t;
+  if (c.active) f = f + cond_t;
+  return f;
+}
+
 struct CachedConsts {
   float inv_kr;   // 1 / kr (rounded)
   int kr_pow2;    // kr is a power of two: q * inv_kr == q / kr exactly
@@ -767,12 +814,13 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
   const DevGP& G = S.gp;
   constexpr int ppc = PPC;
   constexpr int nslots = 27 * PPC;
-  const F3 mc = mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]);
   // ray state: identical in the 32 lanes of a group
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
   int rid = 0;
   F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f;
+  SkipInfo sk;
+  sk.kind = 0;
   bool sp = false;
   uint32_t seed = 0;
   Cond c;
@@ -825,11 +873,7 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            {
-              const SkipInfo sk = make_skip(S, c, o, ft);
-              outer2 = sk.outer2;
-              inner2 = sk.inner2;
-            }
+            sk = make_skip(S, c, o, ft);
             nsteps = 0;
             wvalid = false;  // new realization seed
             if (gl == 0) ++rays;
@@ -849,9 +893,8 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
             break;
           }
           const F3 p = madd(d, t, o);
-          const F3 r = sub(p, mc);
-          const float q2 = sq3(r);
-          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          bool fin;
+          const int cs = classify_point(S, sk, p, d, &fin);
           if (cs == 0) {
             q = p;
             do_eval = true;
@@ -862,7 +905,7 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
             finish(true, p);
             break;
           }
-          if ((cs > 0 && dot(d, r) >= 0.0f) || !(t + dt > t)) {
+          if (fin || !(t + dt > t)) {  // no later sign change / the grid cannot advance
             finish(false, p);
             break;
           }
@@ -925,12 +968,7 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
     for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
 
     if (do_eval) {
-      float f = mean_eval<false>(S, q, nullptr) + G.sigma * acc / G.sqrt_var;
-      if (c.active) {
-        const F3 r = sub(q, c.C);
-        const float h = kern_h(G, c.C, q);
-        f = f + fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
-      }
+      const float f = cached_field(S, G, q, acc, c);
       if (gl == 0) ++evals;
       if (phase == 1) {
         fprev = f;
@@ -959,7 +997,21 @@ __global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, Ca
   }
 }
 
-// Batched launch: the same marcher for rays of sphere-mean scenes with any
+// This lane's part of the window sum: slots gl, gl + 32, ... of the 27 x PPC.
+template <int PPC>
+__device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float psi_k) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int sl = gl; sl < 27 * PPC; sl += kGroup) {
+    const float4 im = W[sl];
+    const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
+    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+    acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * psi_k), acc);
+  }
+  return acc;
+}
+
+// Batched launch: the same marcher for rays of every scene with
 // points_per_cell <= 8, each 32-lane group taking its ray's scene constants.
 __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedConsts* ccs) {
   __shared__ float4 win[2][27 * kMaxPpcCached];
@@ -971,13 +1023,14 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
   // the group's ray and its scene
   DevGP G{};
   int ppc = 1, nslots = 27, sidx = 0;
-  F3 mc = mk(0, 0, 0);
   CachedConsts cc{};
   // ray state: identical in the 32 lanes of a group
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
   int rid = 0;
   F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f, outer2 = 0.f, inner2 = 0.f;
+  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f;
+  SkipInfo sk;
+  sk.kind = 0;
   bool sp = false;
   uint32_t seed = 0;
   Cond c;
@@ -1022,7 +1075,6 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
             G = a.scenes[sidx].gp;
             ppc = G.ppc;
             nslots = 27 * ppc;
-            mc = mk(a.scenes[sidx].mean_c[0], a.scenes[sidx].mean_c[1], a.scenes[sidx].mean_c[2]);
             cc = ccs[sidx];
             o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
             d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
@@ -1036,11 +1088,7 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            {
-              const SkipInfo sk = make_skip(a.scenes[sidx], c, o, ft);
-              outer2 = sk.outer2;
-              inner2 = sk.inner2;
-            }
+            sk = make_skip(a.scenes[sidx], c, o, ft);
             nsteps = 0;
             wvalid = false;  // new realization seed
             if (gl == 0) ++rays;
@@ -1060,9 +1108,8 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
             break;
           }
           const F3 p = madd(d, t, o);
-          const F3 r = sub(p, mc);
-          const float q2 = sq3(r);
-          const int cs = q2 > outer2 ? 1 : (q2 < inner2 ? -1 : 0);
+          bool fin;
+          const int cs = classify_point(a.scenes[sidx], sk, p, d, &fin);
           if (cs == 0) {
             q = p;
             do_eval = true;
@@ -1073,7 +1120,7 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
             finish(true, p);
             break;
           }
-          if ((cs > 0 && dot(d, r) >= 0.0f) || !(t + dt > t)) {
+          if (fin || !(t + dt > t)) {  // no later sign change / the grid cannot advance
             finish(false, p);
             break;
           }
@@ -1125,22 +1172,21 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
 
     float acc = 0.0f;
     if (do_eval) {
-      for (int sl = gl; sl < nslots; sl += kGroup) {
-        const float4 im = W[sl];
-        const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
-        const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-        acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * cc.psi_k), acc);
+      switch (ppc) {  // the compile-time slot count of k_march_cached, per ray
+        case 1: acc = window_sum<1>(W, q, gl, cc.psi_k); break;
+        case 2: acc = window_sum<2>(W, q, gl, cc.psi_k); break;
+        case 3: acc = window_sum<3>(W, q, gl, cc.psi_k); break;
+        case 4: acc = window_sum<4>(W, q, gl, cc.psi_k); break;
+        case 5: acc = window_sum<5>(W, q, gl, cc.psi_k); break;
+        case 6: acc = window_sum<6>(W, q, gl, cc.psi_k); break;
+        case 7: acc = window_sum<7>(W, q, gl, cc.psi_k); break;
+        default: acc = window_sum<8>(W, q, gl, cc.psi_k); break;
       }
     }
     for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
 
     if (do_eval) {
-      float f = mean_eval<false>(a.scenes[sidx], q, nullptr) + G.sigma * acc / G.sqrt_var;
-      if (c.active) {
-        const F3 r = sub(q, c.C);
-        const float h = kern_h(G, c.C, q);
-        f = f + fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
-      }
+      const float f = cached_field(a.scenes[sidx], G, q, acc, c);
       if (gl == 0) ++evals;
       if (phase == 1) {
         fprev = f;
@@ -1580,7 +1626,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   }
   wave_push(cont, pid, a.qA2, a.nA2);
   if constexpr (kMulti) {
-    const bool cached = S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached;
+    const bool cached = S.gp.ppc <= kMaxPpcCached;
     wave_push(cont && medium && !S.skip_medium && cached, pid, a.qM2, a.nM2);
     wave_push(cont && medium && !S.skip_medium && !cached, pid, a.qMm2, a.nMm2);
   } else {
@@ -2104,6 +2150,48 @@ struct mupsi_scene {
   mupsi_stats last{};
 };
 
+// Distances from the centres of a grid of cubes to a mean mesh (local space): for
+// any point of a cube the distance is within the half diagonal of the centre's
+// (distance is 1-Lipschitz).
+__global__ __launch_bounds__(256) void k_dist_grid(DevMesh m, float cell, float* out) {
+  const int n = m.dn[0] * m.dn[1] * m.dn[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ix = i % m.dn[0], iy = (i / m.dn[0]) % m.dn[1], iz = i / (m.dn[0] * m.dn[1]);
+  const F3 c = mk(m.dorg[0] + (ix + 0.5f) * cell, m.dorg[1] + (iy + 0.5f) * cell,
+                  m.dorg[2] + (iz + 0.5f) * cell);
+  uint32_t ti = 0;
+  F3 o;
+  const float d = bvh_closest(m, c, ti, o);
+  out[i] = d;
+}
+
+static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
+  float mx = 0.0f;
+  for (int a = 0; a < 3; ++a) mx = std::max(mx, m->lbmax[a] - m->lbmin[a]);
+  if (m->n_nodes == 0 || !(mx > 0.0f) || !std::isfinite(mx)) return MUPSI_OK;
+  constexpr int kGrid = 64;
+  const float pad = 0.25f * mx;
+  const float cell = (mx + 2.0f * pad) / kGrid;
+  for (int a = 0; a < 3; ++a) {
+    m->dorg[a] = m->lbmin[a] - pad;
+    m->dn[a] = std::max(1, std::min(kGrid, (int)std::ceil((m->lbmax[a] - m->lbmin[a] + 2.0f * pad) / cell)));
+  }
+  m->dinv = 1.0f / cell;
+  // half diagonal of a cube, widened for the rounding of the centre distance
+  m->dhalf = cell * 0.8660254037844386f * (1.0f + 1e-5f) + 1e-6f * (1.0f + mx);
+  const int n = m->dn[0] * m->dn[1] * m->dn[2];
+  float* grid = nullptr;
+  HIP_TRY(mem.alloc(n, &grid));
+  DevMesh mm = *m;
+  mm.dlb = nullptr;
+  hipLaunchKernelGGL(k_dist_grid, dim3(grid_for(n)), dim3(256), 0, 0, mm, cell, grid);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  m->dlb = grid;
+  return MUPSI_OK;
+}
+
 static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh* out) {
   DevMesh m{};
   const BvhNode* nodes;
@@ -2153,6 +2241,8 @@ extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_des
   }
   if (H.has_mean_mesh) {
     st = upload_mesh(sc->mem, H.mean_mesh, &sc->mean_mesh);
+    if (st != MUPSI_OK) return st;
+    st = build_dist_grid(sc->mem, &sc->mean_mesh);
     if (st != MUPSI_OK) return st;
   }
   D.mean_mesh = sc->mean_mesh;
@@ -2302,8 +2392,9 @@ extern "C" mupsi_status mupsi_scene_guide_records(mupsi_scene* sc, float* feats,
   return MUPSI_OK;
 }
 
-// The cached-window marcher serves sphere means with points_per_cell <= 8; mesh
-// means (BVH queries per evaluation) and larger ppc use the lane-per-ray k_march.
+// The cached-window marcher serves every scene with points_per_cell <= 8 (mesh
+// means evaluate their BVH only near the surface, see cached_field); larger ppc
+// uses the lane-per-ray k_march.
 static CachedConsts cached_consts(const DevGP& g) {
   CachedConsts cc;
   cc.inv_kr = 1.0f / g.kr;
@@ -2314,7 +2405,7 @@ static CachedConsts cached_consts(const DevGP& g) {
 }
 
 static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma, hipStream_t st) {
-  if (S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached) {
+  if (S.gp.ppc <= kMaxPpcCached) {
     const CachedConsts cc = cached_consts(S.gp);
     const dim3 grid(ctx->march_cached_blocks), block(64);
     switch (S.gp.ppc) {
@@ -2693,9 +2784,7 @@ __global__ void k_batch_counters(int* c, const int64_t* total) {
   if (i < 16) c[i] = i == 0 ? (int)*total : 0;
 }
 
-static bool scene_uses_cached(const DevScene& S) {
-  return S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached;
-}
+static bool scene_uses_cached(const DevScene& S) { return S.gp.ppc <= kMaxPpcCached; }
 
 // The kernels of one batched launch (paths of the device plan), on stream st.
 // Eager or under graph capture: nothing here synchronises or touches the host.
@@ -2940,8 +3029,10 @@ extern "C" mupsi_status mupsi_batch_stats_get(mupsi_batch* b, mupsi_batch_stats*
 }
 
 // Autotuner candidates: paths per launch x persistent marcher blocks per CU.
-static const int64_t kTuneCaps[] = {1LL << 21, 1LL << 22, 1LL << 23, 1LL << 24};
-static const int kTuneBlocks[] = {2, 4, 8};
+// Larger launches amortise the tail of each bounce loop (few live paths, idle
+// CUs), so the caps reach 32M paths (about 5 GB of path state).
+static const int64_t kTuneCaps[] = {1LL << 22, 1LL << 23, 1LL << 24, 1LL << 25};
+static const int kTuneBlocks[] = {4, 8, 16};
 
 extern "C" mupsi_status mupsi_batch_render(mupsi_batch* b, const int32_t* k_begin,
                                            const int32_t* k_count,
@@ -2998,8 +3089,8 @@ extern "C" mupsi_status mupsi_batch_render(mupsi_batch* b, const int32_t* k_begi
       }
     }
     if (best < 1e300) b->tuning = best_t;
-    else if (b->tuning.paths_per_launch == 0) b->tuning.paths_per_launch = std::max<int64_t>(min_fit, 1 << 22);
-    if (b->tuning.march_blocks_per_cu == 0) b->tuning.march_blocks_per_cu = 4;
+    else if (b->tuning.paths_per_launch == 0) b->tuning.paths_per_launch = std::max<int64_t>(min_fit, 1 << 24);
+    if (b->tuning.march_blocks_per_cu == 0) b->tuning.march_blocks_per_cu = 8;
     b->stats.tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
   }
   const int64_t cap = std::max(b->tuning.paths_per_launch, min_fit);

@@ -18,17 +18,19 @@ def _medium_pixels(W, H, rng, n):
     return x, y
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 5])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 5])
 def test_path_replay_matches_oracle(ctx, cfg):
     """Whole paths against the oracle; configuration 5 adds the emissive quad, so
     every medium hit takes a light sample whose shadow ray the GPU marches in a
-    separate queue (the oracle marches it serially)."""
+    separate queue (the oracle marches it serially); configuration 3 is the
+    mesh mean (a 3-times subdivided icosphere keeps the oracle's faithful
+    BVH-per-grid-point march fast enough)."""
     from mupsi_amd import Scene
-    sc = scenes.config_scene(cfg)
+    sc = scenes.config_scene(cfg, mesh_subdivisions=3) if cfg == 3 else scenes.config_scene(cfg)
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
     rng = np.random.default_rng(40 + cfg)
-    n = 1500
+    n = 300 if cfg == 3 else 1500
     x, y = _medium_pixels(sc["width"], sc["height"], rng, n)
     k = rng.integers(0, 64, n).astype(np.int32)
     got = s.trace_paths(x, y, k)
