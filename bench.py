@@ -195,6 +195,141 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
     return out
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def traversal_bench(ctx, sc: dict, n: int = 1 << 22) -> dict:
+    """BVH traversal memory roofline on the mean mesh of `sc`.
+
+    n query points in a shell around the surface (a random vertex scaled by
+    U[0.75, 1.25], then the mean's placement): the region where a mesh-mean
+    march needs exact distance and inside queries.  Algorithmic bytes per query
+    are 32 per node whose bounds are tested plus 52 per triangle tested (index,
+    face, three vertices), counted by the same traversal
+    (mupsi_mesh_query_cost); the kernel time is the HIP-event time of the query
+    kernel on the library's stream (median of 3).  The closest-point search
+    starts from the distance grid's bound (as the renderer runs it); the
+    unbounded search's counts are reported beside it."""
+    import numpy as np
+    import torch
+    from mupsi_amd import Scene, batch
+    s = Scene(ctx, sc)
+    m = sc["gp_medium"]["mesh"]
+    xf = sc["gp_medium"]["transform"]
+    rng = np.random.default_rng(3)
+    v = m["vertices"][rng.integers(0, len(m["vertices"]), n)].astype(np.float64)
+    v *= rng.uniform(0.75, 1.25, (n, 1))
+    p = (np.asarray(xf["position"]) + xf["scale"] * v).astype(np.float32)
+    pt = torch.from_numpy(p).cuda()
+    out = {"queries": n, "triangles": int(len(m["faces"])),
+           "points": "random vertex x U[0.75, 1.25], world space"}
+    n0, t0 = batch.mesh_query_cost(s, -1, 0, pt)
+    out["closest_unbounded_nodes_per_query"] = float(n0.sum().item()) / n
+    out["closest_unbounded_tris_per_query"] = float(t0.sum().item()) / n
+    for kind, name, fn in ((2, "closest", batch.mesh_closest), (1, "inside", batch.mesh_inside)):
+        nodes, tris = batch.mesh_query_cost(s, -1, kind, pt)
+        nb = int(nodes.sum().item())
+        tb = int(tris.sum().item())
+        algo = 32 * nb + 52 * tb
+        fn(s, -1, pt)
+        ms = []
+        for _ in range(3):
+            fn(s, -1, pt)
+            ms.append(batch.last_kernel_ms(ctx))
+        t = float(np.median(ms))
+        gbs = algo / (t * 1e-3) / 1e9
+        out[name] = {"kernel": "k_mesh_closest" if kind == 2 else "k_mesh_inside",
+                     "kernel_ms": t, "Mqueries_s": n / t / 1e3,
+                     "nodes_per_query": nb / n, "tris_per_query": tb / n,
+                     "algorithmic_bytes": algo, "GB_s": gbs,
+                     "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    s.close()
+    return out
+
+
+def configs_bench(ctx, cfg3_passes: int = 4, cfg4_div: int = 16) -> dict:
+    """Throughput on the other configurations: 1 (single_realization, 128x128x16,
+    every pass), 3 (mesh mean, 512x512, the first cfg3_passes of 64 passes: per-
+    path cost does not depend on the pass), 4 (64 generated scenes, the first
+    spp/cfg4_div passes of each, rendered scene by scene and as one autotuned
+    batch; the two must give identical images)."""
+    import numpy as np
+    import torch
+    from mupsi_amd import Batch, Scene, scenes
+    out = {}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    sc1 = scenes.config_scene(1)
+    s1 = Scene(ctx, sc1)
+    s1.render_passes(100, 1)
+    s1.reset()
+    n1 = sc1["width"] * sc1["height"] * sc1["spp"]
+    secs = timed(lambda: s1.render_passes(0, sc1["spp"]))
+    out["cfg1"] = {"workload": "sphere mean, single_realization, 128x128, 16 spp",
+                   "paths": n1, "s": secs, "Mpaths_s": n1 / secs / 1e6}
+    s1.close()
+    print(f"bench: cfg1 {out['cfg1']}", file=sys.stderr, flush=True)
+
+    sc3 = scenes.config_scene(3)
+    t0 = time.perf_counter()
+    s3 = Scene(ctx, sc3)
+    upload = time.perf_counter() - t0
+    s3.render_passes(1000, 1)
+    s3.reset()
+    n3 = sc3["width"] * sc3["height"] * cfg3_passes
+    secs = timed(lambda: s3.render_passes(0, cfg3_passes))
+    st = s3.stats()
+    out["cfg3"] = {"workload": "displaced icosphere mean (327,680 triangles), renewal_plus, "
+                               f"512x512, passes 0..{cfg3_passes - 1} of 64",
+                   "paths": n3, "s": secs, "Mpaths_s": n3 / secs / 1e6,
+                   "upload_s": upload, "march_kernel_ms": st["march_kernel_ms"],
+                   "field_evals": st["field_evals"]}
+    s3.close()
+    print(f"bench: cfg3 {out['cfg3']}", file=sys.stderr, flush=True)
+    out["bvh_traversal"] = traversal_bench(ctx, sc3)
+    print(f"bench: bvh {out['bvh_traversal']}", file=sys.stderr, flush=True)
+
+    scs = scenes.config_batch()
+    kc = [max(1, sc["spp"] // cfg4_div) for sc in scs]
+    n4 = sum(sc["width"] * sc["height"] * k for sc, k in zip(scs, kc))
+    t0 = time.perf_counter()
+    seq = [Scene(ctx, sc) for sc in scs]
+    upload = time.perf_counter() - t0
+    for x in seq:
+        x.render_passes(100, 1)
+        x.reset()
+    secs_seq = timed(lambda: [x.render_passes(0, k) for x, k in zip(seq, kc)])
+    ref = [x.image() for x in seq]
+    for x in seq:
+        x.reset()
+    b = Batch(ctx, seq)
+    secs_b = timed(lambda: b.render([0] * len(scs), kc))
+    same = all(np.array_equal(r, x.image()) for r, x in zip(ref, seq))
+    bst = b.stats()
+    tun = b.tuning()
+    b.close()
+    for x in seq:
+        x.close()
+    out["cfg4"] = {"workload": f"64 generated scenes (sphere or icosphere means, kr logU[0.05,5], "
+                               f"ppc 1..8, 64..512 px), the first spp/{cfg4_div} passes of each",
+                   "paths": n4, "upload_s": upload,
+                   "sequential": {"s": secs_seq, "Mpaths_s": n4 / secs_seq / 1e6},
+                   "batched": {"s": secs_b, "Mpaths_s": n4 / secs_b / 1e6,
+                               "launches": bst["launches"], "tune_ms": bst["tune_ms"],
+                               "paths_per_launch": tun["paths_per_launch"],
+                               "march_blocks_per_cu": tun["march_blocks_per_cu"],
+                               "use_graph": tun["use_graph"]},
+                   "batched_equals_sequential": bool(same)}
+    print(f"bench: cfg4 {out['cfg4']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +337,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the configuration 1 / 3 / 4 and BVH traversal measurements")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy a tensor of this many MiB before the timed region, so that a "
                          "profiler's FETCH_SIZE / WRITE_SIZE can be scaled by a known traffic")
@@ -266,6 +403,10 @@ def main() -> int:
         guiding = guiding_bench(ctx)
         print(f"bench: guiding {json.dumps(guiding)}", file=sys.stderr, flush=True)
 
+    configs = None
+    if not args.no_configs:
+        configs = configs_bench(ctx)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene)
@@ -299,6 +440,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "guiding": guiding,
+        "configs": configs,
     }
     print(json.dumps(line), flush=True)
     s.close()

@@ -249,11 +249,32 @@ mupsi_status mupsi_mesh_occluded(mupsi_scene* scene, int32_t prim,
                                  const float* dx, const float* dy, const float* dz,
                                  const float* near_t, const float* far_t, int64_t n,
                                  uint8_t* occluded);
+/* mupsi_mesh_closest on the GP mean mesh (prim -1) starts the search from the
+ * mesh's distance-grid bound; distance, triangle and point are those of the
+ * unbounded search. */
 mupsi_status mupsi_mesh_closest(mupsi_scene* scene, int32_t prim, const float* px,
                                 const float* py, const float* pz, int64_t n,
                                 float* dist, int32_t* tri, float* closest3);
 mupsi_status mupsi_mesh_inside(mupsi_scene* scene, int32_t prim, const float* px,
                                const float* py, const float* pz, int64_t n, uint8_t* inside);
+
+/* Traversal cost of the mesh-mean queries at n points: per point, the BVH
+ * nodes whose bounds were tested and the triangles tested by
+ * Mesh::distToPoint -> BVH::closest (kind 0; src/primitives/mesh.cpp:325-339,
+ * src/bvh/bvh.cpp:302-353, where both children of an expanded node are
+ * tested), Mesh::inside -> BVH::intersect (kind 1; mesh.cpp:341-362,
+ * bvh.cpp:197-250, every re-cast included), or distToPoint started from the
+ * distance-grid bound as mupsi_mesh_closest runs it on a mean mesh (kind 2).  The reference keeps no such
+ * counters; they give the traversal's algorithmic bytes (32 per node, 52 per
+ * triangle: index, face and three vertices). */
+mupsi_status mupsi_mesh_query_cost(mupsi_scene* scene, int32_t prim, int32_t kind,
+                                   const float* px, const float* py, const float* pz,
+                                   int64_t n, uint32_t* nodes, uint32_t* tris);
+
+/* Device time in milliseconds (HIP events on the context's stream) of the
+ * kernel run by the last mupsi_mesh_* call on this context.  No reference
+ * counterpart: it times the mesh-query kernels for the traversal roofline. */
+mupsi_status mupsi_last_kernel_ms(mupsi_ctx* ctx, float* ms);
 
 /* Host-side BVH build with the reference topology (post-order node array, the
  * root is the last node).  nodes_out: 8 floats/ints per node packed as

@@ -128,6 +128,8 @@ SIGNATURES = {
     "mupsi_mesh_occluded": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP, I64, VP]),
     "mupsi_mesh_closest": (I32, [VP, I32, VP, VP, VP, I64, VP, VP, VP]),
     "mupsi_mesh_inside": (I32, [VP, I32, VP, VP, VP, I64, VP]),
+    "mupsi_mesh_query_cost": (I32, [VP, I32, I32, VP, VP, VP, I64, VP, VP]),
+    "mupsi_last_kernel_ms": (I32, [VP, C.POINTER(C.c_float)]),
     "mupsi_bvh_build": (I32, [VP, I32, VP, I32, VP, C.POINTER(I32), VP, C.POINTER(C.c_uint32)]),
     "mupsi_batch_create": (I32, [VP, VP, I32, C.POINTER(VP)]),
     "mupsi_batch_free": (None, [VP]),
@@ -158,6 +160,13 @@ def load(path: Path | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path else LIB_PATH
+    try:
+        # torch ships its own libamdhip64: loading torch first makes the library
+        # bind to that same HIP runtime (loaded the other way round, two runtimes
+        # end up in the process and torch finds no device)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise RuntimeError(
             f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")

@@ -18,9 +18,13 @@ def _torch():
 
 
 def _dev(t):
+    """Device pointer of a contiguous device tensor.  The library runs on its own
+    stream, so the work torch has queued on the tensor's stream (the copy that
+    made it contiguous, a fill) is waited for first."""
     torch = _torch()
     if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
         raise ValueError("expected a contiguous device tensor")
+    torch.cuda.current_stream(t.device).synchronize()
     return C.c_void_p(t.data_ptr())
 
 
@@ -45,7 +49,7 @@ def _cond(cond):
             cond["u"].float().contiguous(), g[:, 0].contiguous(), g[:, 1].contiguous(),
             g[:, 2].contiguous()]
     keep += cols
-    s = A.CondSoA(*[C.c_void_p(x.data_ptr()) for x in keep])
+    s = A.CondSoA(*[_dev(x) for x in keep])
     return s, keep
 
 
@@ -156,6 +160,27 @@ def mesh_inside(scene, prim: int, p):
     check(scene.lib.mupsi_mesh_inside(scene.handle, prim, _dev(px), _dev(py), _dev(pz), n, _dev(out)),
           "mupsi_mesh_inside")
     return out
+
+
+def mesh_query_cost(scene, prim: int, kind: int, p):
+    """Per point: BVH nodes tested and triangles tested by distToPoint (kind 0)
+    inside (kind 1) or distToPoint from the distance-grid bound (kind 2);
+    int64 tensors (n,)."""
+    torch = _torch()
+    n = p.shape[0]
+    px, py, pz = (p[:, i].contiguous() for i in range(3))
+    nodes = torch.empty(n, dtype=torch.int32, device=p.device)
+    tris = torch.empty(n, dtype=torch.int32, device=p.device)
+    check(scene.lib.mupsi_mesh_query_cost(scene.handle, prim, kind, _dev(px), _dev(py), _dev(pz), n,
+                                          _dev(nodes), _dev(tris)), "mupsi_mesh_query_cost")
+    return nodes.long(), tris.long()
+
+
+def last_kernel_ms(ctx) -> float:
+    """Device time of the kernel run by the last mesh query on this context."""
+    ms = C.c_float(0.0)
+    check(ctx.lib.mupsi_last_kernel_ms(ctx.handle, C.byref(ms)), "mupsi_last_kernel_ms")
+    return float(ms.value)
 
 
 # ---------------------------------------------------------------- guiding network

@@ -328,8 +328,23 @@ __device__ __forceinline__ void push_children(const DevMesh& m, const BvhNode& n
   }
 }
 
+// Traversal cost counters: a node is counted when its bounds are tested, a
+// triangle when its intersection or distance test runs.  NoCount compiles to
+// nothing; TravCount is used by the cost entry point only.
+struct NoCount {
+  __device__ __forceinline__ void node() {}
+  __device__ __forceinline__ void tri() {}
+};
+struct TravCount {
+  uint32_t nodes = 0, tris = 0;
+  __device__ __forceinline__ void node() { ++nodes; }
+  __device__ __forceinline__ void tri() { ++tris; }
+};
+
 // BVH::intersect (bvh.cpp:197-250): front-to-back stack traversal from the root.
-__device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, float& v) {
+template <class C = NoCount>
+__device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, float& v,
+                              C&& cnt = C()) {
   if (m.n_nodes == 0) return false;
   bool hit = false;
   uint32_t st[kStack];
@@ -338,10 +353,12 @@ __device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, 
   while (sp > 0) {
     const uint32_t ni = st[--sp];
     const BvhNode nd = load_node(m.nodes, ni);
+    cnt.node();
     if (!aabb_hit(nd, r)) continue;
     if (nd.right & kLeafFlag) {
       const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
       for (uint32_t i = s0; i < s0 + c; ++i) {
+        cnt.tri();
         F3 v0, v1, v2;
         uint32_t id;
         leaf_tri(m, i, v0, v1, v2, id);
@@ -432,19 +449,28 @@ __device__ __forceinline__ float dist2_tri(F3 p, F3 a, F3 b, F3 c, F3& out) {
   return mn;
 }
 
-// BVH::closest (bvh.cpp:302-353): best-first with |.|^2 pruning.
-__device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out) {
+// BVH::closest (bvh.cpp:302-353): best-first with |.|^2 pruning.  md0 (default
+// FLT_MAX, the reference's start) may be any value strictly above the squared
+// distance the search returns: the first triangle in traversal order that attains
+// the minimum is still reached (every node above it is nearer than md0 and than
+// any distance found before it), so ti, out and the result are unchanged while
+// nodes farther than md0 are never visited.
+template <class C = NoCount>
+__device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C(),
+                             float md0 = 3.402823466e+38f) {
   if (m.n_nodes == 0) return 0.0f;
   uint32_t st[kStack];
   int sp = 0;
   st[sp++] = m.root;
-  float md = 3.402823466e+38f;
+  float md = md0;
   while (sp > 0) {
     const BvhNode nd = load_node(m.nodes, st[--sp]);
+    cnt.node();
     if (dist2_aabb(p, nd) >= md) continue;
     if (nd.right & kLeafFlag) {
       const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
       for (uint32_t i = s0; i < s0 + c; ++i) {
+        cnt.tri();
         F3 v0, v1, v2;
         uint32_t id;
         leaf_tri(m, i, v0, v1, v2, id);
@@ -459,6 +485,8 @@ __device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out) {
     } else {
       const float dl = dist2_aabb(p, load_node(m.nodes, nd.left));
       const float dr = dist2_aabb(p, load_node(m.nodes, nd.right));
+      cnt.node();
+      cnt.node();
       if (sp > kStack - 2) continue;
       if (dl < dr) {
         if (dr < md) st[sp++] = nd.right;
@@ -500,27 +528,33 @@ __device__ bool mesh_occluded(const DevMesh& m, const Ray& ray) {
   return bvh_occluded(m, lr);
 }
 // Mesh::distToPoint (mesh.cpp:325-339)
-__device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out) {
+// bound_local: an upper bound of the local-space distance (INFINITY: none).
+template <class C = NoCount>
+__device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C(),
+                           float bound_local = INFINITY) {
   const F3 pl = xf_to_local(m.xf, p);
   F3 ol = mk(0, 0, 0);
   ti = 0;
-  bvh_closest(m, pl, ti, ol);
+  // widened so that it stays strictly above the rounded squared distance
+  const float b = fmaf(bound_local, 1.0001f, 1e-5f);
+  bvh_closest(m, pl, ti, ol, cnt, fminf(b * b, 3.402823466e+38f));
   out = xf_to_world(m.xf, ol);
   return norm(sub(out, p));
 }
 // Mesh::inside (mesh.cpp:341-362): parity of +x crossings, restarting 1e-4 past each hit.
-__device__ bool mesh_inside(const DevMesh& m, F3 p) {
+template <class C = NoCount>
+__device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
   const F3 pl = xf_to_local(m.xf, p);
   Ray r = make_ray(pl, mk(1, 0, 0));
   int hits = 0;
   uint32_t ti;
   float u, v;
-  if (bvh_intersect(m, r, ti, u, v)) {
+  if (bvh_intersect(m, r, ti, u, v, cnt)) {
     ++hits;
     while (r.farT < INFINITY) {
       r.nearT = r.farT + 1e-4f;
       r.farT = INFINITY;
-      if (!bvh_intersect(m, r, ti, u, v)) break;
+      if (!bvh_intersect(m, r, ti, u, v, cnt)) break;
       ++hits;
     }
   }
@@ -554,6 +588,15 @@ __device__ __forceinline__ bool mesh_in_parity_slab(const DevMesh& m, F3 pl) {
          m.lbmax[0] - pl.x >= 0.0f;
 }
 
+// Mesh::distToPoint with the closest-point search started from the distance
+// grid's upper bound at p (same ti, closest point and distance; fewer nodes).
+template <class C = NoCount>
+__device__ float mesh_dist_grid(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C()) {
+  float lo, hi;
+  mesh_dist_bounds_local(m, xf_to_local(m.xf, p), lo, hi);
+  return mesh_dist(m, p, ti, out, cnt, hi);
+}
+
 __device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {
   if (m.xf.identity) return n;
   return normalized(mul(n, m.xf.inv_s));
@@ -573,7 +616,7 @@ __device__ __forceinline__ float mean_eval(const DevScene& S, F3 p, F3* grad) {
   const float sg = mesh_inside(S.mean_mesh, p) ? -1.0f : 1.0f;
   uint32_t ti = 0;
   F3 o;
-  const float d = mesh_dist(S.mean_mesh, p, ti, o);
+  const float d = mesh_dist_grid(S.mean_mesh, p, ti, o);
   if (kGrad) {
     if (d < 1e-6f) {
       const float* fnp = S.mean_mesh.fnormals + 3 * ti;

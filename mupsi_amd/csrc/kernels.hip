@@ -529,8 +529,11 @@ __device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, 
 // Returns +1 / -1 when the sign of f(p) is certain, 0 otherwise.  *final: no later
 // grid point of the ray can have another certain-or-uncertain sign (the march can
 // stop once this sign equals the origin's).
+// kMesh false: the scene's mean is known not to be a mesh (no BVH code).
+// ins: Mesh::inside(p) when the caller knows it (0 / 1), else -1.
+template <bool kMesh = true>
 __device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo& k, F3 p, F3 d,
-                                              bool* final) {
+                                              bool* final, int ins = -1) {
   *final = false;
   if (k.kind == 1) {
     const F3 r = sub(p, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
@@ -541,7 +544,7 @@ __device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo&
     }
     return q2 < k.inner2 ? -1 : 0;
   }
-  if (k.kind == 2) {
+  if (kMesh && k.kind == 2) {
     const DevMesh& M = S.mean_mesh;
     const F3 pl = xf_to_local(M.xf, p);
     const float dx = fmaxf(fmaxf(M.lbmin[0] - pl.x, 0.0f), pl.x - M.lbmax[0]);
@@ -550,7 +553,12 @@ __device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo&
     const float dw = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) * M.xf.s;
     if (!(dw > k.Bm)) return 0;
     // the root-box test of the +x parity ray (AABB::intersect with d = (1, 0, 0))
-    if (mesh_in_parity_slab(M, pl)) return mesh_inside(M, p) ? -1 : 1;
+    if (ins >= 0) {
+      if (ins) return -1;
+      if (mesh_in_parity_slab(M, pl)) return 1;
+    } else if (mesh_in_parity_slab(M, pl)) {
+      return mesh_inside(M, p) ? -1 : 1;
+    }
     const F3 rc = sub(p, k.cb);
     const float e = 1e-3f * (1.0f + fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
     const bool leaves = (p.x > k.wmax[0] + e && d.x >= 0.0f) || (p.y > k.wmax[1] + e && d.y >= 0.0f) ||
@@ -756,8 +764,9 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
 // returned value carries the sign with a magnitude no larger than |f|.  Every
 // decision the marcher takes from it (sign, and |f_prev - f| > 1e-6 across a
 // sign change) is the one the exact value gives.
+template <bool kMesh = true>
 __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G, F3 q, float acc,
-                                              const Cond& c) {
+                                              const Cond& c, int ins = -1) {
   const float psi_t = G.sigma * acc / G.sqrt_var;
   float cond_t = 0.0f;
   if (c.active) {
@@ -765,21 +774,25 @@ __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G,
     const float h = kern_h(G, c.C, q);
     cond_t = fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
   }
-  if (S.mean_type == MUPSI_MEAN_MESH && S.mean_mesh.xf.s > 0.0f) {
+  if (kMesh && S.mean_type == MUPSI_MEAN_MESH && S.mean_mesh.xf.s > 0.0f) {
     const DevMesh& M = S.mean_mesh;
     const F3 pl = xf_to_local(M.xf, q);
     float lo, hi;
     mesh_dist_bounds_local(M, pl, lo, hi);
+    const float hi_local = hi;
     lo = lo * M.xf.s * (1.0f - 1e-5f);
     hi = hi * M.xf.s * (1.0f + 1e-5f);
     const float e = psi_t + cond_t;
     const float margin = 1e-3f + 1e-5f * (fabsf(psi_t) + fabsf(cond_t) + lo);
+    const auto inside = [&]() {
+      return ins >= 0 ? ins != 0 : (mesh_in_parity_slab(M, pl) && mesh_inside(M, q));
+    };
     if (lo > fabsf(e) + margin) {  // |mu| alone decides: no sign query beyond the parity ray
-      const float sg = (mesh_in_parity_slab(M, pl) && mesh_inside(M, q)) ? -1.0f : 1.0f;
+      const float sg = inside() ? -1.0f : 1.0f;
       return sg * (lo - fabsf(e));
     }
     if (hi < INFINITY) {
-      const float sg = (mesh_in_parity_slab(M, pl) && mesh_inside(M, q)) ? -1.0f : 1.0f;
+      const float sg = inside() ? -1.0f : 1.0f;
       const float a = sg * lo + e, b = sg * hi + e;   // f lies between a and b
       const float fmin = fminf(a, b), fmax = fmaxf(a, b);
       if (fmin > margin) return fmin;
@@ -787,7 +800,7 @@ __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G,
       // within the interval a zero crossing is possible: evaluate mu exactly
       uint32_t ti = 0;
       F3 o;
-      float f = sg * mesh_dist(M, q, ti, o) + psi_t;  // mean_eval's mu = sg * d
+      float f = sg * mesh_dist(M, q, ti, o, NoCount(), hi_local) + psi_t;  // mu = sg * d
       if (c.active) f = f + cond_t;
       return f;
     }
@@ -803,200 +816,6 @@ struct CachedConsts {
   float psi_k;    // -log2(e) / L^2: h = exp2(|p - s|^2 * psi_k)
 };
 
-template <int PPC>
-__global__ __launch_bounds__(64) void k_march_cached(DevScene S, MarchArgs a, CachedConsts cc) {
-  __shared__ float4 win[2][27 * PPC];
-  const int n = a.count_dev ? *a.count_dev : a.count_host;
-  const int lane = threadIdx.x;
-  const int grp = lane >> 5;
-  const int gl = lane & (kGroup - 1);
-  float4* W = win[grp];
-  const DevGP& G = S.gp;
-  constexpr int ppc = PPC;
-  constexpr int nslots = 27 * PPC;
-  // ray state: identical in the 32 lanes of a group
-  int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
-  int rid = 0;
-  F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-  float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f;
-  SkipInfo sk;
-  sk.kind = 0;
-  bool sp = false;
-  uint32_t seed = 0;
-  Cond c;
-  c.active = false;
-  int nsteps = 0;
-  int wx = 0, wy = 0, wz = 0;
-  bool wvalid = false;
-  unsigned long long evals = 0, refsteps = 0, rays = 0;
-
-  auto finish = [&](bool hit, F3 p) {
-    if (gl == 0) {
-      a.hit[rid] = hit ? 1 : 0;
-      if (hit) {
-        a.t[rid] = t;
-        a.px[rid] = p.x;
-        a.py[rid] = p.y;
-        a.pz[rid] = p.z;
-      }
-      if (a.steps) a.steps[rid] = hit ? nsteps : -1;
-      refsteps += hit ? (unsigned long long)nsteps
-                      : nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
-    }
-    phase = 0;
-  };
-
-  while (true) {
-    bool do_eval = false;
-    F3 q = mk(0, 0, 0);
-    for (int round = 0; round < 2; ++round) {
-      const bool need = phase == 0;
-      const uint64_t m = __ballot(need && gl == 0);
-      if (m) {
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(a.work, __popcll(m));
-        base = __shfl(base, leader);
-        if (need) {
-          const int qi = base + __popcll(m & ((1ull << (grp * kGroup)) - 1ull));
-          if (qi < n) {
-            rid = a.queue ? a.queue[qi] : qi;
-            o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
-            d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
-            nt = a.nt[rid];
-            ft = fminf(a.ft ? a.ft[rid] : 3.402823466e+38f, 2000.0f);
-            dt = fminf(0.5f, (ft - nt) / 100);
-            seed = a.seed[rid];
-            c.active = a.flags ? (a.flags[rid] & kFlagCond) != 0 : (a.active && a.active[rid]);
-            if (c.active) {
-              c.C = mk(a.cx[rid], a.cy[rid], a.cz[rid]);
-              c.u = a.cu[rid];
-              c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
-            }
-            sk = make_skip(S, c, o, ft);
-            nsteps = 0;
-            wvalid = false;  // new realization seed
-            if (gl == 0) ++rays;
-            phase = 1;
-          } else {
-            phase = 3;
-          }
-        }
-      }
-      if (phase == 1) {
-        q = o;
-        do_eval = true;
-      } else if (phase == 2 && !do_eval) {
-        while (true) {
-          if (!(t < ft)) {
-            finish(false, q);
-            break;
-          }
-          const F3 p = madd(d, t, o);
-          bool fin;
-          const int cs = classify_point(S, sk, p, d, &fin);
-          if (cs == 0) {
-            q = p;
-            do_eval = true;
-            break;
-          }
-          ++nsteps;
-          if ((cs > 0) != sp) {
-            finish(true, p);
-            break;
-          }
-          if (fin || !(t + dt > t)) {  // no later sign change / the grid cannot advance
-            finish(false, p);
-            break;
-          }
-          t += dt;
-        }
-      }
-      if (__ballot(phase == 0) == 0) break;
-    }
-    if (__ballot(phase != 3) == 0) break;
-
-    // Window: cells entering the 27-cell neighbourhood of the evaluation point.
-    int newcells = -1;
-    int cx = 0, cy = 0, cz = 0;
-    if (do_eval) {
-      cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
-      cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
-      cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
-      if (!wvalid) {
-        newcells = 27;
-      } else {
-        const int ex = cx - wx, ey = cy - wy, ez = cz - wz;
-        if (ex | ey | ez) {
-          newcells = (abs(ex) <= 1 && abs(ey) <= 1 && abs(ez) <= 1) ? (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)
-                                                                   : 27;
-        }
-      }
-    }
-    __syncthreads();  // the previous evaluation's reads of the window are complete
-    if (newcells >= 0) {
-      const int items = kNewCells.n[newcells] * ppc;
-      const int mx = mod3(cx), my = mod3(cy), mz = mod3(cz);  // residues of the window centre
-      for (int it = gl; it < items; it += kGroup) {
-        const int k = it / ppc;
-        const int i = it - k * ppc;
-        const int ni = kNewCells.idx[newcells][k];
-        const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
-        int rx = mx + ox, ry = my + oy, rz = mz + oz;
-        rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
-        ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
-        rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
-        W[(rx * 9 + ry * 3 + rz) * ppc + i] = make_impulse(G, cx + ox, cy + oy, cz + oz, seed, i);
-      }
-      wx = cx;
-      wy = cy;
-      wz = cz;
-      wvalid = true;
-    }
-    __syncthreads();
-
-    float acc = 0.0f;
-    if (do_eval) {
-#pragma unroll
-      for (int sl = gl; sl < nslots; sl += kGroup) {
-        const float4 im = W[sl];
-        const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
-        const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-        acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * cc.psi_k), acc);
-      }
-    }
-    for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-
-    if (do_eval) {
-      const float f = cached_field(S, G, q, acc, c);
-      if (gl == 0) ++evals;
-      if (phase == 1) {
-        fprev = f;
-        sp = f > 0.0f;
-        t = nt;
-        phase = 2;
-      } else {
-        ++nsteps;
-        if ((f > 0.0f) != sp && fabsf(fprev - f) > 1e-6f) {
-          finish(true, q);
-        } else if (!(t + dt > t)) {
-          finish(false, q);
-        } else {
-          t += dt;
-        }
-      }
-    }
-  }
-  evals = wave_sum(evals);
-  refsteps = wave_sum(refsteps);
-  rays = wave_sum(rays);
-  if (lane == 0 && a.stats) {
-    atomicAdd(&a.stats[0], evals);
-    atomicAdd(&a.stats[1], refsteps);
-    atomicAdd(&a.stats[2], rays);
-  }
-}
-
 // This lane's part of the window sum: slots gl, gl + 32, ... of the 27 x PPC.
 template <int PPC>
 __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float psi_k) {
@@ -1011,19 +830,36 @@ __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float
   return acc;
 }
 
-// Batched launch: the same marcher for rays of every scene with
-// points_per_cell <= 8, each 32-lane group taking its ray's scene constants.
-__global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedConsts* ccs) {
-  __shared__ float4 win[2][27 * kMaxPpcCached];
+// The window marcher: GPMedium::sampleDistance / transmittance
+// (gpmedium.cpp:46-129) with one 32-lane group per ray (two rays per wave).  The
+// 27 x ppc impulses of the 3x3x3 cell window around the evaluation point live in
+// LDS as a toroidal window indexed by cell mod 3; when the march point enters a
+// new cell only the 9/3/1 newly entered cells are regenerated (one PCG jump-ahead
+// per impulse, lanes in parallel).  Each lane sums its slots; a butterfly
+// finishes the sum.
+//   PPC      points per cell (kMulti: the LDS capacity, 8; each ray uses its own)
+//   kMesh    mean meshes possible: inside() at the next 32 grid points of a ray
+//            is evaluated at once, one point per lane, and kept as a bit mask
+//   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
+template <int PPC, bool kMesh, bool kMulti>
+__global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
+                                                     CachedConsts cc0,
+                                                     const CachedConsts* ccs) {
+  __shared__ float4 win[2][27 * PPC];
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
   const int grp = lane >> 5;
   const int gl = lane & (kGroup - 1);
   float4* W = win[grp];
-  // the group's ray and its scene
-  DevGP G{};
-  int ppc = 1, nslots = 27, sidx = 0;
-  CachedConsts cc{};
+  // the group's scene (batched launches: that of its current ray)
+  int sidx = 0;
+  DevGP G = S0.gp;
+  CachedConsts cc = cc0;
+  int ppc = PPC;
+  const auto scene = [&]() -> const DevScene& {
+    if constexpr (kMulti) return a.scenes[sidx];
+    else return S0;
+  };
   // ray state: identical in the 32 lanes of a group
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
   int rid = 0;
@@ -1039,6 +875,21 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   unsigned long long evals = 0, refsteps = 0, rays = 0;
+  // mesh means: Mesh::inside at grid points gi0 .. gi0 + 31 of the current ray
+  int gi = 0, gi0 = -1;
+  uint32_t imask = 0;
+  const auto inside_here = [&]() -> int {
+    if (gi0 < 0 || gi - gi0 >= kGroup) {
+      float tl = t;  // the march's own f32 accumulation of t, gl steps ahead
+      for (int i = 0; i < gl; ++i) tl += dt;
+      const F3 pp = madd(d, tl, o);
+      const DevMesh& M = scene().mean_mesh;
+      const bool in = mesh_in_parity_slab(M, xf_to_local(M.xf, pp)) && mesh_inside(M, pp);
+      imask = (uint32_t)(__ballot(in) >> (grp * kGroup));
+      gi0 = gi;
+    }
+    return (int)((imask >> (gi - gi0)) & 1u);
+  };
 
   auto finish = [&](bool hit, F3 p) {
     if (gl == 0) {
@@ -1059,6 +910,7 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
   while (true) {
     bool do_eval = false;
     F3 q = mk(0, 0, 0);
+    int qins = -1;  // Mesh::inside(q) if known
     for (int round = 0; round < 2; ++round) {
       const bool need = phase == 0;
       const uint64_t m = __ballot(need && gl == 0);
@@ -1071,11 +923,12 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
           const int qi = base + __popcll(m & ((1ull << (grp * kGroup)) - 1ull));
           if (qi < n) {
             rid = a.queue ? a.queue[qi] : qi;
-            sidx = a.sid[rid];
-            G = a.scenes[sidx].gp;
-            ppc = G.ppc;
-            nslots = 27 * ppc;
-            cc = ccs[sidx];
+            if constexpr (kMulti) {
+              sidx = a.sid[rid];
+              G = a.scenes[sidx].gp;
+              ppc = G.ppc;
+              cc = ccs[sidx];
+            }
             o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
             d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
             nt = a.nt[rid];
@@ -1088,9 +941,10 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sk = make_skip(a.scenes[sidx], c, o, ft);
+            sk = make_skip(scene(), c, o, ft);
             nsteps = 0;
             wvalid = false;  // new realization seed
+            gi0 = -1;
             if (gl == 0) ++rays;
             phase = 1;
           } else {
@@ -1109,9 +963,12 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
           }
           const F3 p = madd(d, t, o);
           bool fin;
-          const int cs = classify_point(a.scenes[sidx], sk, p, d, &fin);
+          int ins = -1;
+          if (kMesh && sk.kind == 2) ins = inside_here();
+          const int cs = classify_point<kMesh>(scene(), sk, p, d, &fin, ins);
           if (cs == 0) {
             q = p;
+            qins = ins;
             do_eval = true;
             break;
           }
@@ -1125,6 +982,7 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
             break;
           }
           t += dt;
+          ++gi;
         }
       }
       if (__ballot(phase == 0) == 0) break;
@@ -1172,26 +1030,32 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
 
     float acc = 0.0f;
     if (do_eval) {
-      switch (ppc) {  // the compile-time slot count of k_march_cached, per ray
-        case 1: acc = window_sum<1>(W, q, gl, cc.psi_k); break;
-        case 2: acc = window_sum<2>(W, q, gl, cc.psi_k); break;
-        case 3: acc = window_sum<3>(W, q, gl, cc.psi_k); break;
-        case 4: acc = window_sum<4>(W, q, gl, cc.psi_k); break;
-        case 5: acc = window_sum<5>(W, q, gl, cc.psi_k); break;
-        case 6: acc = window_sum<6>(W, q, gl, cc.psi_k); break;
-        case 7: acc = window_sum<7>(W, q, gl, cc.psi_k); break;
-        default: acc = window_sum<8>(W, q, gl, cc.psi_k); break;
+      if constexpr (kMulti) {
+        switch (ppc) {  // the compile-time slot count, per ray
+          case 1: acc = window_sum<1>(W, q, gl, cc.psi_k); break;
+          case 2: acc = window_sum<2>(W, q, gl, cc.psi_k); break;
+          case 3: acc = window_sum<3>(W, q, gl, cc.psi_k); break;
+          case 4: acc = window_sum<4>(W, q, gl, cc.psi_k); break;
+          case 5: acc = window_sum<5>(W, q, gl, cc.psi_k); break;
+          case 6: acc = window_sum<6>(W, q, gl, cc.psi_k); break;
+          case 7: acc = window_sum<7>(W, q, gl, cc.psi_k); break;
+          default: acc = window_sum<8>(W, q, gl, cc.psi_k); break;
+        }
+      } else {
+        acc = window_sum<PPC>(W, q, gl, cc.psi_k);
       }
     }
     for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
 
     if (do_eval) {
-      const float f = cached_field(a.scenes[sidx], G, q, acc, c);
+      const float f = cached_field<kMesh>(scene(), G, q, acc, c, qins);
       if (gl == 0) ++evals;
       if (phase == 1) {
         fprev = f;
         sp = f > 0.0f;
         t = nt;
+        gi = 0;
+        gi0 = -1;
         phase = 2;
       } else {
         ++nsteps;
@@ -1201,6 +1065,7 @@ __global__ __launch_bounds__(64) void k_march_multi(MarchArgs a, const CachedCon
           finish(false, q);
         } else {
           t += dt;
+          ++gi;
         }
       }
     }
@@ -1348,8 +1213,10 @@ struct ShadeArgs {
   int* qS;
   int* nS;
   const DevScene* scenes;  // batched launch: per-scene constants; medium paths of
-  int* qMm2;               // mesh-mean scenes (or ppc > 8) go to queue Mm
+  int* qMm2;               // scenes with ppc > 8 go to queue Mm (lane-per-ray marcher)
   int* nMm2;
+  int* qMs2;               // and those of mesh-mean scenes with ppc <= 8 to queue Ms
+  int* nMs2;
 };
 
 // The guided tracer keeps T as the plain tracer's throughput (albedo weights and
@@ -1627,8 +1494,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   wave_push(cont, pid, a.qA2, a.nA2);
   if constexpr (kMulti) {
     const bool cached = S.gp.ppc <= kMaxPpcCached;
-    wave_push(cont && medium && !S.skip_medium && cached, pid, a.qM2, a.nM2);
-    wave_push(cont && medium && !S.skip_medium && !cached, pid, a.qMm2, a.nMm2);
+    const bool mesh = S.mean_type == MUPSI_MEAN_MESH;
+    const bool go = cont && medium && !S.skip_medium;
+    wave_push(go && cached && !mesh, pid, a.qM2, a.nM2);
+    wave_push(go && cached && mesh, pid, a.qMs2, a.nMs2);
+    wave_push(go && !cached, pid, a.qMm2, a.nMm2);
   } else {
     wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
   }
@@ -2012,7 +1882,9 @@ __global__ __launch_bounds__(256) void k_mesh_closest(DevMesh m, const float* px
   if (i >= n) return;
   uint32_t ti = 0;
   F3 o = mk(0, 0, 0);
-  dist[i] = mesh_dist(m, mk(px[i], py[i], pz[i]), ti, o);
+  // a mean mesh carries a distance grid: its bound only prunes (same results)
+  dist[i] = m.dlb ? mesh_dist_grid(m, mk(px[i], py[i], pz[i]), ti, o)
+                  : mesh_dist(m, mk(px[i], py[i], pz[i]), ti, o);
   tri[i] = (int32_t)ti;
   cl[3 * i] = o.x;
   cl[3 * i + 1] = o.y;
@@ -2023,6 +1895,26 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
+}
+// Traversal cost per point of Mesh::distToPoint (kind 0), Mesh::inside (kind 1), or
+// Mesh::distToPoint searched from the distance grid's bound (kind 2).
+__global__ __launch_bounds__(256) void k_mesh_cost(DevMesh m, int kind, const float* px,
+                                                   const float* py, const float* pz, int64_t n,
+                                                   uint32_t* nodes, uint32_t* tris) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TravCount c;
+  const F3 p = mk(px[i], py[i], pz[i]);
+  if (kind == 0 || kind == 2) {
+    uint32_t ti = 0;
+    F3 o = mk(0, 0, 0);
+    if (kind == 0) (void)mesh_dist(m, p, ti, o, c);
+    else (void)mesh_dist_grid(m, p, ti, o, c);
+  } else {
+    (void)mesh_inside(m, p, c);
+  }
+  nodes[i] = c.nodes;
+  tris[i] = c.tris;
 }
 
 // ====================================================================== host: context
@@ -2059,10 +1951,12 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   ctx->name = prop.name;
   ctx->cus = prop.multiProcessorCount;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&ctx->ev_begin));
+  HIP_TRY(hipEventCreate(&ctx->ev_end));
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_cached<3>, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   *out = ctx.release();
   return MUPSI_OK;
@@ -2085,6 +1979,8 @@ extern "C" void mupsi_close(mupsi_ctx* ctx) {
   std::vector<mupsi_guide*> guides;
   guides.swap(ctx->guides);
   for (mupsi_guide* g : guides) destroy_guide(g);
+  if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
+  if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -2114,6 +2010,7 @@ struct PathBuffers {
   ShadowSoA SH{};
   int* qS = nullptr;
   int* qMm[2]{};  // batched launches: medium paths for the lane-per-ray marcher
+  int* qMs[2]{};  // batched launches: window-marcher paths of mesh-mean scenes
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -2323,7 +2220,10 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   for (int i = 0; i < 2; ++i) {
     HIP_TRY(m.alloc(n, &pb->qA[i]));
     HIP_TRY(m.alloc(n, &pb->qM[i]));
-    if (mesh_queue) HIP_TRY(m.alloc(n, &pb->qMm[i]));
+    if (mesh_queue) {
+      HIP_TRY(m.alloc(n, &pb->qMm[i]));
+      HIP_TRY(m.alloc(n, &pb->qMs[i]));
+    }
   }
   HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
@@ -2408,16 +2308,27 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
   if (S.gp.ppc <= kMaxPpcCached) {
     const CachedConsts cc = cached_consts(S.gp);
     const dim3 grid(ctx->march_cached_blocks), block(64);
+    const bool mesh = S.mean_type == MUPSI_MEAN_MESH;
+#define MUPSI_CACHED(P)                                                                     \
+  case P:                                                                                   \
+    if (mesh)                                                                               \
+      hipLaunchKernelGGL((k_march_window<P, true, false>), grid, block, 0, st, S, ma, cc,     \
+                         nullptr);                                                          \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_march_window<P, false, false>), grid, block, 0, st, S, ma, cc,    \
+                         nullptr);                                                          \
+    break;
     switch (S.gp.ppc) {
-      case 1: hipLaunchKernelGGL(k_march_cached<1>, grid, block, 0, st, S, ma, cc); break;
-      case 2: hipLaunchKernelGGL(k_march_cached<2>, grid, block, 0, st, S, ma, cc); break;
-      case 3: hipLaunchKernelGGL(k_march_cached<3>, grid, block, 0, st, S, ma, cc); break;
-      case 4: hipLaunchKernelGGL(k_march_cached<4>, grid, block, 0, st, S, ma, cc); break;
-      case 5: hipLaunchKernelGGL(k_march_cached<5>, grid, block, 0, st, S, ma, cc); break;
-      case 6: hipLaunchKernelGGL(k_march_cached<6>, grid, block, 0, st, S, ma, cc); break;
-      case 7: hipLaunchKernelGGL(k_march_cached<7>, grid, block, 0, st, S, ma, cc); break;
-      default: hipLaunchKernelGGL(k_march_cached<8>, grid, block, 0, st, S, ma, cc); break;
+      MUPSI_CACHED(1)
+      MUPSI_CACHED(2)
+      MUPSI_CACHED(3)
+      MUPSI_CACHED(4)
+      MUPSI_CACHED(5)
+      MUPSI_CACHED(6)
+      MUPSI_CACHED(7)
+      default: MUPSI_CACHED(8)
     }
+#undef MUPSI_CACHED
   } else {
     hipLaunchKernelGGL(k_march<false>, dim3(ctx->march_blocks), dim3(256), 0, st, S, ma);
   }
@@ -2509,7 +2420,7 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
     int* nS = pb.counters + 7;
     if (defer) hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nS, 1, 0, -1);
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
-                 defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr};
+                 defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
     if (defer) {
@@ -2766,6 +2677,7 @@ struct mupsi_batch {
   std::unique_ptr<PathBuffers> pb;
   int max_bounce = 1;
   bool any_cached = false, any_plain = false;
+  bool any_cached_mesh = false;  // window-marcher scenes with a mean mesh
   mupsi_batch_tuning tuning{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2804,6 +2716,8 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
     int* nMm = pb.counters + 8 + cur;
     int* work = pb.counters + 4;
     int* work2 = pb.counters + 10;
+    int* nMs = pb.counters + 12 + cur;
+    int* work3 = pb.counters + 14;
     if (it > 0) {
       MarchArgs ma{};
       ma.ox = pb.P.ox; ma.oy = pb.P.oy; ma.oz = pb.P.oz;
@@ -2819,12 +2733,22 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
       ma.stats = pb.stats;
       ma.scenes = b->d_scenes;
       ma.sid = pb.P.sid;
+      const dim3 wgrid(per_cu * ctx->cus * 2), wblock(64);
       if (b->any_cached) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work, 1, 0, -1);
         ma.queue = pb.qM[cur];
         ma.count_dev = nM;
         ma.work = work;
-        hipLaunchKernelGGL(k_march_multi, dim3(per_cu * ctx->cus * 2), dim3(64), 0, st, ma, b->d_cc);
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true>), wgrid, wblock, 0, st,
+                           DevScene{}, ma, CachedConsts{}, b->d_cc);
+      }
+      if (b->any_cached_mesh) {
+        hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work3, 1, 0, -1);
+        ma.queue = pb.qMs[cur];
+        ma.count_dev = nMs;
+        ma.work = work3;
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true>), wgrid, wblock, 0, st,
+                           DevScene{}, ma, CachedConsts{}, b->d_cc);
       }
       if (b->any_plain) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work2, 1, 0, -1);
@@ -2836,12 +2760,14 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
     }
     int* nA2 = pb.counters + 2 * nxt;
     int* nMm2 = pb.counters + 8 + nxt;
+    int* nMs2 = pb.counters + 12 + nxt;
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nA2, 2, 0, -1);
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nMm2, 1, 0, -1);
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nMs2, 1, 0, -1);
     GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], pb.counters + 2 * nxt + 1,
                  gs, 0, ShadowSoA{}, nullptr, nullptr, b->d_scenes,
-                 b->any_plain ? pb.qMm[nxt] : nullptr, nMm2};
+                 pb.qMm[nxt], nMm2, pb.qMs[nxt], nMs2};
     hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
   }
   for (size_t i = 0; i < b->scenes.size(); ++i) {
@@ -2862,7 +2788,7 @@ static mupsi_status batch_buffers(mupsi_batch* b, int64_t cap) {
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
-  return alloc_path_buffers(b->pb, cap, false, b->any_plain);
+  return alloc_path_buffers(b->pb, cap, false, b->any_plain || b->any_cached_mesh);
 }
 
 // Capture the launch's kernels into a graph for the current buffers and tuning.
@@ -2969,8 +2895,9 @@ extern "C" mupsi_status mupsi_batch_create(mupsi_ctx* ctx, mupsi_scene* const* s
     cc.push_back(cached_consts(sc->ds.gp));
     b->max_bounce = std::max(b->max_bounce, sc->ds.max_bounce);
     if (sc->ds.has_gp && !sc->ds.skip_medium) {
-      if (scene_uses_cached(sc->ds)) b->any_cached = true;
-      else b->any_plain = true;
+      if (!scene_uses_cached(sc->ds)) b->any_plain = true;
+      else if (sc->ds.mean_type == MUPSI_MEAN_MESH) b->any_cached_mesh = true;
+      else b->any_cached = true;
     }
   }
   const DevScene* dds;
@@ -3028,11 +2955,14 @@ extern "C" mupsi_status mupsi_batch_stats_get(mupsi_batch* b, mupsi_batch_stats*
   return MUPSI_OK;
 }
 
-// Autotuner candidates: paths per launch x persistent marcher blocks per CU.
-// Larger launches amortise the tail of each bounce loop (few live paths, idle
-// CUs), so the caps reach 32M paths (about 5 GB of path state).
-static const int64_t kTuneCaps[] = {1LL << 22, 1LL << 23, 1LL << 24, 1LL << 25};
-static const int kTuneBlocks[] = {4, 8, 16};
+// Autotuning.  Larger launches amortise the tail of each bounce loop (few live
+// paths, idle CUs): measured on the generated 64-scene workload, 16M paths per
+// launch beat 8M, 4M and 2M at every marcher grid.  The launch size is therefore
+// the whole remaining work up to kTuneCap (about 2.5 GB of path state); only the
+// persistent marchers' blocks per CU are probed, on real passes, when the work
+// spans enough launches to pay for the probes.
+static constexpr int64_t kTuneCap = 1LL << 24;
+static const int kTuneBlocks[] = {8, 16, 4};
 
 extern "C" mupsi_status mupsi_batch_render(mupsi_batch* b, const int32_t* k_begin,
                                            const int32_t* k_count,
@@ -3055,42 +2985,41 @@ extern "C" mupsi_status mupsi_batch_render(mupsi_batch* b, const int32_t* k_begi
   HIP_TRY(hipSetDevice(b->ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   size_t rotate = 0;
-  // autotuning: probe the candidates on real passes, keep the fastest per path
+  // autotuning: launch size from the work, blocks per CU probed on real passes
   if (b->tuning.paths_per_launch == 0 || b->tuning.march_blocks_per_cu == 0) {
     const auto tt0 = std::chrono::steady_clock::now();
-    double best = 1e300;
-    mupsi_batch_tuning best_t = b->tuning;
-    for (int64_t cap : kTuneCaps) {
-      if (cap < min_fit) continue;
-      for (int blk : kTuneBlocks) {
-        if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
-        mupsi_batch_tuning t = b->tuning;
-        t.paths_per_launch = std::min<int64_t>(cap, std::max<int64_t>(work, min_fit));
-        t.march_blocks_per_cu = blk;
-        b->tuning = t;
-        mupsi_status st = batch_buffers(b, t.paths_per_launch);
+    if (b->tuning.paths_per_launch == 0)
+      b->tuning.paths_per_launch = std::max(min_fit, std::min(work, kTuneCap));
+    if (b->tuning.march_blocks_per_cu == 0) {
+      const int64_t cap = b->tuning.paths_per_launch;
+      const int nb = (int)(sizeof(kTuneBlocks) / sizeof(kTuneBlocks[0]));
+      b->tuning.march_blocks_per_cu = kTuneBlocks[0];
+      if (work >= (int64_t)(2 * nb + 2) * cap) {
+        mupsi_status st = batch_buffers(b, cap);
         if (st != MUPSI_OK) return st;
-        const int64_t total = batch_plan(b, next_k, left, t.paths_per_launch, rotate++);
-        if (total == 0) break;
-        double ms = 0;
-        // a first launch on a new graph includes its capture: time a second one
-        st = batch_launch(b, total, &ms);
-        if (st != MUPSI_OK) return st;
-        const int64_t total2 = batch_plan(b, next_k, left, t.paths_per_launch, rotate++);
-        if (total2 > 0) {
-          st = batch_launch(b, total2, &ms);
-          if (st != MUPSI_OK) return st;
-          const double per = ms / (double)total2;
-          if (per < best) {
+        double best = 1e300;
+        int best_blk = kTuneBlocks[0];
+        for (int blk : kTuneBlocks) {
+          if (abort_flag && *abort_flag) return fail(MUPSI_ERR_ABORTED, "render aborted");
+          b->tuning.march_blocks_per_cu = blk;
+          double per = 0.0;
+          // a first launch on a new graph includes its capture: time a second one
+          for (int rep = 0; rep < 2; ++rep) {
+            const int64_t total = batch_plan(b, next_k, left, cap, rotate++);
+            if (total == 0) break;
+            double ms = 0;
+            st = batch_launch(b, total, &ms);
+            if (st != MUPSI_OK) return st;
+            per = ms / (double)total;
+          }
+          if (per > 0.0 && per < best) {
             best = per;
-            best_t = t;
+            best_blk = blk;
           }
         }
+        b->tuning.march_blocks_per_cu = best_blk;
       }
     }
-    if (best < 1e300) b->tuning = best_t;
-    else if (b->tuning.paths_per_launch == 0) b->tuning.paths_per_launch = std::max<int64_t>(min_fit, 1 << 24);
-    if (b->tuning.march_blocks_per_cu == 0) b->tuning.march_blocks_per_cu = 8;
     b->stats.tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
   }
   const int64_t cap = std::max(b->tuning.paths_per_launch, min_fit);
@@ -3260,6 +3189,27 @@ static const DevMesh* pick_mesh(mupsi_scene* sc, int32_t prim) {
   return &sc->meshes[pr.mesh];
 }
 
+// Launches one mesh-query kernel on the context's stream, waits for it, and
+// keeps its device time (HIP events on that stream) for mupsi_last_kernel_ms.
+template <class F>
+static mupsi_status mesh_query_launch(mupsi_scene* sc, F&& launch) {
+  mupsi_ctx* ctx = sc->ctx;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipEventRecord(ctx->ev_begin, ctx->stream));
+  launch(ctx->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ctx->ev_end, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipEventElapsedTime(&ctx->last_kernel_ms, ctx->ev_begin, ctx->ev_end));
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_last_kernel_ms(mupsi_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return fail(MUPSI_ERR_INVALID, "mupsi_last_kernel_ms: null argument");
+  *ms = ctx->last_kernel_ms;
+  return MUPSI_OK;
+}
+
 extern "C" mupsi_status mupsi_mesh_intersect(mupsi_scene* sc, int32_t prim, const float* ox,
                                              const float* oy, const float* oz, const float* dx,
                                              const float* dy, const float* dz,
@@ -3268,13 +3218,11 @@ extern "C" mupsi_status mupsi_mesh_intersect(mupsi_scene* sc, int32_t prim, cons
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_intersect: no such mesh");
   if (n <= 0) return MUPSI_OK;
-  HIP_TRY(hipSetDevice(sc->ctx->device));
   RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
-  hipLaunchKernelGGL(k_mesh_intersect, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, rs, n,
-                     tri, t, u, v);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
-  return MUPSI_OK;
+  return mesh_query_launch(sc, [&](hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_intersect, dim3(grid_for(n)), dim3(256), 0, st, *m, rs, n, tri, t,
+                       u, v);
+  });
 }
 
 extern "C" mupsi_status mupsi_mesh_occluded(mupsi_scene* sc, int32_t prim, const float* ox,
@@ -3284,13 +3232,10 @@ extern "C" mupsi_status mupsi_mesh_occluded(mupsi_scene* sc, int32_t prim, const
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_occluded: no such mesh");
   if (n <= 0) return MUPSI_OK;
-  HIP_TRY(hipSetDevice(sc->ctx->device));
   RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
-  hipLaunchKernelGGL(k_mesh_occluded, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, rs, n,
-                     occ);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
-  return MUPSI_OK;
+  return mesh_query_launch(sc, [&](hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_occluded, dim3(grid_for(n)), dim3(256), 0, st, *m, rs, n, occ);
+  });
 }
 
 extern "C" mupsi_status mupsi_mesh_closest(mupsi_scene* sc, int32_t prim, const float* px,
@@ -3299,12 +3244,10 @@ extern "C" mupsi_status mupsi_mesh_closest(mupsi_scene* sc, int32_t prim, const 
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_closest: no such mesh");
   if (n <= 0) return MUPSI_OK;
-  HIP_TRY(hipSetDevice(sc->ctx->device));
-  hipLaunchKernelGGL(k_mesh_closest, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, px, py,
-                     pz, n, dist, tri, closest3);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
-  return MUPSI_OK;
+  return mesh_query_launch(sc, [&](hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_closest, dim3(grid_for(n)), dim3(256), 0, st, *m, px, py, pz, n,
+                       dist, tri, closest3);
+  });
 }
 
 extern "C" mupsi_status mupsi_mesh_inside(mupsi_scene* sc, int32_t prim, const float* px,
@@ -3313,12 +3256,25 @@ extern "C" mupsi_status mupsi_mesh_inside(mupsi_scene* sc, int32_t prim, const f
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_inside: no such mesh");
   if (n <= 0) return MUPSI_OK;
-  HIP_TRY(hipSetDevice(sc->ctx->device));
-  hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(n)), dim3(256), 0, sc->ctx->stream, *m, px, py,
-                     pz, n, inside);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
-  return MUPSI_OK;
+  return mesh_query_launch(sc, [&](hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(n)), dim3(256), 0, st, *m, px, py, pz, n,
+                       inside);
+  });
+}
+
+extern "C" mupsi_status mupsi_mesh_query_cost(mupsi_scene* sc, int32_t prim, int32_t kind,
+                                              const float* px, const float* py, const float* pz,
+                                              int64_t n, uint32_t* nodes, uint32_t* tris) {
+  const DevMesh* m = pick_mesh(sc, prim);
+  if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: no such mesh");
+  if (kind < 0 || kind > 2) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: kind must be 0, 1 or 2");
+  if (n <= 0) return MUPSI_OK;
+  if (!px || !py || !pz || !nodes || !tris)
+    return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: null buffer");
+  return mesh_query_launch(sc, [&](hipStream_t st) {
+    hipLaunchKernelGGL(k_mesh_cost, dim3(grid_for(n)), dim3(256), 0, st, *m, (int)kind, px, py,
+                       pz, n, nodes, tris);
+  });
 }
 
 extern "C" mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices,

@@ -21,6 +21,9 @@ struct mupsi_ctx {
   std::vector<mupsi_scene*> scenes;  // live scenes, freed by mupsi_close
   std::vector<mupsi_guide*> guides;  // live guides, freed by mupsi_close
   std::vector<mupsi_batch*> batches; // live batches, freed by mupsi_close
+  // device time of the last mesh-query kernel (mupsi_last_kernel_ms)
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  float last_kernel_ms = 0.0f;
 };
 
 namespace mupsi {

@@ -380,6 +380,11 @@ static float dist2_tri(V3 p, V3 a, V3 b, V3 c, V3& out) {  // bvh.cpp:42-80
   return mn;
 }
 
+// Traversal cost of one query: nodes whose bounds were tested, triangles tested.
+struct TravCost {
+  uint64_t nodes = 0, tris = 0;
+};
+
 struct BVH {
   std::vector<Node> nodes;
   std::vector<uint32_t> tri;
@@ -460,8 +465,10 @@ struct BVH {
     first = nearFirst ? nd.left : nd.right;
     second = nearFirst ? nd.right : nd.left;
   }
-  // BVH::intersect (bvh.cpp:197-250); narrows r.farT
-  bool intersect(Ray& r, uint32_t& ti_out, float& u_out, float& v_out) const {
+  // BVH::intersect (bvh.cpp:197-250); narrows r.farT.  cost (optional) counts
+  // the nodes whose bounds are tested and the triangles tested.
+  bool intersect(Ray& r, uint32_t& ti_out, float& u_out, float& v_out,
+                 TravCost* cost = nullptr) const {
     if (nodes.empty()) return false;
     bool hit = false;
     uint32_t st[64];
@@ -469,10 +476,12 @@ struct BVH {
     st[sp++] = root;
     while (sp > 0) {
       const Node& nd = nodes[st[--sp]];
+      if (cost) ++cost->nodes;
       if (!aabb_hit(nd, r)) continue;
       if (nd.right & LEAF) {
         uint32_t s = nd.left, c = nd.right & ~LEAF;
         for (uint32_t i = s; i < s + c; ++i) {
+          if (cost) ++cost->tris;
           uint32_t ti = tri[i];
           float t, u, vv;
           if (ray_tri(r, v[f[ti].a], v[f[ti].b], v[f[ti].c], t, u, vv)) {
@@ -521,8 +530,9 @@ struct BVH {
     }
     return false;
   }
-  // BVH::closest (bvh.cpp:302-353)
-  float closest(V3 p, uint32_t& ti_out, V3& out) const {
+  // BVH::closest (bvh.cpp:302-353); cost as for intersect, where expanding a
+  // node tests the bounds of both children
+  float closest(V3 p, uint32_t& ti_out, V3& out, TravCost* cost = nullptr) const {
     if (nodes.empty()) return 0.0f;
     uint32_t st[64];
     int sp = 0;
@@ -530,10 +540,12 @@ struct BVH {
     float md = std::numeric_limits<float>::max();
     while (sp > 0) {
       const Node& nd = nodes[st[--sp]];
+      if (cost) ++cost->nodes;
       if (dist2_aabb(p, nd) >= md) continue;
       if (nd.right & LEAF) {
         uint32_t s = nd.left, c = nd.right & ~LEAF;
         for (uint32_t i = s; i < s + c; ++i) {
+          if (cost) ++cost->tris;
           uint32_t ti = tri[i];
           V3 tmp{0, 0, 0};
           float d = dist2_tri(p, v[f[ti].a], v[f[ti].b], v[f[ti].c], tmp);
@@ -546,6 +558,7 @@ struct BVH {
       } else {
         float dl = dist2_aabb(p, nodes[nd.left]);
         float dr = dist2_aabb(p, nodes[nd.right]);
+        if (cost) cost->nodes += 2;
         if (sp > 62) continue;
         if (dl < dr) {
           if (dr < md) st[sp++] = nd.right;
@@ -686,27 +699,27 @@ struct Mesh {
     return bvh.occluded(lr);
   }
   // Mesh::distToPoint (mesh.cpp:325-339)
-  float dist_to_point(V3 p, uint32_t& ti, V3& out) const {
+  float dist_to_point(V3 p, uint32_t& ti, V3& out, TravCost* cost = nullptr) const {
     V3 pl = xf_to_local(xf, p);
     V3 ol = mk(0, 0, 0);
     ti = 0;
-    bvh.closest(pl, ti, ol);
+    bvh.closest(pl, ti, ol, cost);
     out = xf_to_world(xf, ol);
     return norm(sub(out, p));
   }
   // Mesh::inside (mesh.cpp:341-362)
-  bool inside(V3 p) const {
+  bool inside(V3 p, TravCost* cost = nullptr) const {
     V3 pl = xf_to_local(xf, p);
     Ray r = make_ray(pl, mk(1, 0, 0));
     int hits = 0;
     uint32_t ti;
     float u, v;
-    if (bvh.intersect(r, ti, u, v)) {
+    if (bvh.intersect(r, ti, u, v, cost)) {
       ++hits;
       while (r.farT < std::numeric_limits<float>::infinity()) {
         r.nearT = r.farT + 1e-4f;
         r.farT = std::numeric_limits<float>::infinity();
-        if (!bvh.intersect(r, ti, u, v)) break;
+        if (!bvh.intersect(r, ti, u, v, cost)) break;
         ++hits;
       }
     }
@@ -1621,6 +1634,28 @@ void orc_mesh_inside(void* sc, int32_t prim, const float* p3, int64_t n, uint8_t
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = 0; i < n; ++i)
     inside[i] = m && m->inside(mk(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]));
+}
+
+// Traversal cost per point of distToPoint (kind 0) or inside (kind 1).
+void orc_mesh_query_cost(void* sc, int32_t prim, int32_t kind, const float* p3, int64_t n,
+                         uint32_t* nodes, uint32_t* tris) {
+  const Mesh* m = mesh_of((Scene*)sc, prim);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i) {
+    TravCost c;
+    const V3 p = mk(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
+    if (m) {
+      if (kind == 0) {
+        uint32_t ti = 0;
+        V3 o = mk(0, 0, 0);
+        (void)m->dist_to_point(p, ti, o, &c);
+      } else {
+        (void)m->inside(p, &c);
+      }
+    }
+    nodes[i] = (uint32_t)c.nodes;
+    tris[i] = (uint32_t)c.tris;
+  }
 }
 
 void orc_camera_rays(void* sc, const int32_t* x, const int32_t* y, int64_t n, float* dir3) {

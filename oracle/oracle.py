@@ -37,6 +37,7 @@ _SIGS = {
     "orc_mesh_occluded": (None, [VP, I32, VP, I64, VP]),
     "orc_mesh_closest": (None, [VP, I32, VP, I64, VP, VP, VP]),
     "orc_mesh_inside": (None, [VP, I32, VP, I64, VP]),
+    "orc_mesh_query_cost": (None, [VP, I32, I32, VP, I64, VP, VP]),
     "orc_camera_rays": (None, [VP, VP, VP, I64, VP]),
     "orc_sky_lookup": (None, [VP, VP, I64, VP, VP]),
     "orc_trace_paths": (None, [VP, VP, VP, VP, I64, VP, I32]),
@@ -223,6 +224,16 @@ class OracleScene:
         cl = np.zeros((n, 3), np.float32)
         lib().orc_mesh_closest(self.h, prim, _p(p), n, _p(d), _p(tri), _p(cl))
         return d, tri, cl
+
+    def mesh_query_cost(self, prim, kind, p):
+        """Per point (n, 3): BVH nodes and triangles tested by distToPoint (kind 0)
+        or inside (kind 1)."""
+        p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+        n = p.shape[0]
+        nodes = np.zeros(n, dtype=np.uint32)
+        tris = np.zeros(n, dtype=np.uint32)
+        lib().orc_mesh_query_cost(self.h, prim, kind, _p(p), n, _p(nodes), _p(tris))
+        return nodes.astype(np.int64), tris.astype(np.int64)
 
     def mesh_inside(self, prim, p):
         p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)

@@ -59,3 +59,25 @@ def test_batch_autotune_and_repeat(ctx):
     for s1, s2 in zip(ref, bs):
         np.testing.assert_array_equal(s1.read_accum()[0], s2.read_accum()[0])
     b.close()
+
+
+def test_batch_probes_marcher_grid(ctx):
+    """With a fixed launch size and the marcher grid left to the autotuner, the
+    candidates are probed on real passes (several launches each) and the image
+    stays that of the scenes rendered alone."""
+    from mupsi_amd import Batch, Scene
+    scs = _small_batch()[:3]
+    ref = [Scene(ctx, sc) for sc in scs]
+    for s in ref:
+        s.render_passes(0, 16)
+    bs = [Scene(ctx, sc) for sc in scs]
+    cap = max(sc["width"] * sc["height"] for sc in scs)
+    b = Batch(ctx, bs, tuning=dict(paths_per_launch=cap, march_blocks_per_cu=0, use_graph=True))
+    b.render(0, 16)
+    t = b.tuning()
+    st = b.stats()
+    assert t["paths_per_launch"] == cap and t["march_blocks_per_cu"] in (4, 8, 16)
+    assert st["launches"] >= 6 and st["tune_ms"] > 0
+    for s1, s2 in zip(ref, bs):
+        np.testing.assert_array_equal(s1.read_accum()[0], s2.read_accum()[0])
+    b.close()

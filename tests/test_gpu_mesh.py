@@ -94,3 +94,25 @@ def test_mesh_mean_march(ctx, dev):
     ok = (hit == ref64["hit"]) & ((~hit.astype(bool)) | (t.view(np.uint32) == ref64["t"].view(np.uint32)))
     assert ok[unamb].all()
     assert hit.sum() > 20
+
+
+@pytest.mark.parametrize("prim", [0, -1])
+def test_traversal_cost_equals_oracle(ctx, dev, prim):
+    """The device traversal tests the same nodes and triangles as the oracle's
+    restatement of bvh.cpp (same topology, same near/far order, same pruning)."""
+    import torch
+    from mupsi_amd import Scene, batch
+    sc = _mesh_scene()
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(12)
+    p = (np.array([0, 0, -5.0]) + rng.normal(0, 2.5, (20000, 3))).astype(np.float32)
+    pt = torch.from_numpy(p).to(dev)
+    for kind in (0, 1):
+        nodes, tris = (x.cpu().numpy() for x in batch.mesh_query_cost(s, prim, kind, pt))
+        onodes, otris = osc.mesh_query_cost(prim, kind, p)
+        assert np.array_equal(nodes, onodes), kind
+        assert np.array_equal(tris, otris), kind
+        assert tris.sum() > 20000
+    batch.mesh_closest(s, prim, pt)
+    assert batch.last_kernel_ms(ctx) > 0.0

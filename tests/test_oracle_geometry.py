@@ -186,3 +186,47 @@ def test_sky_lookup_texel_closed_form():
     ok = (tx == y * 64 + x) | near_edge
     assert ok.all()
     np.testing.assert_array_equal(rgb, sky.reshape(-1, 3)[tx])
+
+
+def _quad_mesh_scene():
+    """Two triangles in the plane z = 0 spanning [-1, 1]^2: one BVH leaf."""
+    v = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return {"width": 8, "height": 8, "camera": dict(scenes.CAMERA),
+            "primitives": [{"type": "mesh", "bsdf": "lambertian",
+                            "mesh": {"vertices": v, "faces": f}}]}
+
+
+def test_traversal_cost_closed_forms():
+    """A single-leaf mesh: closest tests the root and both triangles; inside casts
+    (crossings + 1) rays, each testing the root and, when its box is hit, both
+    triangles.  A point off the +x slab of the box misses the root: 1 node, 0
+    triangles."""
+    osc = O.OracleScene(_quad_mesh_scene())
+    p = np.array([[0.2, 0.3, 0.5],      # above the quad: the +x ray misses the box (z = 0.5)
+                  [-3.0, 0.1, 0.0],     # in the plane: the +x ray runs inside the quad's plane
+                  [0.0, 0.0, 0.0]], np.float32)
+    nodes, tris = osc.mesh_query_cost(0, 0, p)
+    assert np.array_equal(nodes, [1, 1, 1]) and np.array_equal(tris, [2, 2, 2])
+    nodes, tris = osc.mesh_query_cost(0, 1, p[:1])
+    assert (nodes[0], tris[0]) == (1, 0)
+
+
+def test_traversal_cost_bounds_on_icosphere():
+    """Closest: the root is expanded (its two children tested) and at least one
+    leaf's triangles are tested; inside: at least one node per cast and a
+    triangle test for every crossing found."""
+    sc, m = _mesh_scene(sub=3, disp=0.05)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(11)
+    p = (np.array([0.5, -0.25, -8.0]) + rng.normal(0, 2.5, (3000, 3))).astype(np.float32)
+    nodes, tris = osc.mesh_query_cost(0, 0, p)
+    assert np.all(nodes >= 3) and np.all(tris >= 1)
+    n_faces = m["faces"].shape[0]
+    assert np.all(tris <= n_faces)
+    # points near the surface prune most of the tree
+    assert np.median(tris) < 0.05 * n_faces
+    ni, ti = osc.mesh_query_cost(0, 1, p)
+    ins = osc.mesh_inside(0, p).astype(bool)
+    assert np.all(ni >= 1)
+    assert np.all(ti[ins] >= 1)          # an inside point crossed at least one triangle
