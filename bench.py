@@ -7,7 +7,7 @@ Workload (one step): the sphere-mean GPIS scene in renewal_plus mode at
 every pixel, i.e. 4,194,304 paths, rendered into the FP64 accumulator.
 
 Reported next to it:
-  roofline      the GP-medium march kernel (k_march_cached<ppc>): algorithmic FP32
+  roofline      the GP-medium march kernel (k_march_window<ppc>): algorithmic FP32
                 flops per launch (field evaluations x (27*ppc*16 + 9)) over its
                 HIP-event time, against the MI355X FP32 vector peak; its HBM traffic
                 per launch is the rocprofv3 FETCH_SIZE + WRITE_SIZE figure measured for
@@ -20,8 +20,9 @@ Reported next to it:
                 the plain and the guided tracer to reach 1e-3 relative MSE
                 (mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)).
                 Both tracers are unbiased, so relMSE is the per-pixel variance of
-                the progressive average, measured from independent renders; the
-                guided time includes its 32 online-training passes.
+                the progressive average over R^2 + 0.01, measured from independent
+                renders against a 1024-pass plain R; the guided time includes its
+                16 online-training passes.
 """
 from __future__ import annotations
 
@@ -36,7 +37,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per k_march_cached<3> launch on this workload: rocprofv3 --pmc FETCH_SIZE and
+# HBM bytes per k_march_window<3> launch on this workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`, FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
 # profiles/k_march_cached_sphere_renewal_256x256x64.md holds the table.
@@ -67,9 +68,12 @@ def cpu_baseline(scene: dict, target_s: float = 15.0) -> dict:
     }
 
 
-def _chunk_stats(s, k0: int, chunks: int, spp: int):
-    """Per-pass relative variance v (mean over pixels and channels) and seconds
-    per pass, from `chunks` independent renders of `spp` passes."""
+def _chunk_stats(s, k0: int, chunks: int, spp: int, R):
+    """Per-pass relative variance v (mean over pixels and channels of the
+    per-pixel variance of one pass over R^2 + 0.01, R an independent reference
+    image) and seconds per pass, from `chunks` independent renders of `spp`
+    passes.  R must not come from the same samples: a pixel whose samples hold a
+    rare bright path would otherwise carry it in its own denominator."""
     import numpy as np
     import torch
     ims = []
@@ -83,7 +87,6 @@ def _chunk_stats(s, k0: int, chunks: int, spp: int):
         secs += time.perf_counter() - t0
         ims.append(s.image())
     ims = np.array(ims)
-    R = ims.mean(0)
     v = float(np.mean(ims.var(0, ddof=1) * spp / (R ** 2 + 0.01)))
     return v, secs / (chunks * spp)
 
@@ -97,16 +100,20 @@ def _passes_to_target(v_t: float, n_t: int, v_g: float, target: float) -> int:
     return n
 
 
-def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs: int = 4) -> dict:
+def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs: int = 4,
+                  ref_passes: int = 1024) -> dict:
     """Time to `target` relMSE, plain vs guided, on the learned-guiding workload.
 
-    Plain: per-pass relative variance v_p from independent chunks -> v_p / target
+    relMSE(I) = mean over pixels and channels of (I - R)^2 / (R^2 + 0.01) with R
+    the converged image; both tracers are unbiased, so the relMSE of an N-pass
+    progressive image is v / N with v the per-pass variance over R^2 + 0.01.  R
+    here is a ref_passes-pass plain render on its own pass range (its noise only
+    enters the denominators).  Plain: v_p from independent chunks -> v_p / target
     passes.  Guided (progressive, as render_passes accumulates it): `runs`
-    independent online trainings of train_passes passes give the training-phase
-    image's variance (network randomness included) and the training time; the
-    last trained network's per-pass variance v_g comes from chunks; the time to
-    target counts every pass of the progressive image.  The figure that discards
-    the training passes' samples is reported too."""
+    independent online trainings give the training-phase variance (network
+    randomness included) and the training time; the trained network's v_g comes
+    from chunks; the time to target counts every pass of the progressive image.
+    The figure that discards the training passes' samples is reported too."""
     import numpy as np
     import torch
     from mupsi_amd import Guide, Scene, scenes
@@ -114,10 +121,14 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
     W, H = sc["width"], sc["height"]
     plain = Scene(ctx, sc)
     plain.render_passes(0, 1)                         # allocations outside the timing
-    v_p, t_p = _chunk_stats(plain, 1000, chunks, spp)
     plain.reset()
-    plain.render_passes(20000, 256)
-    R = plain.image()                                 # 256-pass mean, for the denominators
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.render_passes(100000, ref_passes)
+    torch.cuda.synchronize()
+    ref_s = time.perf_counter() - t0
+    R = plain.image()
+    v_p, t_p = _chunk_stats(plain, 1000, chunks, spp, R)
     plain.close()
     guided = Scene(ctx, sc)
     warm = Guide.for_scene(ctx, sc)
@@ -144,17 +155,17 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
     imgs = np.array(imgs)
     v_t = float(np.mean(imgs.var(0, ddof=1) * n_t / (R ** 2 + 0.01)))
     t_train = float(np.median(t_trains))
-    v_g, t_g = _chunk_stats(guided, 3000, chunks, spp)
+    v_g, t_g = _chunk_stats(guided, 3000, chunks, spp, R)
     st = g.stats()
     # the same guided tracer with its roulette but without the network (alpha = 0)
     ctl = Guide.for_scene(ctx, sc, alpha=0.0, train_passes=0)
     guided.set_guide(ctl)
-    v_c, _ = _chunk_stats(guided, 4000, chunks // 2, spp)
+    v_c, _ = _chunk_stats(guided, 4000, chunks // 2, spp, R)
     # the network with the reference's roulette
     g1 = Guide.for_scene(ctx, sc, rr_boost=1.0)
     guided.set_guide(g1)
     guided.render_passes(5000, g1.config["train_passes"])
-    v_1, _ = _chunk_stats(guided, 6000, chunks // 2, spp)
+    v_1, _ = _chunk_stats(guided, 6000, chunks // 2, spp, R)
     guided.close()
     time_plain = t_p * v_p / target
     n_star = _passes_to_target(v_t, n_t, v_g, target)
@@ -169,6 +180,8 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
                     f"(area 0.01, radiance 200), {W}x{H}, guide trained online on the first "
                     f"{n_t} passes",
         "target_relmse": target,
+        "reference": {"passes": ref_passes, "s": ref_s,
+                      "relmse_of_reference": v_p / ref_passes},
         "plain": {"v_per_pass": v_p, "s_per_pass": t_p, "Mpaths_s": paths / t_p / 1e6,
                   "passes_to_target": v_p / target, "time_to_target_s": time_plain},
         "guided": {"v_per_pass_trained": v_g, "v_per_pass_training_phase": v_t,
@@ -339,6 +352,8 @@ def main() -> int:
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the configuration 1 / 3 / 4 and BVH traversal measurements")
+    ap.add_argument("--only", choices=["guiding", "configs", "bvh"], default=None,
+                    help="run one section alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="copy a tensor of this many MiB before the timed region, so that a "
                          "profiler's FETCH_SIZE / WRITE_SIZE can be scaled by a known traffic")
@@ -351,6 +366,19 @@ def main() -> int:
 
     import torch
     from mupsi_amd import Context, Scene, scenes
+
+    if args.only:
+        ctx = Context(0)
+        print(f"bench: {ctx.info()} section {args.only}", file=sys.stderr, flush=True)
+        if args.only == "guiding":
+            out = guiding_bench(ctx)
+        elif args.only == "configs":
+            out = configs_bench(ctx)
+        else:
+            out = traversal_bench(ctx, scenes.config_scene(3))
+        print(json.dumps(out), flush=True)
+        ctx.close()
+        return 0
 
     scene = scenes.config_scene(CFG)
     W, H, spp = scene["width"], scene["height"], scene["spp"]
@@ -429,7 +457,7 @@ def main() -> int:
                    "paths_per_step": W * H * spp, "tracer": "plain wavefront"},
         "roofline": {
             "bound": "valu",
-            "kernel": f"k_march_cached<{ppc}>",
+            "kernel": f"k_march_window<{ppc}, false, false>",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

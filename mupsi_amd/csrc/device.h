@@ -628,12 +628,25 @@ __device__ __forceinline__ float mean_eval(const DevScene& S, F3 p, F3* grad) {
   return sg * d;
 }
 
-// GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).
+// The mean and its gradient at one point, for callers that need them twice.
+struct MeanAt {
+  float mu;
+  F3 grad;
+};
+
+// GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).  mean: the
+// mean's value and gradient at p when the caller has them already.
 template <bool kGrad>
 __device__ __forceinline__ float field_eval(const DevScene& S, F3 p, uint32_t seed, const Cond& c,
-                                            F3* grad) {
+                                            F3* grad, const MeanAt* mean = nullptr) {
   F3 mg = mk(0, 0, 0), pg = mk(0, 0, 0);
-  const float mu = mean_eval<kGrad>(S, p, &mg);
+  float mu;
+  if (mean) {
+    mu = mean->mu;
+    mg = mean->grad;
+  } else {
+    mu = mean_eval<kGrad>(S, p, &mg);
+  }
   float pv;
   psi_eval<kGrad>(S.gp, p, seed, pv, pg, nullptr);
   float f = mu + pv;

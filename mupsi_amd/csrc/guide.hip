@@ -454,10 +454,10 @@ std::vector<float> init_params(uint64_t seed) {
   fill(kOffW3, kHid, kHid, 1.0f);
   fill(kOffW4, kOut, kHid, 0.1f);
   // log kappa = -3 + 11 sigmoid(s): s = -0.56 starts every lobe near kappa = e;
-  // the selection output starts low (alpha = alpha_max sigmoid(-2) = 0.12 alpha_max),
+  // the selection output starts low (alpha = alpha_max sigmoid(-4) = 0.018 alpha_max),
   // so the first passes sample close to the cosine lobe until the guide has learnt
   for (int k = 0; k < kLobes; ++k) th[kOffB4 + 5 * k + 3] = -0.56f;
-  th[kOffB4 + kAlphaIdx] = -2.0f;
+  th[kOffB4 + kAlphaIdx] = -4.0f;
   return th;
 }
 

@@ -1089,12 +1089,18 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
 // which gives the reference's closed form u = -mu/sigma - psi,
 // g = (-L^2/2)(n - grad mu - grad psi).
 __device__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next, double& u,
-                              double g[3]) {
+                              double g[3], const MeanAt* mean = nullptr) {
   F3 pg;
   float pv;
   psi_eval<true>(S.gp, C, seed_next, pv, pg, nullptr);
   F3 mg;
-  const float mu = mean_eval<true>(S, C, &mg);
+  float mu;
+  if (mean) {  // the same mean_eval<true>(S, C) the caller made
+    mu = mean->mu;
+    mg = mean->grad;
+  } else {
+    mu = mean_eval<true>(S, C, &mg);
+  }
   const double L2 = S.gp.L2;
   const double r[3] = {0.0, 0.0, 0.0};  // C - C
   const double h = exp(-(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) / L2);
@@ -1286,8 +1292,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
     close_record(L);
     const uint32_t iseed = P.mseed[pid];
     const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
+    // the mean at the hit serves the normal and the Renewal+ solve at C = Pp
+    MeanAt mp;
+    mp.mu = mean_eval<true>(S, Pp, &mp.grad);
     F3 gr;
-    field_eval<true>(S, Pp, iseed, cur, &gr);
+    field_eval<true>(S, Pp, iseed, cur, &gr, &mp);
     const F3 nrm = normalized(gr);
     const F3 wo = neg(ray.d);
     const F3 alb = mk(0.8f, 0.8f, 0.8f);
@@ -1366,7 +1375,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
         const uint32_t ns =
             constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u, (uint32_t)(bounce + 1)));
         double u, g[3];
-        renewal_solve(S, Pp, nrm, ns, u, g);
+        renewal_solve(S, Pp, nrm, ns, u, g, &mp);
         cur.active = true;
         cur.C = Pp;
         cur.u = (float)u;
