@@ -40,8 +40,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per k_march_window<3> launch on this workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`, FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
-# profiles/k_march_cached_sphere_renewal_256x256x64.md holds the table.
-MARCH_TRAFFIC_BYTES = 101_084_731
+# profiles/k_march_window_sphere_renewal_256x256x64.md holds the table.
+MARCH_TRAFFIC_BYTES = 101_294_490
 CFG = 2
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "

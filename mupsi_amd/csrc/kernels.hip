@@ -1514,7 +1514,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   if (a.defer_nee) wave_push(shadow, pid, a.qS, a.nS);
 }
 
-// The deferred light samples after their march (k_march / k_march_cached over
+// The deferred light samples after their march (k_march / k_march_window over
 // queue S): add the contribution where the shadow ray found no sign change, in
 // the reference's operation order, before any later event of the path adds to L.
 struct NeeArgs {
