@@ -83,6 +83,9 @@ enum { MUPSI_GP_SINGLE_REALIZATION = 0, MUPSI_GP_RENEWAL_PLUS = 1 };
 enum { MUPSI_MEAN_SPHERE = 0, MUPSI_MEAN_MESH = 1 };
 enum { MUPSI_PRIM_SPHERE = 0, MUPSI_PRIM_MESH = 1 };
 enum { MUPSI_BSDF_LAMBERTIAN = 0, MUPSI_BSDF_NULL = 1, MUPSI_BSDF_SPECULAR = 2 };
+/* The medium inside int_medium spheres (controller.cpp:80-190, 230-234: gp_medium
+ * wins over sdf_medium, which wins over external_gp_medium). */
+enum { MUPSI_MEDIUM_GP = 0, MUPSI_MEDIUM_SDF = 1, MUPSI_MEDIUM_GRID = 2 };
 
 /* SparseSEKernel(sigma, kernelRadius = kernel.length_scale, lengthScale = aniso)
  * + SparseGPNoiseGenerator(kernel, points_per_cell)   controller.cpp:112-120 */
@@ -105,6 +108,17 @@ typedef struct {
   float position[3];
   float scale;
 } mupsi_mesh_desc;
+
+/* ExternalGPMedium (externalgpmedium.h:10-38): mean and optional variance
+ * GridMeanFunction grids (meanfunction.cpp:41-86), res^3 floats each, x-major
+ * (index (i * res + j) * res + k), over the box [bounds_min, bounds_max]. */
+typedef struct {
+  const float* mean;
+  const float* variance;   /* NULL: deterministic medium */
+  int32_t resolution;      /* external_gp_medium.resolution (48) */
+  float bounds_min[3], bounds_max[3];
+  float noise_scale;       /* external_gp_medium.noise_scale (1) */
+} mupsi_grid_medium_desc;
 
 /* RGB float texture, row-major, row 0 at the top (BitmapTexture, texture.cpp:52-69). */
 typedef struct {
@@ -142,6 +156,11 @@ typedef struct {
   mupsi_texture_desc sky;      /* skydrome texture; rgb NULL = no skydrome */
   const mupsi_primitive_desc* primitives;
   int32_t n_primitives;
+  /* which medium has_gp_medium describes: MUPSI_MEDIUM_GP (mean_* + kernel),
+   * MUPSI_MEDIUM_SDF (SDFMedium over the mean_* function, sdfmedium.cpp:16-98)
+   * or MUPSI_MEDIUM_GRID (ExternalGPMedium over grid, externalgpmedium.cpp:9-130) */
+  int32_t medium_kind;
+  mupsi_grid_medium_desc grid;
 } mupsi_scene_desc;
 
 /* Conditioning state (GPConditioningState, medium.h:12-17) as SoA device arrays. */

@@ -21,6 +21,7 @@ STATUS = {
 GP_SINGLE_REALIZATION, GP_RENEWAL_PLUS = 0, 1
 MEAN_SPHERE, MEAN_MESH = 0, 1
 PRIM_SPHERE, PRIM_MESH = 0, 1
+MEDIUM_GP, MEDIUM_SDF, MEDIUM_GRID = 0, 1, 2
 BSDF_LAMBERTIAN, BSDF_NULL, BSDF_SPECULAR = 0, 1, 2
 BSDF_BY_NAME = {"lambertian": BSDF_LAMBERTIAN, "null": BSDF_NULL, "specular": BSDF_SPECULAR}
 
@@ -47,6 +48,12 @@ class PrimitiveDesc(C.Structure):
                 ("int_medium", C.c_int32), ("mesh", MeshDesc), ("albedo", TextureDesc)]
 
 
+class GridMediumDesc(C.Structure):
+    _fields_ = [("mean", c_float_p), ("variance", c_float_p), ("resolution", C.c_int32),
+                ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3),
+                ("noise_scale", C.c_float)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("cam_pos", C.c_float * 3),
                 ("cam_dir", C.c_float * 3), ("cam_up", C.c_float * 3), ("fov", C.c_float),
@@ -54,7 +61,8 @@ class SceneDesc(C.Structure):
                 ("has_gp_medium", C.c_int32), ("mean_type", C.c_int32),
                 ("mean_center", C.c_float * 3), ("mean_radius", C.c_float),
                 ("mean_mesh", MeshDesc), ("kernel", GpParams), ("sky", TextureDesc),
-                ("primitives", C.POINTER(PrimitiveDesc)), ("n_primitives", C.c_int32)]
+                ("primitives", C.POINTER(PrimitiveDesc)), ("n_primitives", C.c_int32),
+                ("medium_kind", C.c_int32), ("grid", GridMediumDesc)]
 
 
 class CondSoA(C.Structure):

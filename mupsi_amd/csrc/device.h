@@ -144,11 +144,22 @@ struct DevGP {
   float grad_h_max;                    // max |grad h| = sqrt(2) e^{-1/2} / L
   int ppc;
 };
+// ExternalGPMedium's grids (GridMeanFunction, meanfunction.cpp:41-86).
+struct DevGrid {
+  const float* mean;
+  const float* var;  // or null
+  int res;
+  float bmin[3], inv[3];  // inv = res / (bmax - bmin)
+  float noise_scale;
+};
+
 struct DevScene {
   float cam_pos[3], cam_dir[3], cam_right[3], cam_tup[3];
   float aspect, tan_half;
   int w, h;
-  int max_bounce, gp_mode, skip_medium, has_gp;
+  int max_bounce, gp_mode, skip_medium, has_gp;  // has_gp: a medium of medium_kind exists
+  int medium_kind;                               // MUPSI_MEDIUM_*
+  DevGrid grid;
   int mean_type;
   float mean_c[3], mean_r;
   DevMesh mean_mesh;
@@ -633,6 +644,78 @@ struct MeanAt {
   float mu;
   F3 grad;
 };
+
+// ------------------------------------------------------------------ SDF / grid media
+// GridMeanFunction::eval (meanfunction.cpp:56-76): trilinear over cell centres,
+// indices clamped to [0, res - 2], weights clamped to [0, 1]; data x-major.
+__device__ __forceinline__ float grid_eval(const DevGrid& g, const float* data, F3 p) {
+  const float fx = fmaf(p.x - g.bmin[0], g.inv[0], -0.5f);
+  const float fy = fmaf(p.y - g.bmin[1], g.inv[1], -0.5f);
+  const float fz = fmaf(p.z - g.bmin[2], g.inv[2], -0.5f);
+  const int i0 = max(0, min(g.res - 2, (int)floorf(fx)));
+  const int j0 = max(0, min(g.res - 2, (int)floorf(fy)));
+  const int k0 = max(0, min(g.res - 2, (int)floorf(fz)));
+  const float wx = fminf(fmaxf(fx - (float)i0, 0.0f), 1.0f);
+  const float wy = fminf(fmaxf(fy - (float)j0, 0.0f), 1.0f);
+  const float wz = fminf(fmaxf(fz - (float)k0, 0.0f), 1.0f);
+  const size_t sx = (size_t)g.res * g.res, sy = (size_t)g.res;
+  const auto at = [&](int i, int j, int k) { return data[i * sx + j * sy + k]; };
+  const float ux = 1.0f - wx, uy = 1.0f - wy, uz = 1.0f - wz;
+  const float c00 = fmaf(at(i0 + 1, j0, k0), wx, at(i0, j0, k0) * ux);
+  const float c01 = fmaf(at(i0 + 1, j0, k0 + 1), wx, at(i0, j0, k0 + 1) * ux);
+  const float c10 = fmaf(at(i0 + 1, j0 + 1, k0), wx, at(i0, j0 + 1, k0) * ux);
+  const float c11 = fmaf(at(i0 + 1, j0 + 1, k0 + 1), wx, at(i0, j0 + 1, k0 + 1) * ux);
+  const float c0 = fmaf(c10, wy, c00 * uy);
+  const float c1 = fmaf(c11, wy, c01 * uy);
+  return fmaf(c1, wz, c0 * uz);
+}
+// GridMeanFunction::gradient (meanfunction.cpp:78-84): central differences, eps 1e-4.
+__device__ __forceinline__ F3 grid_grad(const DevGrid& g, const float* data, F3 p) {
+  const float e = 1e-4f, e2 = 2.0f * e;
+  const float dx = grid_eval(g, data, mk(p.x + e, p.y, p.z)) - grid_eval(g, data, mk(p.x - e, p.y, p.z));
+  const float dy = grid_eval(g, data, mk(p.x, p.y + e, p.z)) - grid_eval(g, data, mk(p.x, p.y - e, p.z));
+  const float dz = grid_eval(g, data, mk(p.x, p.y, p.z + e)) - grid_eval(g, data, mk(p.x, p.y, p.z - e));
+  return mk(dx / e2, dy / e2, dz / e2);
+}
+// hash11 (externalgpmedium.cpp:8-17): the float bits of p mixed into seed.
+__device__ __forceinline__ float hash11(float x, float y, float z, uint32_t seed) {
+  uint32_t h = seed;
+  h = h * 0x9e3779b1u + __float_as_uint(x);
+  h = h * 0x9e3779b1u + __float_as_uint(y);
+  h = h * 0x9e3779b1u + __float_as_uint(z);
+  h = (h ^ (h >> 16)) * 0x85ebca6bu;
+  h = (h ^ (h >> 13)) * 0xc2b2ae35u;
+  h = h ^ (h >> 16);
+  return (float)(h & 0xFFFFFFu) / 16777215.0f * 2.0f - 1.0f;
+}
+// ExternalGPMedium::evalWithGradient (externalgpmedium.cpp:31-58): grid mean
+// plus noise_scale * sqrt(variance) * hash(p, seed); the gradient adds the
+// hash's central differences (eps 0.01).
+template <bool kGrad>
+__device__ __forceinline__ float grid_medium_eval(const DevScene& S, F3 p, uint32_t seed, F3* grad) {
+  const DevGrid& g = S.grid;
+  const float mu = grid_eval(g, g.mean, p);
+  float noise = 0.0f;
+  F3 ng = mk(0, 0, 0);
+  if (g.var && g.noise_scale > 0.0f) {
+    const float var = grid_eval(g, g.var, p);
+    if (var > 0.0f) {
+      const float amp = g.noise_scale * sqrtf(var);
+      noise = amp * hash11(p.x, p.y, p.z, seed);
+      if (kGrad) {
+        const float e = 0.01f, e2 = 2.0f * e;
+        ng.x = amp * (hash11(p.x + e, p.y, p.z, seed) - hash11(p.x - e, p.y, p.z, seed)) / e2;
+        ng.y = amp * (hash11(p.x, p.y + e, p.z, seed) - hash11(p.x, p.y - e, p.z, seed)) / e2;
+        ng.z = amp * (hash11(p.x, p.y, p.z + e, seed) - hash11(p.x, p.y, p.z - e, seed)) / e2;
+      }
+    }
+  }
+  if (kGrad) {
+    const F3 mg = grid_grad(g, g.mean, p);
+    *grad = mk(mg.x + ng.x, mg.y + ng.y, mg.z + ng.z);
+  }
+  return mu + noise;
+}
 
 // GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).  mean: the
 // mean's value and gradient at p when the caller has them already.

@@ -570,9 +570,109 @@ __device__ __forceinline__ int classify_point(const DevScene& S, const SkipInfo&
   return 0;
 }
 
+// The medium's field and gradient at p, any medium kind: GPMedium (mean + psi +
+// conditioning), SDFMedium (the mean function alone, sdfmedium.cpp:10-14) or
+// ExternalGPMedium (grids + hash noise; the conditioning state is ignored,
+// externalgpmedium.cpp:26-29,60-63).
+template <bool kGrad>
+__device__ __forceinline__ float medium_field(const DevScene& S, F3 p, uint32_t seed, const Cond& c,
+                                              F3* grad) {
+  if (S.medium_kind == MUPSI_MEDIUM_SDF) return mean_eval<kGrad>(S, p, grad);
+  if (S.medium_kind == MUPSI_MEDIUM_GRID) return grid_medium_eval<kGrad>(S, p, seed, grad);
+  return field_eval<kGrad>(S, p, seed, c, grad);
+}
+
+// std::max(a, b) of the reference's step rules: (a < b) ? b : a
+__device__ __forceinline__ float std_max(float a, float b) { return a < b ? b : a; }
+
+// SDFMedium::sampleDistance / transmittance (sdfmedium.cpp:16-98): sphere
+// tracing on the mean function, step max(|f|, 1e-3), at most 512 steps; a sign
+// change (f_prev * f < 0) is refined by 16 bisection steps and the hit is hi.
+// iters: steps taken.  Transmittance takes the same decision without the
+// bisection.
+__device__ bool sdf_march(const DevScene& S, F3 o, F3 d, float nt, float ft, float& t_hit,
+                          F3& p_hit, int& iters) {
+  float t = nt;
+  float fc = mean_eval<false>(S, o, nullptr);
+  int iter = 0;
+  while (t < ft && iter < 512) {
+    const F3 p = madd(d, t, o);
+    const float fp = fc, tp = t;
+    fc = mean_eval<false>(S, p, nullptr);
+    if (fp * fc < 0.0f) {
+      float lo = tp, hi = t, flo = fp;
+      for (int b = 0; b < 16; ++b) {
+        const float mid = (lo + hi) * 0.5f;
+        const float fm = mean_eval<false>(S, madd(d, mid, o), nullptr);
+        if (flo * fm <= 0.0f) {
+          hi = mid;
+        } else {
+          lo = mid;
+          flo = fm;
+        }
+      }
+      t_hit = hi;
+      p_hit = madd(d, hi, o);
+      iters = iter + 1;
+      return true;
+    }
+    t += std_max(fabsf(fc), 1e-3f);
+    ++iter;
+  }
+  iters = iter;
+  return false;
+}
+
+// ExternalGPMedium::sampleDistance / transmittance (externalgpmedium.cpp:68-130):
+// the same stepping with min step 0.005 and at most 1024 steps, GPMedium's sign
+// test (sign(x) = x > 0 and |f_prev - f| > 1e-6), bisection from
+// t - max(|f_prev|, 0.005) to t keeping f_prev's sign at lo.
+__device__ bool grid_march(const DevScene& S, F3 o, F3 d, float nt, float ft, uint32_t seed,
+                           float& t_hit, F3& p_hit, int& iters) {
+  const float min_step = 0.005f;
+  float t = nt;
+  float fprev = grid_medium_eval<false>(S, o, seed, nullptr);
+  int iter = 0;
+  for (; iter < 1024 && t < ft; ++iter) {
+    const F3 p = madd(d, t, o);
+    const float fc = grid_medium_eval<false>(S, p, seed, nullptr);
+    if ((fc > 0.0f) != (fprev > 0.0f) && fabsf(fprev - fc) > 1e-6f) {
+      float lo = t - std_max(fabsf(fprev), min_step);
+      float hi = t;
+      for (int b = 0; b < 16; ++b) {
+        const float mid = (lo + hi) * 0.5f;
+        const float fm = grid_medium_eval<false>(S, madd(d, mid, o), seed, nullptr);
+        if ((fm > 0.0f) == (fprev > 0.0f)) lo = mid;
+        else hi = mid;
+      }
+      t_hit = hi;
+      p_hit = madd(d, hi, o);
+      iters = iter + 1;
+      return true;
+    }
+    fprev = fc;
+    t += std_max(fabsf(fc), min_step);
+  }
+  iters = iter;
+  return false;
+}
+
+// The march of a medium other than the GP medium (hit flag, t, point).
+__device__ __forceinline__ bool alt_march(const DevScene& S, F3 o, F3 d, float nt, float ft,
+                                          uint32_t seed, float& t_hit, F3& p_hit, int& iters) {
+  if (S.medium_kind == MUPSI_MEDIUM_SDF) return sdf_march(S, o, d, nt, ft, t_hit, p_hit, iters);
+  return grid_march(S, o, d, nt, ft, seed, t_hit, p_hit, iters);
+}
+
 // Serial march of one ray (used for NEE shadow rays in k_shade): the same
 // decisions as k_march without the certain-sign skip.
 __device__ bool march_serial(const DevScene& S, const Ray& ray, uint32_t seed, const Cond& c) {
+  if (S.medium_kind != MUPSI_MEDIUM_GP) {
+    float th;
+    F3 ph;
+    int it;
+    return alt_march(S, ray.o, ray.d, ray.nearT, fminf(ray.farT, 2000.0f), seed, th, ph, it);
+  }
   const float nt = ray.nearT, ft = fminf(ray.farT, 2000.0f);
   const float dt = fminf(0.5f, (ft - nt) / 100);
   float t = nt;
@@ -682,10 +782,29 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sk = make_skip(scene(), c, o, ft);
-            nsteps = 0;
             ++rays;
-            phase = 1;
+            if (scene().medium_kind != MUPSI_MEDIUM_GP) {
+              // SDF / grid media: the whole sphere-traced march of this ray here
+              float th = 0.0f;
+              F3 ph = mk(0, 0, 0);
+              int it = 0;
+              const bool hit = alt_march(scene(), o, d, nt, ft, seed, th, ph, it);
+              a.hit[rid] = hit ? 1 : 0;
+              if (hit) {
+                a.t[rid] = th;
+                a.px[rid] = ph.x;
+                a.py[rid] = ph.y;
+                a.pz[rid] = ph.z;
+              }
+              if (a.steps) a.steps[rid] = hit ? it : -1;
+              evals += it + 1;
+              refsteps += it;
+              phase = 0;
+            } else {
+              sk = make_skip(scene(), c, o, ft);
+              nsteps = 0;
+              phase = 1;
+            }
           } else {
             phase = 3;
           }
@@ -1293,10 +1412,14 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
     const uint32_t iseed = P.mseed[pid];
     const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
     // the mean at the hit serves the normal and the Renewal+ solve at C = Pp
-    MeanAt mp;
-    mp.mu = mean_eval<true>(S, Pp, &mp.grad);
+    MeanAt mp{};
     F3 gr;
-    field_eval<true>(S, Pp, iseed, cur, &gr, &mp);
+    if (S.medium_kind == MUPSI_MEDIUM_GP) {
+      mp.mu = mean_eval<true>(S, Pp, &mp.grad);
+      field_eval<true>(S, Pp, iseed, cur, &gr, &mp);
+    } else {
+      medium_field<true>(S, Pp, iseed, cur, &gr);
+    }
     const F3 nrm = normalized(gr);
     const F3 wo = neg(ray.d);
     const F3 alb = mk(0.8f, 0.8f, 0.8f);
@@ -1371,7 +1494,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
         rec_q = q;
         rec_L = L;
       }
-      if (!single) {
+      if (!single && S.medium_kind == MUPSI_MEDIUM_GP) {  // only GPMedium re-conditions
         const uint32_t ns =
             constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u, (uint32_t)(bounce + 1)));
         double u, g[3];
@@ -1502,7 +1625,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   }
   wave_push(cont, pid, a.qA2, a.nA2);
   if constexpr (kMulti) {
-    const bool cached = S.gp.ppc <= kMaxPpcCached;
+    const bool cached = S.medium_kind == MUPSI_MEDIUM_GP && S.gp.ppc <= kMaxPpcCached;
     const bool mesh = S.mean_type == MUPSI_MEAN_MESH;
     const bool go = cont && medium && !S.skip_medium;
     wave_push(go && cached && !mesh, pid, a.qM2, a.nM2);
@@ -1585,7 +1708,7 @@ __global__ __launch_bounds__(256) void k_guide_prep(DevScene S, GuidePrepArgs a)
   }
   const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
   F3 gr;
-  field_eval<true>(S, Pp, P.mseed[pid], cur, &gr);
+  medium_field<true>(S, Pp, P.mseed[pid], cur, &gr);
   const F3 nrm = normalized(gr);
   float f[guide::kIn];
   guide::encode_features(a.box, Pp.x, Pp.y, Pp.z, nrm.x, nrm.y, nrm.z, -P.dx[pid], -P.dy[pid],
@@ -1791,14 +1914,15 @@ __global__ __launch_bounds__(256) void k_field(DevScene S, const float* px, cons
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   F3 g;
-  const float v = field_eval<true>(S, mk(px[i], py[i], pz[i]), seed[i], load_cond(c, i), &g);
+  const float v = medium_field<true>(S, mk(px[i], py[i], pz[i]), seed[i], load_cond(c, i), &g);
   out4[4 * i] = v;
   out4[4 * i + 1] = g.x;
   out4[4 * i + 2] = g.y;
   out4[4 * i + 3] = g.z;
 }
 
-// Hit normal of sampleDistance: normalize(grad f(p)) (gpmedium.cpp:81-82).
+// Hit normal of sampleDistance: normalize(grad f(p)) (gpmedium.cpp:81-82,
+// sdfmedium.cpp:52-53, externalgpmedium.cpp:101-102).
 __global__ __launch_bounds__(256) void k_hit_normals(DevScene S, const uint8_t* hit,
                                                      const float* p3x, const float* p3y,
                                                      const float* p3z, const uint32_t* seed,
@@ -1807,7 +1931,7 @@ __global__ __launch_bounds__(256) void k_hit_normals(DevScene S, const uint8_t* 
   if (i >= n) return;
   F3 g = mk(0, 0, 0);
   if (hit[i]) {
-    field_eval<true>(S, mk(p3x[i], p3y[i], p3z[i]), seed[i], load_cond(c, i), &g);
+    medium_field<true>(S, mk(p3x[i], p3y[i], p3z[i]), seed[i], load_cond(c, i), &g);
     g = normalized(g);
   }
   nrm[3 * i] = g.x;
@@ -2169,6 +2293,11 @@ extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_des
   const float* dsky = nullptr;
   HIP_TRY(sc->mem.upload(H.sky.data(), H.sky.size(), &dsky));
   D.sky.rgb = dsky;
+  const float *dgm = nullptr, *dgv = nullptr;
+  HIP_TRY(sc->mem.upload(H.grid_mean.data(), H.grid_mean.size(), &dgm));
+  HIP_TRY(sc->mem.upload(H.grid_var.data(), H.grid_var.size(), &dgv));
+  D.grid.mean = dgm;
+  D.grid.var = dgv;
 
   const size_t npix = (size_t)D.w * D.h;
   HIP_TRY(sc->mem.alloc(npix * 3, &sc->accum));
@@ -2314,7 +2443,7 @@ static CachedConsts cached_consts(const DevGP& g) {
 }
 
 static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma, hipStream_t st) {
-  if (S.gp.ppc <= kMaxPpcCached) {
+  if (S.medium_kind == MUPSI_MEDIUM_GP && S.gp.ppc <= kMaxPpcCached) {
     const CachedConsts cc = cached_consts(S.gp);
     const dim3 grid(ctx->march_cached_blocks), block(64);
     const bool mesh = S.mean_type == MUPSI_MEAN_MESH;
@@ -2705,7 +2834,9 @@ __global__ void k_batch_counters(int* c, const int64_t* total) {
   if (i < 16) c[i] = i == 0 ? (int)*total : 0;
 }
 
-static bool scene_uses_cached(const DevScene& S) { return S.gp.ppc <= kMaxPpcCached; }
+static bool scene_uses_cached(const DevScene& S) {
+  return S.medium_kind == MUPSI_MEDIUM_GP && S.gp.ppc <= kMaxPpcCached;
+}
 
 // The kernels of one batched launch (paths of the device plan), on stream st.
 // Eager or under graph capture: nothing here synchronises or touches the host.
@@ -3102,12 +3233,15 @@ static CondSoA cond_soa(const mupsi_cond_soa* c) {
   if (c) r = CondSoA{c->active, c->cx, c->cy, c->cz, c->u_tilde, c->gx, c->gy, c->gz};
   return r;
 }
-static bool gp_ready(mupsi_scene* sc) { return sc && sc->ds.has_gp; }
+static bool medium_ready(mupsi_scene* sc) { return sc && sc->ds.has_gp; }
+static bool gp_ready(mupsi_scene* sc) {
+  return medium_ready(sc) && sc->ds.medium_kind == MUPSI_MEDIUM_GP;
+}
 
 extern "C" mupsi_status mupsi_field_eval(mupsi_scene* sc, const float* px, const float* py,
                                          const float* pz, const uint32_t* seed,
                                          const mupsi_cond_soa* cond, int64_t n, float* out4) {
-  if (!gp_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_field_eval: scene has no GP medium");
+  if (!medium_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_field_eval: scene has no medium");
   if (n <= 0) return MUPSI_OK;
   if (!px || !py || !pz || !seed || !out4) return fail(MUPSI_ERR_INVALID, "mupsi_field_eval: null array");
   HIP_TRY(hipSetDevice(sc->ctx->device));
@@ -3124,7 +3258,7 @@ extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* 
                                     const float* far_t, const uint32_t* seed,
                                     const mupsi_cond_soa* cond, int64_t n, uint8_t* hit, float* t,
                                     float* p, float* nrm, int32_t* steps) {
-  if (!gp_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_march: scene has no GP medium");
+  if (!medium_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_march: scene has no medium");
   if (mode != 0 && mode != 1) return fail(MUPSI_ERR_INVALID, "mupsi_march: mode must be 0 or 1");
   if (n <= 0) return MUPSI_OK;
   if (n > INT32_MAX) return fail(MUPSI_ERR_INVALID, "mupsi_march: too many rays");

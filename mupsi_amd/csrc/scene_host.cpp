@@ -213,15 +213,48 @@ mupsi_status lower_scene(const mupsi_scene_desc& d, HostScene* out, std::string*
   S.gp_mode = d.gp_mode == MUPSI_GP_RENEWAL_PLUS ? 1 : 0;
   S.skip_medium = d.skip_medium ? 1 : 0;
   S.has_gp = d.has_gp_medium ? 1 : 0;
+  S.medium_kind = d.has_gp_medium ? d.medium_kind : MUPSI_MEDIUM_GP;
+  if (S.medium_kind < MUPSI_MEDIUM_GP || S.medium_kind > MUPSI_MEDIUM_GRID) {
+    *err = "unknown medium kind";
+    return MUPSI_ERR_INVALID;
+  }
   S.mean_type = d.mean_type;
   for (int a = 0; a < 3; ++a) S.mean_c[a] = d.mean_center[a];
   S.mean_r = d.mean_radius;
   S.eps = 1e-4f;
   S.near_eps = (float)((1e-4) * 15);
   S.single_seed_i = host_constant_sampler_i(42u);
-  if (S.has_gp) {
-    if (!validate_gp(d.kernel, err)) return MUPSI_ERR_INVALID;
-    S.gp = make_dev_gp(d.kernel);
+  if (S.has_gp && S.medium_kind == MUPSI_MEDIUM_GRID) {
+    // ExternalGPMedium: GridMeanFunction(res, bounds) with inv cell = res / (max - min)
+    const mupsi_grid_medium_desc& g = d.grid;
+    if (!g.mean || g.resolution < 2 || g.resolution > 1024) {
+      *err = "external_gp_medium: mean grid missing or resolution outside [2, 1024]";
+      return MUPSI_ERR_INVALID;
+    }
+    for (int a = 0; a < 3; ++a) {
+      if (!(g.bounds_max[a] > g.bounds_min[a]) || !std::isfinite(g.bounds_max[a] - g.bounds_min[a])) {
+        *err = "external_gp_medium: bounds_max must exceed bounds_min";
+        return MUPSI_ERR_INVALID;
+      }
+      S.grid.bmin[a] = g.bounds_min[a];
+      S.grid.inv[a] = (float)g.resolution / (g.bounds_max[a] - g.bounds_min[a]);
+    }
+    const size_t n = (size_t)g.resolution * g.resolution * g.resolution;
+    H.grid_mean.assign(g.mean, g.mean + n);
+    if (g.variance) H.grid_var.assign(g.variance, g.variance + n);
+    S.grid.res = g.resolution;
+    S.grid.noise_scale = g.noise_scale;
+  }
+  if (S.has_gp && S.medium_kind == MUPSI_MEDIUM_GP && !validate_gp(d.kernel, err))
+    return MUPSI_ERR_INVALID;
+  if (S.has_gp && S.medium_kind != MUPSI_MEDIUM_GRID) {
+    S.gp = S.medium_kind == MUPSI_MEDIUM_GP ? make_dev_gp(d.kernel) : dev::DevGP{};
+    if (S.medium_kind != MUPSI_MEDIUM_GP) {
+      S.gp.ppc = 1;
+      S.gp.kr = 1.0f;
+      S.gp.L2 = 1.0f;
+      S.gp.sqrt_var = 1.0f;
+    }
     if (d.mean_type == MUPSI_MEAN_MESH) {
       if (!lower_mesh(d.mean_mesh, &H.mean_mesh, err)) return MUPSI_ERR_INVALID;
       H.has_mean_mesh = true;

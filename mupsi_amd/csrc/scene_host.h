@@ -37,6 +37,7 @@ struct HostScene {
   std::vector<dev::DevPrim> prims;
   std::vector<std::vector<float>> prim_albedo;  // per primitive (empty = constant)
   std::vector<float> sky;
+  std::vector<float> grid_mean, grid_var;  // ExternalGPMedium grids (res^3 each)
 };
 
 // Validate and lower; returns MUPSI_OK or an error status with *err set.

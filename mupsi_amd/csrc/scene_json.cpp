@@ -5,7 +5,9 @@
 // (src/controller/controller.cpp:40-264):
 //   - "transform": {"position", "scale"} (the README's "translate" is ignored);
 //   - "emission" and "int_medium" are read for spheres only; int_medium is
-//     presence-only and attaches the GP medium;
+//     presence-only and attaches the GP medium, else the sdf_medium, else the
+//     external_gp_medium (whose grid files resolve against the scene file's
+//     directory and, as GridMeanFunction, read as zeros where absent);
 //   - a mesh "texture" applies only to a lambertian bsdf;
 //   - an unknown bsdf name falls back to lambertian, an unknown primitive type
 //     is skipped, an unknown mean_type or a missing "primitives" array fails.
@@ -458,7 +460,21 @@ struct DescStore {
   std::vector<std::unique_ptr<ObjMesh>> meshes;
   std::vector<std::unique_ptr<Image>> images;
   std::vector<mupsi_primitive_desc> prims;
+  std::vector<std::unique_ptr<std::vector<float>>> grids;
 };
+
+// GridMeanFunction(filepath, res, ...) (meanfunction.cpp:41-54): res^3 floats,
+// zero-filled first, so a missing or short file leaves zeros.
+std::vector<float>* read_grid(DescStore& store, const std::string& path, int res) {
+  auto g = std::make_unique<std::vector<float>>((size_t)res * res * res, 0.0f);
+  if (FILE* fp = std::fopen(path.c_str(), "rb")) {
+    const size_t got = std::fread(g->data(), sizeof(float), g->size(), fp);
+    (void)got;
+    std::fclose(fp);
+  }
+  store.grids.push_back(std::move(g));
+  return store.grids.back().get();
+}
 
 void fill_mesh(const ObjMesh& m, mupsi_mesh_desc& d) {
   std::memset(&d, 0, sizeof d);
@@ -530,31 +546,61 @@ extern "C" mupsi_status mupsi_scene_load_json(mupsi_ctx* ctx, const char* path, 
     d.gp_mode = str_v(j.get("gp_mode"), "single_realization") == "renewal_plus"
                     ? MUPSI_GP_RENEWAL_PLUS
                     : MUPSI_GP_SINGLE_REALIZATION;
-    if (j.has("sdf_medium") || j.has("external_gp_medium")) {
-      g_json_error = "sdf_medium / external_gp_medium are not lowered by this build";
-      return MUPSI_ERR_UNSUPPORTED;
-    }
-    if (const Json* gpm = j.get("gp_medium")) {
-      d.has_gp_medium = 1;
-      const std::string mt = str_v(gpm->get("mean_type"), "sphere");
+    // mean function of a gp_medium / sdf_medium block (controller.cpp:85-109,140-168)
+    const auto parse_mean = [&](const Json* blk, const char* what) -> mupsi_status {
+      const std::string mt = str_v(blk->get("mean_type"), "sphere");
       if (mt == "sphere") {
         d.mean_type = MUPSI_MEAN_SPHERE;
-        vec3(gpm->get("mean_center"), d.mean_center);
-        d.mean_radius = num_f(gpm->get("mean_radius"), 70.0f);
+        vec3(blk->get("mean_center"), d.mean_center);
+        d.mean_radius = num_f(blk->get("mean_radius"), 70.0f);
       } else if (mt == "mesh") {
         d.mean_type = MUPSI_MEAN_MESH;
         auto m = std::make_unique<ObjMesh>();
-        if (!load_obj(resolve(dir, str_v(gpm->get("file"), "")), *m, err)) {
-          g_json_error = "gp_medium mesh: " + err;
+        if (!load_obj(resolve(dir, str_v(blk->get("file"), "")), *m, err)) {
+          g_json_error = std::string(what) + " mesh: " + err;
           return MUPSI_ERR_IO;
         }
         fill_mesh(*m, d.mean_mesh);
-        read_transform(gpm->get("transform"), d.mean_mesh);
+        read_transform(blk->get("transform"), d.mean_mesh);
         store.meshes.push_back(std::move(m));
       } else {
         g_json_error = "unknown mean_type: " + mt;
         return MUPSI_ERR_PARSE;
       }
+      return MUPSI_OK;
+    };
+    const Json* gpm = j.get("gp_medium");
+    if (!gpm && j.has("sdf_medium")) {
+      d.has_gp_medium = 1;
+      d.medium_kind = MUPSI_MEDIUM_SDF;
+      const mupsi_status ms = parse_mean(j.get("sdf_medium"), "sdf_medium");
+      if (ms != MUPSI_OK) return ms;
+    } else if (!gpm && j.has("external_gp_medium")) {
+      // ExternalGPMedium (controller.cpp:170-189): grid files relative to the
+      // scene file's directory
+      const Json* e = j.get("external_gp_medium");
+      const std::string sp(path);
+      const size_t slash = sp.find_last_of("/\\");
+      const std::string cfg_dir = slash == std::string::npos ? "" : sp.substr(0, slash + 1);
+      d.has_gp_medium = 1;
+      d.medium_kind = MUPSI_MEDIUM_GRID;
+      mupsi_grid_medium_desc& g = d.grid;
+      g.resolution = num_i(e->get("resolution"), 48);
+      if (g.resolution < 2 || g.resolution > 1024) throw SchemaError{"external_gp_medium.resolution outside [2, 1024]"};
+      vec3(e->get("bounds_min"), g.bounds_min);
+      vec3(e->get("bounds_max"), g.bounds_max);
+      g.noise_scale = num_f(e->get("noise_scale"), 1.0f);
+      g.mean = read_grid(store, cfg_dir + str_v(e->get("mean_file"), ""), g.resolution)->data();
+      // the reference builds a variance grid whenever configDir + variance_file is
+      // non-empty; a missing file reads as zeros (no noise), as no grid does
+      const std::string vf = str_v(e->get("variance_file"), "");
+      g.variance = vf.empty() ? nullptr : read_grid(store, cfg_dir + vf, g.resolution)->data();
+    }
+    if (gpm) {
+      d.has_gp_medium = 1;
+      d.medium_kind = MUPSI_MEDIUM_GP;
+      const mupsi_status ms = parse_mean(gpm, "gp_medium");
+      if (ms != MUPSI_OK) return ms;
       const Json* k = j.get("kernel");
       d.kernel.sigma = num_f(k ? k->get("sigma") : nullptr, 1.0f);
       d.kernel.kernel_radius = num_f(k ? k->get("length_scale") : nullptr, 1.0f);

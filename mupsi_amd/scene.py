@@ -166,6 +166,33 @@ def _texture(h: SceneHolder, img) -> A.TextureDesc:
     return t
 
 
+def read_grid(path: str | os.PathLike, res: int) -> np.ndarray:
+    """A GridMeanFunction file (meanfunction.cpp:41-54): res^3 raw float32, x-major.
+    As the reference, a missing or short file leaves zeros where data is absent."""
+    out = np.zeros(res ** 3, np.float32)
+    try:
+        data = np.fromfile(path, dtype="<f4", count=res ** 3)
+    except (OSError, ValueError):
+        return out
+    out[: data.size] = data
+    return out
+
+
+def _mean_fields(d: A.SceneDesc, h: SceneHolder, block: dict) -> None:
+    """mean_type / mean_center / mean_radius / file+transform of a gp_medium or
+    sdf_medium block (controller.cpp:85-109, 140-168)."""
+    mt = block.get("mean_type", "sphere")
+    if mt == "sphere":
+        d.mean_type = A.MEAN_SPHERE
+        d.mean_center[:] = [float(x) for x in block["mean_center"]]
+        d.mean_radius = float(block.get("mean_radius", 70.0))
+    elif mt == "mesh":
+        d.mean_type = A.MEAN_MESH
+        d.mean_mesh = _mesh_desc(h, block.get("mesh", block.get("file")), block.get("transform"))
+    else:
+        raise ValueError(f"unknown mean_type: {mt}")
+
+
 def build_desc(scene: dict) -> tuple[A.SceneDesc, SceneHolder]:
     """Lower a scene dict to a mupsi_scene_desc (same keys and defaults as Controller::load)."""
     h = SceneHolder()
@@ -181,18 +208,41 @@ def build_desc(scene: dict) -> tuple[A.SceneDesc, SceneHolder]:
     d.gp_mode = A.GP_RENEWAL_PLUS if scene.get("gp_mode") == "renewal_plus" else A.GP_SINGLE_REALIZATION
     d.skip_medium = int(bool(scene.get("skip_medium", False)))
     gpm = scene.get("gp_medium")
+    sdf = scene.get("sdf_medium")
+    ext = scene.get("external_gp_medium")
+    # int_medium spheres take the GP medium, else the SDF medium, else the
+    # external GP medium (controller.cpp:230-234)
+    if gpm is None and sdf is not None:
+        d.has_gp_medium = 1
+        d.medium_kind = A.MEDIUM_SDF
+        _mean_fields(d, h, sdf)
+    elif gpm is None and ext is not None:
+        d.has_gp_medium = 1
+        d.medium_kind = A.MEDIUM_GRID
+        res = int(ext.get("resolution", 48))
+        base = Path(scene.get("config_dir", "."))
+
+        def grid(key_arr, key_file):
+            if ext.get(key_arr) is not None:
+                return h.arr(np.asarray(ext[key_arr], np.float32).reshape(-1), "f")
+            if ext.get(key_file):
+                return h.arr(read_grid(base / ext[key_file], res), "f")
+            return None
+        mean = grid("mean", "mean_file")
+        if mean is None:
+            mean = h.arr(np.zeros(res ** 3, np.float32), "f")
+        var = grid("variance", "variance_file")
+        g = d.grid
+        g.mean = _ptr(mean, C.c_float)
+        g.variance = _ptr(var, C.c_float) if var is not None else None
+        g.resolution = res
+        g.bounds_min[:] = [float(x) for x in ext["bounds_min"]]
+        g.bounds_max[:] = [float(x) for x in ext["bounds_max"]]
+        g.noise_scale = float(ext.get("noise_scale", 1.0))
     if gpm is not None:
         d.has_gp_medium = 1
-        mt = gpm.get("mean_type", "sphere")
-        if mt == "sphere":
-            d.mean_type = A.MEAN_SPHERE
-            d.mean_center[:] = [float(x) for x in gpm["mean_center"]]
-            d.mean_radius = float(gpm.get("mean_radius", 70.0))
-        elif mt == "mesh":
-            d.mean_type = A.MEAN_MESH
-            d.mean_mesh = _mesh_desc(h, gpm.get("mesh", gpm.get("file")), gpm.get("transform"))
-        else:
-            raise ValueError(f"unknown mean_type: {mt}")
+        d.medium_kind = A.MEDIUM_GP
+        _mean_fields(d, h, gpm)
         k = scene.get("kernel", {})
         d.kernel.sigma = float(k.get("sigma", 1.0))
         d.kernel.kernel_radius = float(k.get("length_scale", 1.0))
@@ -237,19 +287,31 @@ def write_scene_files(scene: dict, directory: str | os.PathLike, name: str = "sc
     """Write scene.json (+ OBJ/PFM files for array payloads) in the reference's schema."""
     directory = Path(directory)
     directory.mkdir(parents=True, exist_ok=True)
-    out = {k: v for k, v in scene.items() if k not in ("skydrome", "primitives", "gp_medium")}
+    out = {k: v for k, v in scene.items()
+           if k not in ("skydrome", "primitives", "gp_medium", "sdf_medium", "external_gp_medium")}
     if scene.get("skydrome") is not None:
         sky = scene["skydrome"]
         if not isinstance(sky, (str, os.PathLike)):
             write_pfm(directory / "sky.pfm", sky)
             sky = str(directory / "sky.pfm")
         out["skydrome"] = str(sky)
-    if scene.get("gp_medium") is not None:
-        g = dict(scene["gp_medium"])
-        if "mesh" in g:
-            write_obj(directory / "mean_mesh.obj", g.pop("mesh"))
-            g["file"] = str(directory / "mean_mesh.obj")
-        out["gp_medium"] = g
+    for key in ("gp_medium", "sdf_medium"):
+        if scene.get(key) is not None:
+            g = dict(scene[key])
+            if "mesh" in g:
+                write_obj(directory / f"{key}_mesh.obj", g.pop("mesh"))
+                g["file"] = str(directory / f"{key}_mesh.obj")
+            out[key] = g
+    if scene.get("external_gp_medium") is not None:
+        # grid files are named relative to the scene file's directory
+        # (controller.cpp:177-181), as raw little-endian float32
+        g = dict(scene["external_gp_medium"])
+        for arr_key, file_key in (("mean", "mean_file"), ("variance", "variance_file")):
+            a = g.pop(arr_key, None)
+            if a is not None:
+                np.asarray(a, "<f4").reshape(-1).tofile(directory / f"{arr_key}.raw")
+                g[file_key] = f"{arr_key}.raw"
+        out["external_gp_medium"] = g
     prims = []
     for i, pj in enumerate(scene.get("primitives", [])):
         pj = dict(pj)

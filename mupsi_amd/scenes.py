@@ -182,6 +182,45 @@ def config_scene(cfg: int, *, width: int | None = None, height: int | None = Non
     raise ValueError(f"configuration {cfg} has no scene generator here")
 
 
+def sphere_sdf_grid(res: int, bmin, bmax, center, radius: float) -> np.ndarray:
+    """A GridMeanFunction payload: |p - center| - radius at the cell centres of a
+    res^3 grid over [bmin, bmax], x-major (index (i * res + j) * res + k)."""
+    bmin = np.asarray(bmin, np.float64)
+    bmax = np.asarray(bmax, np.float64)
+    c = [bmin[a] + (np.arange(res) + 0.5) * (bmax[a] - bmin[a]) / res for a in range(3)]
+    X, Y, Z = np.meshgrid(c[0], c[1], c[2], indexing="ij")
+    d = np.sqrt((X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2) - radius
+    return d.astype(np.float32).reshape(-1)
+
+
+def medium_scene(kind: str, *, width: int = 128, height: int = 128, mean: str = "sphere",
+                 noise_variance: float = 25.0, res: int = 48) -> dict:
+    """The configuration-2 layout (enclosing null sphere r130 at z = -500, env
+    map) with an SDFMedium ("sdf") or ExternalGPMedium ("grid") inside instead of
+    the GP medium.  sdf: the mean sphere r70 (or, mean="mesh", a displaced
+    icosphere of radius 70); grid: the sphere's signed distance on a res^3 grid
+    with a constant variance grid (noise_scale 1)."""
+    sc = config_scene(2, width=width, height=height)
+    sc.pop("gp_medium")
+    sc.pop("kernel")
+    sc["spp"] = 16
+    if kind == "sdf":
+        if mean == "sphere":
+            sc["sdf_medium"] = {"mean_type": "sphere", "mean_center": [0, 0, -500], "mean_radius": 70}
+        else:
+            sc["sdf_medium"] = {"mean_type": "mesh", "mesh": icosphere(3, displacement=0.05),
+                                "transform": {"scale": 70.0, "position": [0.0, 0.0, -500.0]}}
+    elif kind == "grid":
+        bmin, bmax = [-130.0, -130.0, -630.0], [130.0, 130.0, -370.0]
+        sc["external_gp_medium"] = {
+            "resolution": res, "bounds_min": bmin, "bounds_max": bmax, "noise_scale": 1.0,
+            "mean": sphere_sdf_grid(res, bmin, bmax, [0, 0, -500], 70.0),
+            "variance": np.full(res ** 3, noise_variance, np.float32)}
+    else:
+        raise ValueError(kind)
+    return sc
+
+
 def emissive_quad(center, side: float) -> dict:
     """Horizontal square of the given side centred at `center`, face normals -y."""
     cx, cy, cz = (float(c) for c in center)

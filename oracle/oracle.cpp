@@ -23,6 +23,8 @@
 //   BVH build/intersect/occluded/closest/inside .............. brute force over triangles
 //   camera, sky lookup, BSDF, sphere ......................... closed forms
 //   trace / render ........................................... furnace / sky-only scenes
+//   SDF / grid media (sdfmedium.cpp, externalgpmedium.cpp) ... analytic sphere hits,
+//       linear fields through the trilinear grid, hash11 vs an integer restatement
 // Exact float bits of Eigen-mediated expressions in the reference binary are
 // parity unpinned (the reference cannot be built here: Eigen is absent).
 
@@ -755,10 +757,20 @@ struct Cond {
   V3 g{0, 0, 0};
 };
 
+// ExternalGPMedium grids (GridMeanFunction, meanfunction.cpp:41-86)
+struct Grid {
+  int res = 0;
+  V3 bmin{0, 0, 0}, inv{0, 0, 0};
+  float noise_scale = 1.0f;
+  std::vector<float> mean, var;
+};
+
 struct Scene {
   Cam cam;
   int max_bounce = 3, gp_mode = 0, skip_medium = 0;
-  bool has_gp = false;
+  bool has_gp = false;       // a medium of medium_kind exists
+  int medium_kind = MUPSI_MEDIUM_GP;
+  Grid grid;
   int mean_type = 0;
   V3 mean_c{0, 0, 0};
   float mean_r = 70;
@@ -898,8 +910,12 @@ static double cell_margin(const GP& g, V3 p) {
 // GPMedium::sampleDistance (mode 0) / transmittance (mode 1)  (gpmedium.cpp:46-129).
 // The FP32 path also runs the reference's discarded lerp/0.9 refinement so the
 // CPU baseline does the same work as the reference.
+static MarchOut sdf_march(const Scene& s, const Ray& ray);
+static MarchOut grid_march(const Scene& s, const Ray& ray, uint32_t seed);
 static MarchOut march(const Scene& s, int mode, const Ray& ray, uint32_t seed, const Cond& c,
                       bool fp64, bool track) {
+  if (s.medium_kind == MUPSI_MEDIUM_SDF) return sdf_march(s, ray);
+  if (s.medium_kind == MUPSI_MEDIUM_GRID) return grid_march(s, ray, seed);
   MarchOut m;
   float nt = ray.nearT, ft = std::min(ray.farT, 2000.0f);
   float dt = std::min(0.5f, (ft - nt) / 100);
@@ -984,6 +1000,145 @@ static void condition_f64(const Scene& s, V3 C, V3 n, uint32_t seed_next, double
   g[0] = k * ((double)n.x - gm.x - ps.gx);
   g[1] = k * ((double)n.y - gm.y - ps.gy);
   g[2] = k * ((double)n.z - gm.z - ps.gz);
+}
+
+// ---------------------------------------------------------------- SDF / grid media
+// GridMeanFunction::eval (meanfunction.cpp:56-76): frac = (p - min) * inv - 0.5,
+// indices clamped to [0, res - 2], weights clamped to [0, 1], data x-major;
+// every a*b + c*d as fmaf(c, d, a*b).
+static float grid_eval(const Grid& g, const std::vector<float>& data, V3 p) {
+  float fx = fmaf(p.x - g.bmin.x, g.inv.x, -0.5f);
+  float fy = fmaf(p.y - g.bmin.y, g.inv.y, -0.5f);
+  float fz = fmaf(p.z - g.bmin.z, g.inv.z, -0.5f);
+  int i0 = std::max(0, std::min(g.res - 2, (int)std::floor(fx)));
+  int j0 = std::max(0, std::min(g.res - 2, (int)std::floor(fy)));
+  int k0 = std::max(0, std::min(g.res - 2, (int)std::floor(fz)));
+  float wx = std::clamp(fx - (float)i0, 0.0f, 1.0f);
+  float wy = std::clamp(fy - (float)j0, 0.0f, 1.0f);
+  float wz = std::clamp(fz - (float)k0, 0.0f, 1.0f);
+  auto at = [&](int i, int j, int k) {
+    return data[(size_t)i * g.res * g.res + (size_t)j * g.res + k];
+  };
+  float c00 = fmaf(at(i0 + 1, j0, k0), wx, at(i0, j0, k0) * (1 - wx));
+  float c01 = fmaf(at(i0 + 1, j0, k0 + 1), wx, at(i0, j0, k0 + 1) * (1 - wx));
+  float c10 = fmaf(at(i0 + 1, j0 + 1, k0), wx, at(i0, j0 + 1, k0) * (1 - wx));
+  float c11 = fmaf(at(i0 + 1, j0 + 1, k0 + 1), wx, at(i0, j0 + 1, k0 + 1) * (1 - wx));
+  float c0 = fmaf(c10, wy, c00 * (1 - wy));
+  float c1 = fmaf(c11, wy, c01 * (1 - wy));
+  return fmaf(c1, wz, c0 * (1 - wz));
+}
+// GridMeanFunction::gradient (meanfunction.cpp:78-84)
+static V3 grid_grad(const Grid& g, const std::vector<float>& data, V3 p) {
+  const float e = 1e-4f;
+  float dx = grid_eval(g, data, mk(p.x + e, p.y, p.z)) - grid_eval(g, data, mk(p.x - e, p.y, p.z));
+  float dy = grid_eval(g, data, mk(p.x, p.y + e, p.z)) - grid_eval(g, data, mk(p.x, p.y - e, p.z));
+  float dz = grid_eval(g, data, mk(p.x, p.y, p.z + e)) - grid_eval(g, data, mk(p.x, p.y, p.z - e));
+  return divs(mk(dx, dy, dz), 2.0f * e);
+}
+// hash11 (externalgpmedium.cpp:8-17)
+static float hash11(float x, float y, float z, uint32_t seed) {
+  uint32_t h = seed, b;
+  std::memcpy(&b, &x, 4);
+  h = h * 0x9e3779b1u + b;
+  std::memcpy(&b, &y, 4);
+  h = h * 0x9e3779b1u + b;
+  std::memcpy(&b, &z, 4);
+  h = h * 0x9e3779b1u + b;
+  h = (h ^ (h >> 16)) * 0x85ebca6bu;
+  h = (h ^ (h >> 13)) * 0xc2b2ae35u;
+  h = h ^ (h >> 16);
+  return (float)(h & 0xFFFFFFu) / (float)0xFFFFFFu * 2.0f - 1.0f;
+}
+// ExternalGPMedium::evalWithGradient (externalgpmedium.cpp:31-58)
+static F4 grid_medium(const Scene& s, V3 p, uint32_t seed) {
+  const Grid& g = s.grid;
+  float mu = grid_eval(g, g.mean, p);
+  V3 mg = grid_grad(g, g.mean, p);
+  float noise = 0.0f;
+  V3 ng{0, 0, 0};
+  if (!g.var.empty() && g.noise_scale > 0.0f) {
+    float var = grid_eval(g, g.var, p);
+    if (var > 0.0f) {
+      float amp = g.noise_scale * std::sqrt(var);
+      noise = amp * hash11(p.x, p.y, p.z, seed);
+      float e = 0.01f;
+      ng.x = amp * (hash11(p.x + e, p.y, p.z, seed) - hash11(p.x - e, p.y, p.z, seed)) / (2.0f * e);
+      ng.y = amp * (hash11(p.x, p.y + e, p.z, seed) - hash11(p.x, p.y - e, p.z, seed)) / (2.0f * e);
+      ng.z = amp * (hash11(p.x, p.y, p.z + e, seed) - hash11(p.x, p.y, p.z - e, seed)) / (2.0f * e);
+    }
+  }
+  return F4{mu + noise, mg.x + ng.x, mg.y + ng.y, mg.z + ng.z};
+}
+static float std_max(float a, float b) { return std::max(a, b); }
+// SDFMedium::sampleDistance / transmittance (sdfmedium.cpp:16-98)
+static MarchOut sdf_march(const Scene& s, const Ray& ray) {
+  MarchOut m;
+  float nt = ray.nearT, ft = std::min(ray.farT, 2000.0f);
+  float t = nt;
+  float fc = mean_eval(s, ray.o);
+  int iter = 0;
+  while (t < ft && iter < 512) {
+    V3 p = madd(ray.d, t, ray.o);
+    float fp = fc, tp = t;
+    fc = mean_eval(s, p);
+    if (fp * fc < 0.0f) {
+      float lo = tp, hi = t, flo = fp;
+      for (int b = 0; b < 16; b++) {
+        float mid = (lo + hi) * 0.5f;
+        float fm = mean_eval(s, madd(ray.d, mid, ray.o));
+        if (flo * fm <= 0.0f) {
+          hi = mid;
+        } else {
+          lo = mid;
+          flo = fm;
+        }
+      }
+      m.hit = true;
+      m.t = hi;
+      m.p = madd(ray.d, hi, ray.o);
+      m.n = normalized(mean_grad(s, m.p));
+      m.steps = iter + 1;
+      return m;
+    }
+    t += std_max(std::fabs(fc), 1e-3f);
+    iter++;
+  }
+  m.steps = iter;
+  return m;
+}
+// ExternalGPMedium::sampleDistance / transmittance (externalgpmedium.cpp:68-130)
+static MarchOut grid_march(const Scene& s, const Ray& ray, uint32_t seed) {
+  MarchOut m;
+  float nt = ray.nearT, ft = std::min(ray.farT, 2000.0f);
+  const float min_step = 0.005f;
+  float t = nt;
+  float f_prev = grid_medium(s, ray.o, seed).v;
+  int iter = 0;
+  for (; iter < 1024 && t < ft; ++iter) {
+    V3 p = madd(ray.d, t, ray.o);
+    float f_c = grid_medium(s, p, seed).v;
+    if (sgn(f_c) != sgn(f_prev) && std::fabs(f_prev - f_c) > 1e-6f) {
+      float lo = t - std_max(std::fabs(f_prev), min_step);
+      float hi = t;
+      for (int b = 0; b < 16; ++b) {
+        float mid = (lo + hi) * 0.5f;
+        float f_mid = grid_medium(s, madd(ray.d, mid, ray.o), seed).v;
+        if (sgn(f_mid) == sgn(f_prev)) lo = mid;
+        else hi = mid;
+      }
+      m.hit = true;
+      m.t = hi;
+      m.p = madd(ray.d, hi, ray.o);
+      F4 f = grid_medium(s, m.p, seed);
+      m.n = normalized(mk(f.gx, f.gy, f.gz));
+      m.steps = iter + 1;
+      return m;
+    }
+    f_prev = f_c;
+    t += std_max(std::fabs(f_c), min_step);
+  }
+  m.steps = iter;
+  return m;
 }
 
 // ---------------------------------------------------------------- primitives
@@ -1275,7 +1430,8 @@ static V3 trace(const Scene& s, int x, int y, int k, uint32_t seed) {
         if (rng.next1D() > surv) break;
         T = divs(T, surv);
         uint32_t nseed = single ? seed : make_seed5(x, y, k, seed, (uint32_t)(bounce + 1));
-        cond = condition_f32(s, P, n, constant_sampler_i(nseed));
+        if (s.medium_kind == MUPSI_MEDIUM_GP)  // only GPMedium re-conditions (trace.cpp:157-160)
+          cond = condition_f32(s, P, n, constant_sampler_i(nseed));
         ray = make_ray(madd(n, s.eps, P), wi);
         ray.nearT = s.near_eps;
         ++bounce;
@@ -1338,11 +1494,26 @@ static Scene* build_scene(const mupsi_scene_desc& d) {
   s->gp_mode = d.gp_mode;
   s->skip_medium = d.skip_medium;
   s->has_gp = d.has_gp_medium != 0;
+  s->medium_kind = s->has_gp ? d.medium_kind : MUPSI_MEDIUM_GP;
   s->mean_type = d.mean_type;
   s->mean_c = mk(d.mean_center[0], d.mean_center[1], d.mean_center[2]);
   s->mean_r = d.mean_radius;
-  if (s->has_gp && d.mean_type == MUPSI_MEAN_MESH) s->mean_mesh.init(d.mean_mesh);
-  s->gp = make_gp(d.kernel);
+  if (s->has_gp && s->medium_kind != MUPSI_MEDIUM_GRID && d.mean_type == MUPSI_MEAN_MESH)
+    s->mean_mesh.init(d.mean_mesh);
+  if (s->medium_kind == MUPSI_MEDIUM_GP) s->gp = make_gp(d.kernel);
+  if (s->has_gp && s->medium_kind == MUPSI_MEDIUM_GRID && d.grid.mean && d.grid.resolution >= 2) {
+    // GridMeanFunction ctor (meanfunction.cpp:41-54): inv cell = res / (max - min)
+    Grid& g = s->grid;
+    g.res = d.grid.resolution;
+    g.bmin = mk(d.grid.bounds_min[0], d.grid.bounds_min[1], d.grid.bounds_min[2]);
+    g.inv = mk((float)g.res / (d.grid.bounds_max[0] - d.grid.bounds_min[0]),
+               (float)g.res / (d.grid.bounds_max[1] - d.grid.bounds_min[1]),
+               (float)g.res / (d.grid.bounds_max[2] - d.grid.bounds_min[2]));
+    g.noise_scale = d.grid.noise_scale;
+    const size_t n = (size_t)g.res * g.res * g.res;
+    g.mean.assign(d.grid.mean, d.grid.mean + n);
+    if (d.grid.variance) g.var.assign(d.grid.variance, d.grid.variance + n);
+  }
   s->sky = make_tex(d.sky);
   s->meshes.reserve(d.n_primitives);
   for (int i = 0; i < d.n_primitives; ++i) {
@@ -1454,6 +1625,10 @@ void orc_psi_eval(const mupsi_gp_params* gp, const float* p3, const uint32_t* se
 void* orc_scene_create(const mupsi_scene_desc* desc) { return build_scene(*desc); }
 void orc_scene_free(void* s) { delete (Scene*)s; }
 
+void orc_hash11(const float* p3, const uint32_t* seed, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = hash11(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2], seed[i]);
+}
+
 void orc_field_eval(void* sc, const float* p3, const uint32_t* seed, const float* cond,
                     int64_t n, int32_t fp64, double* out4) {
   const Scene& s = *(Scene*)sc;
@@ -1461,7 +1636,20 @@ void orc_field_eval(void* sc, const float* p3, const uint32_t* seed, const float
   for (int64_t i = 0; i < n; ++i) {
     V3 p = mk(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
     Cond c = cond_from(cond, i);
-    if (fp64) {
+    if (s.medium_kind != MUPSI_MEDIUM_GP) {  // SDF: the mean; grid medium: grids + noise
+      F4 f = s.medium_kind == MUPSI_MEDIUM_SDF ? F4{mean_eval(s, p), 0, 0, 0}
+                                               : grid_medium(s, p, seed[i]);
+      if (s.medium_kind == MUPSI_MEDIUM_SDF) {
+        V3 g = mean_grad(s, p);
+        f.gx = g.x;
+        f.gy = g.y;
+        f.gz = g.z;
+      }
+      out4[4 * i] = f.v;
+      out4[4 * i + 1] = f.gx;
+      out4[4 * i + 2] = f.gy;
+      out4[4 * i + 3] = f.gz;
+    } else if (fp64) {
       F4d f = field_f64(s, p, seed[i], c);
       out4[4 * i] = f.v;
       out4[4 * i + 1] = f.gx;

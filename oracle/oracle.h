@@ -42,6 +42,8 @@ float orc_psi_var(const mupsi_gp_params* gp);
 /* ---- scene-level ---- */
 void* orc_scene_create(const mupsi_scene_desc* desc);
 void orc_scene_free(void* scene);
+/* hash11 of the external GP medium (externalgpmedium.cpp:8-17), p3: n x 3 */
+void orc_hash11(const float* p3, const uint32_t* seed, int64_t n, float* out);
 
 /* cond: n x 8 floats {active, Cx, Cy, Cz, u_tilde, gx, gy, gz} or NULL (inactive). */
 void orc_field_eval(void* scene, const float* p3, const uint32_t* seed, const float* cond,

@@ -38,6 +38,7 @@ _SIGS = {
     "orc_mesh_closest": (None, [VP, I32, VP, I64, VP, VP, VP]),
     "orc_mesh_inside": (None, [VP, I32, VP, I64, VP]),
     "orc_mesh_query_cost": (None, [VP, I32, I32, VP, I64, VP, VP]),
+    "orc_hash11": (None, [VP, VP, I64, VP]),
     "orc_camera_rays": (None, [VP, VP, VP, I64, VP]),
     "orc_sky_lookup": (None, [VP, VP, I64, VP, VP]),
     "orc_trace_paths": (None, [VP, VP, VP, VP, I64, VP, I32]),
@@ -269,6 +270,15 @@ class OracleScene:
         secs = lib().orc_render(self.h, k_begin, k_count, pixel_stride, pixel_offset, nthreads,
                                 _p(sums), C.byref(paths))
         return sums, paths.value, secs
+
+
+def hash11(p3: np.ndarray, seeds: np.ndarray) -> np.ndarray:
+    """hash11 of the external GP medium at points p3 (n, 3) with seeds (n,)."""
+    p3 = np.ascontiguousarray(p3, np.float32).reshape(-1, 3)
+    seeds = np.ascontiguousarray(seeds, np.uint32)
+    out = np.zeros(p3.shape[0], np.float32)
+    lib().orc_hash11(_p(p3), _p(seeds), p3.shape[0], _p(out))
+    return out
 
 
 def bvh_build(vertices: np.ndarray, faces: np.ndarray):
