@@ -496,13 +496,15 @@ struct SkipInfo {
   float wmin[3], wmax[3];  // mesh: root box in world space (s > 0)
 };
 
+// kMesh false: the scene's mean is known to be a sphere.
+template <bool kMesh = true>
 __device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, F3 o, float ft) {
   SkipInfo k;
   k.kind = 0;
   k.outer2 = INFINITY;
   k.inner2 = -1.0f;
   const float B = S.gp.bound + cond_bound(S.gp, c);
-  if (S.mean_type == MUPSI_MEAN_SPHERE) {
+  if (!kMesh || S.mean_type == MUPSI_MEAN_SPHERE) {
     const F3 oc = sub(o, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
     const float m = 0.01f + 2e-6f * (norm(oc) + ft + S.mean_r + B);
     const float ro = S.mean_r + B + m;
@@ -512,6 +514,7 @@ __device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, 
     k.kind = 1;
     return k;
   }
+  if constexpr (!kMesh) return k;
   const DevMesh& M = S.mean_mesh;
   if (M.n_nodes == 0 || !(M.xf.s > 0.0f)) return k;
   for (int a = 0; a < 3; ++a) {
@@ -872,10 +875,9 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
   }
 }
 
-// Cached-window marcher (see march_cached.h): same decisions as k_march, two rays
-// per 64-lane workgroup, impulses regenerated once per cell entering the window.
-// Used for sphere means with points_per_cell <= 8.
-// f at q for the cached marchers, given the window's psi sum acc: the mean is
+// f at q for the window marchers, given psi_t = psi_scale * (the window's impulse
+// sum; the product replaces the reference's sigma * sum / sqrt(Var), a difference
+// of an ulp that no unambiguous decision sees): the mean is
 // evaluated exactly unless, for a mesh mean, bounds of the distance decide the
 // sign: with mu = s d, d in [lo, hi] and s from inside() (called only where the
 // parity ray can reach the mesh), f lies in [s lo, s hi] + (psi term + conditioning
@@ -884,9 +886,8 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
 // decision the marcher takes from it (sign, and |f_prev - f| > 1e-6 across a
 // sign change) is the one the exact value gives.
 template <bool kMesh = true>
-__device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G, F3 q, float acc,
+__device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G, F3 q, float psi_t,
                                               const Cond& c, int ins = -1) {
-  const float psi_t = G.sigma * acc / G.sqrt_var;
   float cond_t = 0.0f;
   if (c.active) {
     const F3 r = sub(q, c.C);
@@ -924,16 +925,32 @@ __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G,
       return f;
     }
   }
-  float f = mean_eval<false>(S, q, nullptr) + psi_t;
+  // the sphere mean of mean_eval, without the mesh code where no mesh can occur
+  const float mu = kMesh ? mean_eval<false>(S, q, nullptr)
+                         : norm(sub(q, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]))) - S.mean_r;
+  float f = mu + psi_t;
   if (c.active) f = f + cond_t;
   return f;
 }
 
 struct CachedConsts {
-  float inv_kr;   // 1 / kr (rounded)
-  int kr_pow2;    // kr is a power of two: q * inv_kr == q / kr exactly
-  float psi_k;    // -log2(e) / L^2: h = exp2(|p - s|^2 * psi_k)
+  float inv_kr;     // 1 / kr (rounded)
+  int kr_pow2;      // kr is a power of two: q * inv_kr == q / kr exactly
+  float psi_k;      // -log2(e) / L^2: h = exp2(|p - s|^2 * psi_k)
+  float psi_scale;  // sigma / sqrt(Var): psi = psi_scale * (impulse sum)
 };
+
+// Sum over the 32 lanes of a group by an xor butterfly (16, 8, 4, 2, 1), every
+// lane ending with the same total: ds_swizzle in bit mode (and 0x1f, xor k)
+// exchanges within 32-lane groups without address arithmetic.
+__device__ __forceinline__ float group_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x201F));
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x081F));
+  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x041F));
+  return v;
+}
 
 // This lane's part of the window sum: slots gl, gl + 32, ... of the 27 x PPC.
 template <int PPC>
@@ -1060,7 +1077,7 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
               c.u = a.cu[rid];
               c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
             }
-            sk = make_skip(scene(), c, o, ft);
+            sk = make_skip<kMesh>(scene(), c, o, ft);
             nsteps = 0;
             wvalid = false;  // new realization seed
             gi0 = -1;
@@ -1164,10 +1181,10 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
         acc = window_sum<PPC>(W, q, gl, cc.psi_k);
       }
     }
-    for (int off = kGroup / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    acc = group_sum(acc);
 
     if (do_eval) {
-      const float f = cached_field<kMesh>(scene(), G, q, acc, c, qins);
+      const float f = cached_field<kMesh>(scene(), G, q, acc * cc.psi_scale, c, qins);
       if (gl == 0) ++evals;
       if (phase == 1) {
         fprev = f;
@@ -2439,6 +2456,7 @@ static CachedConsts cached_consts(const DevGP& g) {
   int e = 0;
   cc.kr_pow2 = std::frexp(g.kr, &e) == 0.5f ? 1 : 0;
   cc.psi_k = (float)(-1.4426950408889634 / (double)g.L2);
+  cc.psi_scale = (float)((double)g.sigma / (double)g.sqrt_var);
   return cc;
 }
 
