@@ -314,7 +314,7 @@ __device__ __forceinline__ void sphere_local(const DevPrim& pr, const Ray& r, F3
 }
 
 // Scene::intersect (scene.cpp:25-39) + intersectInfo of the closest primitive.
-__device__ int scene_intersect(const DevScene& S, Ray& ray, F3& P, F3& Ng, float& uu, float& vv) {
+__device__ __forceinline__ int scene_intersect(const DevScene& S, Ray& ray, F3& P, F3& Ng, float& uu, float& vv) {
   float mint = 3.402823466e+38f;
   int hitp = -1;
   uint32_t hit_tri = 0;
@@ -383,7 +383,7 @@ __device__ int scene_intersect(const DevScene& S, Ray& ray, F3& P, F3& Ng, float
 }
 
 // Scene::occluded (scene.cpp:41-48)
-__device__ bool scene_occluded(const DevScene& S, const Ray& r) {
+__device__ __forceinline__ bool scene_occluded(const DevScene& S, const Ray& r) {
   for (int i = 0; i < S.n_prims; ++i) {
     const DevPrim& pr = S.prims[i];
     if (pr.type == MUPSI_PRIM_SPHERE) {
@@ -469,7 +469,7 @@ __device__ bool mesh_sample_direct(const DevPrim& pr, const DevMesh& m, F3 p, Pc
 }
 
 // Scene::chooseLight (scene.cpp:50-61): no draw when the scene has no emitter.
-__device__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls) {
+__device__ __forceinline__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls) {
   if (S.n_prims == 0 || S.n_lights == 0) return false;
   const int idx = (int)(rng.nextI() % (uint32_t)S.n_lights);
   const DevPrim& pr = S.prims[S.lights[idx]];
@@ -593,7 +593,7 @@ __device__ __forceinline__ float std_max(float a, float b) { return a < b ? b : 
 // change (f_prev * f < 0) is refined by 16 bisection steps and the hit is hi.
 // iters: steps taken.  Transmittance takes the same decision without the
 // bisection.
-__device__ bool sdf_march(const DevScene& S, F3 o, F3 d, float nt, float ft, float& t_hit,
+__device__ __forceinline__ bool sdf_march(const DevScene& S, F3 o, F3 d, float nt, float ft, float& t_hit,
                           F3& p_hit, int& iters) {
   float t = nt;
   float fc = mean_eval<false>(S, o, nullptr);
@@ -630,7 +630,7 @@ __device__ bool sdf_march(const DevScene& S, F3 o, F3 d, float nt, float ft, flo
 // the same stepping with min step 0.005 and at most 1024 steps, GPMedium's sign
 // test (sign(x) = x > 0 and |f_prev - f| > 1e-6), bisection from
 // t - max(|f_prev|, 0.005) to t keeping f_prev's sign at lo.
-__device__ bool grid_march(const DevScene& S, F3 o, F3 d, float nt, float ft, uint32_t seed,
+__device__ __forceinline__ bool grid_march(const DevScene& S, F3 o, F3 d, float nt, float ft, uint32_t seed,
                            float& t_hit, F3& p_hit, int& iters) {
   const float min_step = 0.005f;
   float t = nt;
@@ -660,16 +660,26 @@ __device__ bool grid_march(const DevScene& S, F3 o, F3 d, float nt, float ft, ui
   return false;
 }
 
-// The march of a medium other than the GP medium (hit flag, t, point).
+// The march of a medium other than the GP medium (hit flag, t, point).  Every
+// function taking the scene by reference is inlined: an out-of-line call needs
+// the scene's address, and the kernel would copy its scene argument to scratch.
 __device__ __forceinline__ bool alt_march(const DevScene& S, F3 o, F3 d, float nt, float ft,
-                                          uint32_t seed, float& t_hit, F3& p_hit, int& iters) {
+                                       uint32_t seed, float& t_hit, F3& p_hit, int& iters) {
   if (S.medium_kind == MUPSI_MEDIUM_SDF) return sdf_march(S, o, d, nt, ft, t_hit, p_hit, iters);
   return grid_march(S, o, d, nt, ft, seed, t_hit, p_hit, iters);
+}
+// Gradient of a medium other than the GP medium at p.
+__device__ __forceinline__ F3 alt_grad(const DevScene& S, F3 p, uint32_t seed) {
+  F3 g = mk(0, 0, 0);
+  if (S.medium_kind == MUPSI_MEDIUM_SDF) (void)mean_eval<true>(S, p, &g);
+  else (void)grid_medium_eval<true>(S, p, seed, &g);
+  return g;
 }
 
 // Serial march of one ray (used for NEE shadow rays in k_shade): the same
 // decisions as k_march without the certain-sign skip.
-__device__ bool march_serial(const DevScene& S, const Ray& ray, uint32_t seed, const Cond& c) {
+__device__ __forceinline__ bool march_serial(const DevScene& S, const Ray& ray, uint32_t seed,
+                                             const Cond& c) {
   if (S.medium_kind != MUPSI_MEDIUM_GP) {
     float th;
     F3 ph;
@@ -1224,8 +1234,9 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
 // with psi the next realization.  At r = 0 the matrix is diag(1, -2/L^2 I),
 // which gives the reference's closed form u = -mu/sigma - psi,
 // g = (-L^2/2)(n - grad mu - grad psi).
-__device__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next, double& u,
-                              double g[3], const MeanAt* mean = nullptr) {
+__device__ __forceinline__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next,
+                                              double& u, double g[3],
+                                              const MeanAt* mean = nullptr) {
   F3 pg;
   float pv;
   psi_eval<true>(S.gp, C, seed_next, pv, pg, nullptr);
@@ -1435,7 +1446,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
       mp.mu = mean_eval<true>(S, Pp, &mp.grad);
       field_eval<true>(S, Pp, iseed, cur, &gr, &mp);
     } else {
-      medium_field<true>(S, Pp, iseed, cur, &gr);
+      gr = alt_grad(S, Pp, iseed);
     }
     const F3 nrm = normalized(gr);
     const F3 wo = neg(ray.d);
@@ -1725,7 +1736,8 @@ __global__ __launch_bounds__(256) void k_guide_prep(DevScene S, GuidePrepArgs a)
   }
   const F3 Pp = mk(P.mpx[pid], P.mpy[pid], P.mpz[pid]);
   F3 gr;
-  medium_field<true>(S, Pp, P.mseed[pid], cur, &gr);
+  if (S.medium_kind == MUPSI_MEDIUM_GP) field_eval<true>(S, Pp, P.mseed[pid], cur, &gr);
+  else gr = alt_grad(S, Pp, P.mseed[pid]);
   const F3 nrm = normalized(gr);
   float f[guide::kIn];
   guide::encode_features(a.box, Pp.x, Pp.y, Pp.z, nrm.x, nrm.y, nrm.z, -P.dx[pid], -P.dy[pid],
