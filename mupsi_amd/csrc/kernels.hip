@@ -953,21 +953,25 @@ struct CachedConsts {
 // Sum over the 32 lanes of a group by an xor butterfly (16, 8, 4, 2, 1), every
 // lane ending with the same total: ds_swizzle in bit mode (and 0x1f, xor k)
 // exchanges within 32-lane groups without address arithmetic.
+// Smaller groups start the butterfly at xor GS / 2.
+template <int GS = 32>
 __device__ __forceinline__ float group_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
-  v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x201F));
+  if constexpr (GS >= 32)
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+  if constexpr (GS >= 16)
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x201F));
   v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
   v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x081F));
   v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x041F));
   return v;
 }
 
-// This lane's part of the window sum: slots gl, gl + 32, ... of the 27 x PPC.
-template <int PPC>
+// This lane's part of the window sum: slots gl, gl + GS, ... of the 27 x PPC.
+template <int PPC, int GS = kGroup>
 __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float psi_k) {
   float acc = 0.0f;
 #pragma unroll
-  for (int sl = gl; sl < 27 * PPC; sl += kGroup) {
+  for (int sl = gl; sl < 27 * PPC; sl += GS) {
     const float4 im = W[sl];
     const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
     const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
@@ -987,15 +991,17 @@ __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float
 //   kMesh    mean meshes possible: inside() at the next 32 grid points of a ray
 //            is evaluated at once, one point per lane, and kept as a bit mask
 //   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
-template <int PPC, bool kMesh, bool kMulti>
+template <int PPC, bool kMesh, bool kMulti, int GS = kGroup>
 __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
                                                      CachedConsts cc0,
                                                      const CachedConsts* ccs) {
-  __shared__ float4 win[2][27 * PPC];
+  static_assert(GS == 8 || GS == 16 || GS == 32, "a ray's group is 8, 16 or 32 lanes");
+  constexpr uint64_t kGroupMask = (1ull << GS) - 1ull;
+  __shared__ float4 win[64 / GS][27 * PPC];
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
-  const int grp = lane >> 5;
-  const int gl = lane & (kGroup - 1);
+  const int grp = lane / GS;
+  const int gl = lane & (GS - 1);
   float4* W = win[grp];
   // the group's scene (batched launches: that of its current ray)
   int sidx = 0;
@@ -1025,13 +1031,13 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
   int gi = 0, gi0 = -1;
   uint32_t imask = 0;
   const auto inside_here = [&]() -> int {
-    if (gi0 < 0 || gi - gi0 >= kGroup) {
+    if (gi0 < 0 || gi - gi0 >= GS) {
       float tl = t;  // the march's own f32 accumulation of t, gl steps ahead
       for (int i = 0; i < gl; ++i) tl += dt;
       const F3 pp = madd(d, tl, o);
       const DevMesh& M = scene().mean_mesh;
       const bool in = mesh_in_parity_slab(M, xf_to_local(M.xf, pp)) && mesh_inside(M, pp);
-      imask = (uint32_t)(__ballot(in) >> (grp * kGroup));
+      imask = (uint32_t)((__ballot(in) >> (grp * GS)) & kGroupMask);
       gi0 = gi;
     }
     return (int)((imask >> (gi - gi0)) & 1u);
@@ -1066,7 +1072,7 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
         if (lane == leader) base = atomicAdd(a.work, __popcll(m));
         base = __shfl(base, leader);
         if (need) {
-          const int qi = base + __popcll(m & ((1ull << (grp * kGroup)) - 1ull));
+          const int qi = base + __popcll(m & ((1ull << (grp * GS)) - 1ull));
           if (qi < n) {
             rid = a.queue ? a.queue[qi] : qi;
             if constexpr (kMulti) {
@@ -1156,7 +1162,7 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
     if (newcells >= 0) {
       const int items = kNewCells.n[newcells] * ppc;
       const int mx = mod3(cx), my = mod3(cy), mz = mod3(cz);  // residues of the window centre
-      for (int it = gl; it < items; it += kGroup) {
+      for (int it = gl; it < items; it += GS) {
         const int k = it / ppc;
         const int i = it - k * ppc;
         const int ni = kNewCells.idx[newcells][k];
@@ -1178,20 +1184,20 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
     if (do_eval) {
       if constexpr (kMulti) {
         switch (ppc) {  // the compile-time slot count, per ray
-          case 1: acc = window_sum<1>(W, q, gl, cc.psi_k); break;
-          case 2: acc = window_sum<2>(W, q, gl, cc.psi_k); break;
-          case 3: acc = window_sum<3>(W, q, gl, cc.psi_k); break;
-          case 4: acc = window_sum<4>(W, q, gl, cc.psi_k); break;
-          case 5: acc = window_sum<5>(W, q, gl, cc.psi_k); break;
-          case 6: acc = window_sum<6>(W, q, gl, cc.psi_k); break;
-          case 7: acc = window_sum<7>(W, q, gl, cc.psi_k); break;
-          default: acc = window_sum<8>(W, q, gl, cc.psi_k); break;
+          case 1: acc = window_sum<1, GS>(W, q, gl, cc.psi_k); break;
+          case 2: acc = window_sum<2, GS>(W, q, gl, cc.psi_k); break;
+          case 3: acc = window_sum<3, GS>(W, q, gl, cc.psi_k); break;
+          case 4: acc = window_sum<4, GS>(W, q, gl, cc.psi_k); break;
+          case 5: acc = window_sum<5, GS>(W, q, gl, cc.psi_k); break;
+          case 6: acc = window_sum<6, GS>(W, q, gl, cc.psi_k); break;
+          case 7: acc = window_sum<7, GS>(W, q, gl, cc.psi_k); break;
+          default: acc = window_sum<8, GS>(W, q, gl, cc.psi_k); break;
         }
       } else {
-        acc = window_sum<PPC>(W, q, gl, cc.psi_k);
+        acc = window_sum<PPC, GS>(W, q, gl, cc.psi_k);
       }
     }
-    acc = group_sum(acc);
+    acc = group_sum<GS>(acc);
 
     if (do_eval) {
       const float f = cached_field<kMesh>(scene(), G, q, acc * cc.psi_scale, c, qins);
@@ -2118,7 +2124,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false>, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false, kSphereGroup>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   *out = ctx.release();
   return MUPSI_OK;
@@ -2483,8 +2489,8 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
       hipLaunchKernelGGL((k_march_window<P, true, false>), grid, block, 0, st, S, ma, cc,     \
                          nullptr);                                                          \
     else                                                                                    \
-      hipLaunchKernelGGL((k_march_window<P, false, false>), grid, block, 0, st, S, ma, cc,    \
-                         nullptr);                                                          \
+      hipLaunchKernelGGL((k_march_window<P, false, false, kSphereGroup>), grid, block, 0, st, \
+                         S, ma, cc, nullptr);                                               \
     break;
     switch (S.gp.ppc) {
       MUPSI_CACHED(1)
@@ -2909,8 +2915,8 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.queue = pb.qM[cur];
         ma.count_dev = nM;
         ma.work = work;
-        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true>), wgrid, wblock, 0, st,
-                           DevScene{}, ma, CachedConsts{}, b->d_cc);
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true, kSphereGroup>), wgrid,
+                           wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
       }
       if (b->any_cached_mesh) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work3, 1, 0, -1);

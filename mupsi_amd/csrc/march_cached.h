@@ -25,6 +25,7 @@ namespace mupsi {
 namespace dev {
 
 constexpr int kGroup = 32;        // lanes per ray
+constexpr int kSphereGroup = 16;  // lanes per ray of the sphere-mean instantiations
 constexpr int kMaxPpcCached = 8;  // window capacity: 27 * 8 impulses
 
 struct LcgJump {
