@@ -2486,8 +2486,8 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
 #define MUPSI_CACHED(P)                                                                     \
   case P:                                                                                   \
     if (mesh)                                                                               \
-      hipLaunchKernelGGL((k_march_window<P, true, false>), grid, block, 0, st, S, ma, cc,     \
-                         nullptr);                                                          \
+      hipLaunchKernelGGL((k_march_window<P, true, false, kMeshGroup>), grid, block, 0, st, S, \
+                         ma, cc, nullptr);                                                  \
     else                                                                                    \
       hipLaunchKernelGGL((k_march_window<P, false, false, kSphereGroup>), grid, block, 0, st, \
                          S, ma, cc, nullptr);                                               \
@@ -2923,8 +2923,8 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.queue = pb.qMs[cur];
         ma.count_dev = nMs;
         ma.work = work3;
-        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true>), wgrid, wblock, 0, st,
-                           DevScene{}, ma, CachedConsts{}, b->d_cc);
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true, kMeshGroup>), wgrid,
+                           wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
       }
       if (b->any_plain) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work2, 1, 0, -1);

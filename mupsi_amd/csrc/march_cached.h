@@ -26,6 +26,7 @@ namespace dev {
 
 constexpr int kGroup = 32;        // lanes per ray
 constexpr int kSphereGroup = 16;  // lanes per ray of the sphere-mean instantiations
+constexpr int kMeshGroup = 16;    // lanes per ray of the mesh-mean instantiations
 constexpr int kMaxPpcCached = 8;  // window capacity: 27 * 8 impulses
 
 struct LcgJump {
