@@ -2124,7 +2124,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false, kSphereGroup>, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false, sphere_group(3)>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   *out = ctx.release();
   return MUPSI_OK;
@@ -2486,11 +2486,11 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
 #define MUPSI_CACHED(P)                                                                     \
   case P:                                                                                   \
     if (mesh)                                                                               \
-      hipLaunchKernelGGL((k_march_window<P, true, false, kMeshGroup>), grid, block, 0, st, S, \
-                         ma, cc, nullptr);                                                  \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_march_window<P, false, false, kSphereGroup>), grid, block, 0, st, \
+      hipLaunchKernelGGL((k_march_window<P, true, false, mesh_group(P)>), grid, block, 0, st, \
                          S, ma, cc, nullptr);                                               \
+    else                                                                                    \
+      hipLaunchKernelGGL((k_march_window<P, false, false, sphere_group(P)>), grid, block, 0,   \
+                         st, S, ma, cc, nullptr);                                           \
     break;
     switch (S.gp.ppc) {
       MUPSI_CACHED(1)
@@ -2915,7 +2915,7 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.queue = pb.qM[cur];
         ma.count_dev = nM;
         ma.work = work;
-        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true, kSphereGroup>), wgrid,
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true, kMultiGroup>), wgrid,
                            wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
       }
       if (b->any_cached_mesh) {
@@ -2923,7 +2923,7 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.queue = pb.qMs[cur];
         ma.count_dev = nMs;
         ma.work = work3;
-        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true, kMeshGroup>), wgrid,
+        hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true, kMultiGroup>), wgrid,
                            wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
       }
       if (b->any_plain) {

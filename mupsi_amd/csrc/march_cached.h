@@ -25,8 +25,14 @@ namespace mupsi {
 namespace dev {
 
 constexpr int kGroup = 32;        // lanes per ray
-constexpr int kSphereGroup = 16;  // lanes per ray of the sphere-mean instantiations
-constexpr int kMeshGroup = 16;    // lanes per ray of the mesh-mean instantiations
+// Lanes per ray of the per-scene instantiations.  Fewer lanes per ray put more
+// rays in a wave (less per-ray control work, more BVH loads in flight for mesh
+// means) but more window slots on each lane; measured best: sphere means 16
+// lanes at ppc 3 (8 lanes 10% slower), mesh means 8 lanes at ppc 4 and 16 lanes
+// for the larger windows of the generated scenes.
+constexpr int sphere_group(int ppc) { return ppc <= 2 ? 8 : 16; }
+constexpr int mesh_group(int ppc) { return ppc <= 4 ? 8 : 16; }
+constexpr int kMultiGroup = 16;   // batched launches (ppc per ray, LDS for 8)
 constexpr int kMaxPpcCached = 8;  // window capacity: 27 * 8 impulses
 
 struct LcgJump {
