@@ -886,8 +886,10 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
 }
 
 // f at q for the window marchers, given psi_t = psi_scale * (the window's impulse
-// sum; the product replaces the reference's sigma * sum / sqrt(Var), a difference
-// of an ulp that no unambiguous decision sees): the mean is
+// sum).  The product replaces the reference's sigma * sum / sqrt(Var), the
+// hardware exp2 and sqrt the correctly rounded expf and sqrtf of the
+// conditioning kernel and the sphere mean: differences of an ulp or two, which
+// no unambiguous decision sees (the psi sum's order differs anyway).  The mean is
 // evaluated exactly unless, for a mesh mean, bounds of the distance decide the
 // sign: with mu = s d, d in [lo, hi] and s from inside() (called only where the
 // parity ray can reach the mesh), f lies in [s lo, s hi] + (psi term + conditioning
@@ -897,11 +899,11 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
 // sign change) is the one the exact value gives.
 template <bool kMesh = true>
 __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G, F3 q, float psi_t,
-                                              const Cond& c, int ins = -1) {
+                                              const Cond& c, float psi_k, int ins = -1) {
   float cond_t = 0.0f;
-  if (c.active) {
+  if (c.active) {  // h(C, q) = exp2(|q - C|^2 psi_k), the impulses' hardware exp2
     const F3 r = sub(q, c.C);
-    const float h = kern_h(G, c.C, q);
+    const float h = __builtin_amdgcn_exp2f(sq3(r) * psi_k);
     cond_t = fmaf(c.u, h, dot(c.g, mul(r, h * G.nk2)));
   }
   if (kMesh && S.mean_type == MUPSI_MEAN_MESH && S.mean_mesh.xf.s > 0.0f) {
@@ -936,8 +938,9 @@ __device__ __forceinline__ float cached_field(const DevScene& S, const DevGP& G,
     }
   }
   // the sphere mean of mean_eval, without the mesh code where no mesh can occur
-  const float mu = kMesh ? mean_eval<false>(S, q, nullptr)
-                         : norm(sub(q, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]))) - S.mean_r;
+  const float mu =
+      kMesh ? mean_eval<false>(S, q, nullptr)
+            : __builtin_amdgcn_sqrtf(sq3(sub(q, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2])))) - S.mean_r;
   float f = mu + psi_t;
   if (c.active) f = f + cond_t;
   return f;
@@ -1200,7 +1203,7 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
     acc = group_sum<GS>(acc);
 
     if (do_eval) {
-      const float f = cached_field<kMesh>(scene(), G, q, acc * cc.psi_scale, c, qins);
+      const float f = cached_field<kMesh>(scene(), G, q, acc * cc.psi_scale, c, cc.psi_k, qins);
       if (gl == 0) ++evals;
       if (phase == 1) {
         fprev = f;
