@@ -187,6 +187,22 @@ mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_desc* desc, mu
 mupsi_status mupsi_scene_load_json(mupsi_ctx* ctx, const char* path, mupsi_scene** out, int32_t* spp_out);
 void mupsi_scene_free(mupsi_scene* scene);
 mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int32_t* height);
+/* Controller::applyGpMedium (controller.cpp:397-426): give every int_medium
+ * primitive a new GPMedium over the mean (MUPSI_MEAN_SPHERE: mean_center,
+ * mean_radius; MUPSI_MEAN_MESH: mean_mesh) with SparseSEKernel(kernel).  Only
+ * for scenes whose medium is a gp_medium; the sums are kept (reset them as
+ * Renderer::prepareRender does); refused while the scene is in a live batch. */
+mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t mean_type,
+                                       const float* mean_center, float mean_radius,
+                                       const mupsi_mesh_desc* mean_mesh,
+                                       const mupsi_gp_params* kernel);
+/* The RenderContext snapshot (render_context.h:15-19) of g_pathTracerSettings
+ * and g_gpSettings: max_bounce, gp_mode (MUPSI_GP_*) and skip_medium, as
+ * Controller::setGpMode (controller.cpp:389-395) and
+ * Controller::startProgressive (controller.cpp:282-289) change them.  Refused
+ * while the scene is in a live batch. */
+mupsi_status mupsi_scene_set_settings(mupsi_scene* scene, int32_t max_bounce, int32_t gp_mode,
+                                      int32_t skip_medium);
 
 /* Render spp passes k_begin .. k_begin+k_count-1 of every pixel and add the
  * path radiance to the FP64 per-pixel sums.  abort_flag (host memory, may be

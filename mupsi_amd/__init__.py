@@ -107,6 +107,35 @@ class Scene:
     def reset(self) -> None:
         check(self.lib.mupsi_reset_accum(self.handle), "mupsi_reset_accum")
 
+    def set_gp_medium(self, mean_type: str = "sphere", mean_center=(0.0, 0.0, 0.0),
+                      mean_radius: float = 70.0, mesh=None, transform: dict | None = None,
+                      kernel: dict | None = None) -> None:
+        """A new GPMedium for the scene's int_medium primitives (mupsi_scene_set_gp_medium,
+        Controller::applyGpMedium): mean "sphere" (mean_center, mean_radius) or "mesh"
+        (an OBJ path or mesh dict placed by transform {position, scale}); kernel has the
+        scene.json kernel keys (sigma, length_scale, aniso, points_per_cell)."""
+        from .batch import gp_params
+        from .scene import SceneHolder, _mesh_desc
+        h = SceneHolder()
+        mt = {"sphere": _abi.MEAN_SPHERE, "mesh": _abi.MEAN_MESH}.get(mean_type)
+        if mt is None:
+            raise ValueError(f"unknown mean_type: {mean_type}")
+        md = _mesh_desc(h, mesh, transform) if mt == _abi.MEAN_MESH else None
+        centre = (C.c_float * 3)(*[float(x) for x in mean_center])
+        gp = gp_params(kernel or {})
+        check(self.lib.mupsi_scene_set_gp_medium(self.handle, mt, C.cast(centre, C.c_void_p),
+                                                 float(mean_radius),
+                                                 C.byref(md) if md is not None else None,
+                                                 C.byref(gp)), "mupsi_scene_set_gp_medium")
+        del h
+
+    def set_settings(self, max_bounce: int, gp_mode: str, skip_medium: bool) -> None:
+        """The RenderContext snapshot of the path tracer and GP settings
+        (mupsi_scene_set_settings): gp_mode "renewal_plus" or "single_realization"."""
+        mode = _abi.GP_RENEWAL_PLUS if gp_mode == "renewal_plus" else _abi.GP_SINGLE_REALIZATION
+        check(self.lib.mupsi_scene_set_settings(self.handle, int(max_bounce), mode,
+                                                int(bool(skip_medium))), "mupsi_scene_set_settings")
+
     def read_accum(self) -> tuple[np.ndarray, int]:
         buf = np.zeros((self.height, self.width, 3), np.float64)
         cnt = C.c_int64()
@@ -323,8 +352,14 @@ class Renderer:
         self._done = 0
         self._target = 0
 
-    def set_cancel_flag(self, flag: threading.Event | None = None) -> None:
-        self._cancel_event = flag
+    def set_cancel_flag(self, flag: "C.c_int32 | threading.Event | None" = None) -> None:
+        """Renderer::setCancelFlag: a shared C.c_int32, which the library also polls
+        between bounce iterations, or a threading.Event polled between pass groups."""
+        if isinstance(flag, C.c_int32):
+            self._cancel = flag
+            self._cancel_event = None
+        else:
+            self._cancel_event = flag
 
     def cancel(self) -> None:
         self._cancel.value = 1
@@ -364,3 +399,8 @@ class Renderer:
 
     def framebuffer(self) -> np.ndarray:
         return self.scene.image()
+
+
+from .controller import Controller  # noqa: E402  (needs the classes above)
+
+__all__.append("Controller")

@@ -120,6 +120,9 @@ SIGNATURES = {
     "mupsi_scene_load_json": (I32, [VP, C.c_char_p, C.POINTER(VP), C.POINTER(I32)]),
     "mupsi_scene_free": (None, [VP]),
     "mupsi_scene_size": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
+    "mupsi_scene_set_gp_medium": (I32, [VP, I32, VP, C.c_float, C.POINTER(MeshDesc),
+                                        C.POINTER(GpParams)]),
+    "mupsi_scene_set_settings": (I32, [VP, I32, I32, I32]),
     "mupsi_render_passes": (I32, [VP, I32, I32, I32, VP]),
     "mupsi_reset_accum": (I32, [VP]),
     "mupsi_read_accum": (I32, [VP, VP, C.POINTER(I64)]),

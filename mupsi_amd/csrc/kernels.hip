@@ -2136,6 +2136,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
 static void destroy_scene(mupsi_scene* scene);
 static void destroy_batch(mupsi_batch* b);
 static void batch_forget_scene(mupsi_ctx* ctx, const mupsi_scene* scene);
+static bool scene_in_live_batch(const mupsi_ctx* ctx, const mupsi_scene* scene);
 
 // Frees the context and every scene still open on it (their handles become invalid).
 extern "C" void mupsi_close(mupsi_ctx* ctx) {
@@ -2210,6 +2211,7 @@ struct mupsi_scene {
   HostScene host;  // host-side lowered description (meshes keep the BVH for the batch API)
   std::vector<DevMesh> meshes;
   DevMesh mean_mesh{};
+  std::unique_ptr<DeviceBuffers> mean_mem;  // mean mesh + distance grid (replaceable)
   double* accum = nullptr;
   int64_t sample_count = 0;
   std::unique_ptr<PathBuffers> pb;
@@ -2307,10 +2309,11 @@ extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_des
     if (st != MUPSI_OK) return st;
     sc->meshes.push_back(m);
   }
+  sc->mean_mem = std::make_unique<DeviceBuffers>();
   if (H.has_mean_mesh) {
-    st = upload_mesh(sc->mem, H.mean_mesh, &sc->mean_mesh);
+    st = upload_mesh(*sc->mean_mem, H.mean_mesh, &sc->mean_mesh);
     if (st != MUPSI_OK) return st;
-    st = build_dist_grid(sc->mem, &sc->mean_mesh);
+    st = build_dist_grid(*sc->mean_mem, &sc->mean_mesh);
     if (st != MUPSI_OK) return st;
   }
   D.mean_mesh = sc->mean_mesh;
@@ -2358,6 +2361,72 @@ extern "C" void mupsi_scene_free(mupsi_scene* scene) {
   batch_forget_scene(ctx, scene);
   (void)hipSetDevice(ctx->device);
   destroy_scene(scene);
+}
+
+// Controller::applyGpMedium (controller.cpp:397-426) on a live scene: a new
+// GPMedium for every int_medium primitive.  The sums are kept (the caller resets
+// them as Renderer::prepareRender does); batches hold their own copy of the
+// scene, so a scene in a live batch is refused.
+extern "C" mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t mean_type,
+                                                  const float* mean_center, float mean_radius,
+                                                  const mupsi_mesh_desc* mean_mesh,
+                                                  const mupsi_gp_params* kernel) {
+  if (!scene || !kernel) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_gp_medium: null argument");
+  if (scene_in_live_batch(scene->ctx, scene))
+    return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_gp_medium: scene belongs to a live batch");
+  HIP_TRY(hipSetDevice(scene->ctx->device));
+  HostScene& H = scene->host;
+  const bool had_mesh = H.has_mean_mesh;
+  HostMesh old_mesh = std::move(H.mean_mesh);
+  const DevScene old_ds = H.ds;
+  std::string err;
+  mupsi_status st = lower_gp_medium(mean_type, mean_center, mean_radius, mean_mesh, *kernel, &H, &err);
+  if (st != MUPSI_OK) {
+    H.mean_mesh = std::move(old_mesh);
+    return fail(st, err);
+  }
+  auto mem = std::make_unique<DeviceBuffers>();
+  DevMesh mm{};
+  if (H.has_mean_mesh) {
+    st = upload_mesh(*mem, H.mean_mesh, &mm);
+    if (st == MUPSI_OK) st = build_dist_grid(*mem, &mm);
+    if (st != MUPSI_OK) {
+      H.ds = old_ds;
+      H.has_mean_mesh = had_mesh;
+      H.mean_mesh = std::move(old_mesh);
+      return st;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(scene->ctx->stream));  // no launch still reads the old mean
+  DevScene& D = scene->ds;
+  D.gp = H.ds.gp;
+  D.mean_type = H.ds.mean_type;
+  for (int a = 0; a < 3; ++a) D.mean_c[a] = H.ds.mean_c[a];
+  D.mean_r = H.ds.mean_r;
+  scene->mean_mesh = mm;
+  D.mean_mesh = mm;
+  scene->mean_mem = std::move(mem);
+  return MUPSI_OK;
+}
+
+// The render settings a RenderContext snapshots (render_context.h:15-19,
+// trace.h:16-23, gpnoise.h GPSettings): bounce limit, GP correlation mode and
+// skip_medium, as Controller::setGpMode and startProgressive change them.
+extern "C" mupsi_status mupsi_scene_set_settings(mupsi_scene* scene, int32_t max_bounce,
+                                                 int32_t gp_mode, int32_t skip_medium) {
+  if (!scene) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_settings: null scene");
+  if (scene_in_live_batch(scene->ctx, scene))
+    return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_settings: scene belongs to a live batch");
+  if (gp_mode != MUPSI_GP_SINGLE_REALIZATION && gp_mode != MUPSI_GP_RENEWAL_PLUS)
+    return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_settings: unknown gp_mode");
+  HIP_TRY(hipSetDevice(scene->ctx->device));
+  HIP_TRY(hipStreamSynchronize(scene->ctx->stream));
+  for (DevScene* d : {&scene->ds, &scene->host.ds}) {
+    d->max_bounce = max_bounce;
+    d->gp_mode = gp_mode == MUPSI_GP_RENEWAL_PLUS ? 1 : 0;
+    d->skip_medium = skip_medium ? 1 : 0;
+  }
+  return MUPSI_OK;
 }
 
 extern "C" mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int32_t* height) {
@@ -3103,6 +3172,13 @@ static void destroy_batch(mupsi_batch* b) {
 static void batch_forget_scene(mupsi_ctx* ctx, const mupsi_scene* scene) {
   for (mupsi_batch* b : ctx->batches)
     if (std::find(b->scenes.begin(), b->scenes.end(), scene) != b->scenes.end()) b->valid = false;
+}
+
+static bool scene_in_live_batch(const mupsi_ctx* ctx, const mupsi_scene* scene) {
+  for (const mupsi_batch* b : ctx->batches)
+    if (b->valid && std::find(b->scenes.begin(), b->scenes.end(), scene) != b->scenes.end())
+      return true;
+  return false;
 }
 
 extern "C" void mupsi_batch_free(mupsi_batch* b) {

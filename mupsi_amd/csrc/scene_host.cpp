@@ -335,4 +335,40 @@ mupsi_status lower_scene(const mupsi_scene_desc& d, HostScene* out, std::string*
   return MUPSI_OK;
 }
 
+mupsi_status lower_gp_medium(int32_t mean_type, const float* mean_center, float mean_radius,
+                             const mupsi_mesh_desc* mean_mesh, const mupsi_gp_params& kernel,
+                             HostScene* io, std::string* err) {
+  dev::DevScene& S = io->ds;
+  if (!S.has_gp || S.medium_kind != MUPSI_MEDIUM_GP) {
+    *err = "scene has no gp_medium to replace";
+    return MUPSI_ERR_INVALID;
+  }
+  if (!validate_gp(kernel, err)) return MUPSI_ERR_INVALID;
+  HostMesh mesh;
+  if (mean_type == MUPSI_MEAN_MESH) {
+    if (!mean_mesh) {
+      *err = "mean_type mesh needs a mesh";
+      return MUPSI_ERR_INVALID;
+    }
+    if (!lower_mesh(*mean_mesh, &mesh, err)) return MUPSI_ERR_INVALID;
+  } else if (mean_type == MUPSI_MEAN_SPHERE) {
+    if (!mean_center) {
+      *err = "mean_type sphere needs a centre";
+      return MUPSI_ERR_INVALID;
+    }
+  } else {
+    *err = "unknown mean_type";
+    return MUPSI_ERR_INVALID;
+  }
+  S.gp = make_dev_gp(kernel);
+  S.mean_type = mean_type;
+  if (mean_type == MUPSI_MEAN_SPHERE) {
+    for (int a = 0; a < 3; ++a) S.mean_c[a] = mean_center[a];
+    S.mean_r = mean_radius;
+  }
+  io->has_mean_mesh = mean_type == MUPSI_MEAN_MESH;
+  io->mean_mesh = std::move(mesh);
+  return MUPSI_OK;
+}
+
 }  // namespace mupsi

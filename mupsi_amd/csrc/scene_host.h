@@ -42,6 +42,12 @@ struct HostScene {
 
 // Validate and lower; returns MUPSI_OK or an error status with *err set.
 mupsi_status lower_scene(const mupsi_scene_desc& d, HostScene* out, std::string* err);
+// A new GPMedium for a scene that has one (Controller::applyGpMedium,
+// controller.cpp:397-426): updates the GP constants and mean of io->ds and the
+// mean mesh; on failure io is left as it was.
+mupsi_status lower_gp_medium(int32_t mean_type, const float* mean_center, float mean_radius,
+                             const mupsi_mesh_desc* mean_mesh, const mupsi_gp_params& kernel,
+                             HostScene* io, std::string* err);
 bool validate_gp(const mupsi_gp_params& p, std::string* err);
 dev::DevGP make_dev_gp(const mupsi_gp_params& p);
 uint32_t host_constant_sampler_i(uint32_t seed);
