@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -718,6 +719,19 @@ struct MarchArgs {
   const uint16_t* sid;
 };
 
+// A field of a kernel's FIRST argument, read from the kernarg segment where it is
+// used.  Pointers that a persistent kernel needs only when it takes or retires a
+// ray are otherwise held in scalar registers for the whole kernel, and the
+// scalar registers they take are spilled to vector-register lanes that the hot
+// loop then reads back with VALU instructions.
+template <typename Args, typename T, size_t kOff>
+__device__ __forceinline__ T kernarg_field() {
+  using CP = const char __attribute__((address_space(4)))*;
+  const CP base = (CP)__builtin_amdgcn_kernarg_segment_ptr();
+  return *(const volatile T __attribute__((address_space(4)))*)(base + kOff);
+}
+#define MARCH_ARG(field) kernarg_field<MarchArgs, decltype(MarchArgs::field), offsetof(MarchArgs, field)>()
+
 // GPMedium::sampleDistance / transmittance (gpmedium.cpp:46-129), lane per ray.
 //
 // Reference semantics: dt = min(0.5, (min(farT,2000) - nearT)/100); f_prev =
@@ -995,7 +1009,7 @@ __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float
 //            is evaluated at once, one point per lane, and kept as a bit mask
 //   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
 template <int PPC, bool kMesh, bool kMulti, int GS = kGroup>
-__global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
+__global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
                                                      CachedConsts cc0,
                                                      const CachedConsts* ccs) {
   static_assert(GS == 8 || GS == 16 || GS == 32, "a ray's group is 8, 16 or 32 lanes");
@@ -1048,14 +1062,15 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
 
   auto finish = [&](bool hit, F3 p) {
     if (gl == 0) {
-      a.hit[rid] = hit ? 1 : 0;
+      MARCH_ARG(hit)[rid] = hit ? 1 : 0;
       if (hit) {
-        a.t[rid] = t;
-        a.px[rid] = p.x;
-        a.py[rid] = p.y;
-        a.pz[rid] = p.z;
+        MARCH_ARG(t)[rid] = t;
+        MARCH_ARG(px)[rid] = p.x;
+        MARCH_ARG(py)[rid] = p.y;
+        MARCH_ARG(pz)[rid] = p.z;
       }
-      if (a.steps) a.steps[rid] = hit ? nsteps : -1;
+      int32_t* const steps = MARCH_ARG(steps);
+      if (steps) steps[rid] = hit ? nsteps : -1;
       refsteps += hit ? (unsigned long long)nsteps
                       : nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
     }
@@ -1072,29 +1087,33 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
       if (m) {
         const int leader = __ffsll((unsigned long long)m) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(a.work, __popcll(m));
+        if (lane == leader) base = atomicAdd(MARCH_ARG(work), __popcll(m));
         base = __shfl(base, leader);
         if (need) {
           const int qi = base + __popcll(m & ((1ull << (grp * GS)) - 1ull));
           if (qi < n) {
-            rid = a.queue ? a.queue[qi] : qi;
+            const int* const queue = MARCH_ARG(queue);
+            rid = queue ? queue[qi] : qi;
             if constexpr (kMulti) {
-              sidx = a.sid[rid];
+              sidx = MARCH_ARG(sid)[rid];
               G = a.scenes[sidx].gp;
               ppc = G.ppc;
               cc = ccs[sidx];
             }
-            o = mk(a.ox[rid], a.oy[rid], a.oz[rid]);
-            d = mk(a.dx[rid], a.dy[rid], a.dz[rid]);
-            nt = a.nt[rid];
-            ft = fminf(a.ft ? a.ft[rid] : 3.402823466e+38f, 2000.0f);
+            o = mk(MARCH_ARG(ox)[rid], MARCH_ARG(oy)[rid], MARCH_ARG(oz)[rid]);
+            d = mk(MARCH_ARG(dx)[rid], MARCH_ARG(dy)[rid], MARCH_ARG(dz)[rid]);
+            nt = MARCH_ARG(nt)[rid];
+            const float* const ftp = MARCH_ARG(ft);
+            ft = fminf(ftp ? ftp[rid] : 3.402823466e+38f, 2000.0f);
             dt = fminf(0.5f, (ft - nt) / 100);
-            seed = a.seed[rid];
-            c.active = a.flags ? (a.flags[rid] & kFlagCond) != 0 : (a.active && a.active[rid]);
+            seed = MARCH_ARG(seed)[rid];
+            const uint32_t* const flags = MARCH_ARG(flags);
+            const uint8_t* const act = MARCH_ARG(active);
+            c.active = flags ? (flags[rid] & kFlagCond) != 0 : (act && act[rid]);
             if (c.active) {
-              c.C = mk(a.cx[rid], a.cy[rid], a.cz[rid]);
-              c.u = a.cu[rid];
-              c.g = mk(a.cgx[rid], a.cgy[rid], a.cgz[rid]);
+              c.C = mk(MARCH_ARG(cx)[rid], MARCH_ARG(cy)[rid], MARCH_ARG(cz)[rid]);
+              c.u = MARCH_ARG(cu)[rid];
+              c.g = mk(MARCH_ARG(cgx)[rid], MARCH_ARG(cgy)[rid], MARCH_ARG(cgz)[rid]);
             }
             sk = make_skip<kMesh>(scene(), c, o, ft);
             nsteps = 0;
@@ -1228,10 +1247,11 @@ __global__ __launch_bounds__(64) void k_march_window(DevScene S0, MarchArgs a,
   evals = wave_sum(evals);
   refsteps = wave_sum(refsteps);
   rays = wave_sum(rays);
-  if (lane == 0 && a.stats) {
-    atomicAdd(&a.stats[0], evals);
-    atomicAdd(&a.stats[1], refsteps);
-    atomicAdd(&a.stats[2], rays);
+  unsigned long long* const stats = MARCH_ARG(stats);
+  if (lane == 0 && stats) {
+    atomicAdd(&stats[0], evals);
+    atomicAdd(&stats[1], refsteps);
+    atomicAdd(&stats[2], rays);
   }
 }
 
@@ -2559,10 +2579,10 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
   case P:                                                                                   \
     if (mesh)                                                                               \
       hipLaunchKernelGGL((k_march_window<P, true, false, mesh_group(P)>), grid, block, 0, st, \
-                         S, ma, cc, nullptr);                                               \
+                         ma, S, cc, nullptr);                                               \
     else                                                                                    \
       hipLaunchKernelGGL((k_march_window<P, false, false, sphere_group(P)>), grid, block, 0,   \
-                         st, S, ma, cc, nullptr);                                           \
+                         st, ma, S, cc, nullptr);                                           \
     break;
     switch (S.gp.ppc) {
       MUPSI_CACHED(1)
@@ -2988,7 +3008,7 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.count_dev = nM;
         ma.work = work;
         hipLaunchKernelGGL((k_march_window<kMaxPpcCached, false, true, kMultiGroup>), wgrid,
-                           wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
+                           wblock, 0, st, ma, DevScene{}, CachedConsts{}, b->d_cc);
       }
       if (b->any_cached_mesh) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work3, 1, 0, -1);
@@ -2996,7 +3016,7 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
         ma.count_dev = nMs;
         ma.work = work3;
         hipLaunchKernelGGL((k_march_window<kMaxPpcCached, true, true, kMultiGroup>), wgrid,
-                           wblock, 0, st, DevScene{}, ma, CachedConsts{}, b->d_cc);
+                           wblock, 0, st, ma, DevScene{}, CachedConsts{}, b->d_cc);
       }
       if (b->any_plain) {
         hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, work2, 1, 0, -1);
