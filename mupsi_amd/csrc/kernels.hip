@@ -491,6 +491,10 @@ __device__ __forceinline__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, 
 struct SkipInfo {
   int kind;            // 0 none, 1 sphere mean, 2 mesh mean
   float outer2, inner2;
+  // sphere mean: every grid point that classify_point leaves uncertain has t in
+  // [ev0, ev1] and outside (in0, in1); such t need no classification (an exact
+  // evaluation takes the decision a certain sign would)
+  float ev0, ev1, in0, in1;
   float Bm;            // mesh: B + margin (world units)
   F3 cb;               // mesh: centre of the world bounding ball of the root box
   float rb;            // mesh: its radius
@@ -499,11 +503,15 @@ struct SkipInfo {
 
 // kMesh false: the scene's mean is known to be a sphere.
 template <bool kMesh = true>
-__device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, F3 o, float ft) {
+__device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, F3 o, F3 d, float ft) {
   SkipInfo k;
   k.kind = 0;
   k.outer2 = INFINITY;
   k.inner2 = -1.0f;
+  k.ev0 = INFINITY;
+  k.ev1 = -INFINITY;
+  k.in0 = INFINITY;
+  k.in1 = -INFINITY;
   const float B = S.gp.bound + cond_bound(S.gp, c);
   if (!kMesh || S.mean_type == MUPSI_MEAN_SPHERE) {
     const F3 oc = sub(o, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
@@ -513,6 +521,41 @@ __device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, 
     const float ri = S.mean_r - B - m;
     k.inner2 = ri > 0.0f ? ri * ri : -1.0f;
     k.kind = 1;
+    // The t-ranges of the ray inside the ball of radius ro (widened) and inside
+    // that of radius ri (narrowed), in FP64.  A grid point's classification
+    // compares the FP32 |p - c|^2 with outer2 / inner2; p = o + d t and p - c
+    // carry errors below 1e-6 (|o| + |c| + |t|) per component and the square a
+    // few ulps, far inside the widening, so every uncertain point lies in
+    // [ev0, ev1] and outside (in0, in1).
+    const double ox = oc.x, oy = oc.y, oz = oc.z, dx = d.x, dy = d.y, dz = d.z;
+    const double a = dx * dx + dy * dy + dz * dz;
+    const double b = ox * dx + oy * dy + oz * dz;
+    const double cc = ox * ox + oy * oy + oz * oz;
+    const double slack =
+        1e-3 + 1e-5 * ((double)norm(o) + (double)norm(sub(o, oc)) + (double)ft + ro);
+    if (a > 0.0) {
+      const double rw = (double)ro * (1.0 + 1e-5) + slack;
+      const double disc = b * b - a * (cc - rw * rw);
+      if (disc >= 0.0) {
+        const double sq = sqrt(disc);
+        const double w = 1e-5 * (1.0 + fabs(b) + sq) / a + 1e-4;
+        k.ev0 = (float)((-b - sq) / a - w);
+        k.ev1 = (float)((-b + sq) / a + w);
+      }
+      if (ri > 0.0f) {
+        const double rn = (double)ri * (1.0 - 1e-5) - slack;
+        const double disci = b * b - a * (cc - rn * rn);
+        if (rn > 0.0 && disci > 0.0) {
+          const double sq = sqrt(disci);
+          const double w = 1e-5 * (1.0 + fabs(b) + sq) / a + 1e-4;
+          const double i0 = (-b - sq) / a + w, i1 = (-b + sq) / a - w;
+          if (i0 < i1) {
+            k.in0 = (float)i0;
+            k.in1 = (float)i1;
+          }
+        }
+      }
+    }
     return k;
   }
   if constexpr (!kMesh) return k;
@@ -828,7 +871,7 @@ __global__ __launch_bounds__(256) void k_march(DevScene S0, MarchArgs a) {
               refsteps += it;
               phase = 0;
             } else {
-              sk = make_skip(scene(), c, o, ft);
+              sk = make_skip(scene(), c, o, d, ft);
               nsteps = 0;
               phase = 1;
             }
@@ -1043,7 +1086,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   int nsteps = 0;
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
-  unsigned long long evals = 0, refsteps = 0, rays = 0;
+  uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
   // mesh means: Mesh::inside at grid points gi0 .. gi0 + 31 of the current ray
   int gi = 0, gi0 = -1;
   uint32_t imask = 0;
@@ -1071,8 +1114,8 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       }
       int32_t* const steps = MARCH_ARG(steps);
       if (steps) steps[rid] = hit ? nsteps : -1;
-      refsteps += hit ? (unsigned long long)nsteps
-                      : nsteps + (unsigned long long)fmaxf(0.0f, ceilf((ft - t) / dt));
+      refsteps += hit ? (uint32_t)nsteps
+                      : nsteps + (uint32_t)fmaxf(0.0f, ceilf((ft - t) / dt));
     }
     phase = 0;
   };
@@ -1115,7 +1158,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
               c.u = MARCH_ARG(cu)[rid];
               c.g = mk(MARCH_ARG(cgx)[rid], MARCH_ARG(cgy)[rid], MARCH_ARG(cgz)[rid]);
             }
-            sk = make_skip<kMesh>(scene(), c, o, ft);
+            sk = make_skip<kMesh>(scene(), c, o, d, ft);
             nsteps = 0;
             wvalid = false;  // new realization seed
             gi0 = -1;
@@ -1136,6 +1179,11 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
             break;
           }
           const F3 p = madd(d, t, o);
+          if (sk.kind == 1 && t >= sk.ev0 && t <= sk.ev1 && !(t > sk.in0 && t < sk.in1)) {
+            q = p;  // uncertain for any classification: evaluate
+            do_eval = true;
+            break;
+          }
           bool fin;
           int ins = -1;
           if (kMesh && sk.kind == 2) ins = inside_here();
@@ -1244,14 +1292,14 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       }
     }
   }
-  evals = wave_sum(evals);
-  refsteps = wave_sum(refsteps);
-  rays = wave_sum(rays);
+  const unsigned long long evals_w = wave_sum((unsigned long long)evals);
+  const unsigned long long refsteps_w = wave_sum((unsigned long long)refsteps);
+  const unsigned long long rays_w = wave_sum((unsigned long long)rays);
   unsigned long long* const stats = MARCH_ARG(stats);
   if (lane == 0 && stats) {
-    atomicAdd(&stats[0], evals);
-    atomicAdd(&stats[1], refsteps);
-    atomicAdd(&stats[2], rays);
+    atomicAdd(&stats[0], evals_w);
+    atomicAdd(&stats[1], refsteps_w);
+    atomicAdd(&stats[2], rays_w);
   }
 }
 
