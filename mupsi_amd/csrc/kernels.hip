@@ -1026,6 +1026,28 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
+// The xor-k neighbour's value within 32-lane groups (ds_swizzle, bit mode).
+template <int K>
+__device__ __forceinline__ float swizzle_xor(float v) {
+  static_assert(K == 1 || K == 2 || K == 4 || K == 8 || K == 16, "xor within 32 lanes");
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), (K << 10) | 0x1F));
+}
+
+// Two sums over a group at once: lanes gl < GS/2 start with (a = their part of
+// sum 1, b = their part of sum 2), the others with (their part of 2, of 1).  The
+// first step exchanges across the halves, the rest is group_sum's butterfly
+// within each half: lanes gl < GS/2 end with sum 1, the others with sum 2, each
+// added in group_sum's order up to commutation.
+template <int GS>
+__device__ __forceinline__ float group_sum2(float a, float b) {
+  float v = a + swizzle_xor<GS / 2>(b);
+  if constexpr (GS >= 32) v += swizzle_xor<8>(v);
+  if constexpr (GS >= 16) v += swizzle_xor<4>(v);
+  if constexpr (GS >= 8) v += swizzle_xor<2>(v);
+  v += swizzle_xor<1>(v);
+  return v;
+}
+
 // This lane's part of the window sum: slots gl, gl + GS, ... of the 27 x PPC.
 template <int PPC, int GS = kGroup>
 __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float psi_k) {
@@ -1038,6 +1060,47 @@ __device__ __forceinline__ float window_sum(const float4* W, F3 q, int gl, float
     acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * psi_k), acc);
   }
   return acc;
+}
+
+// This lane's parts of the window sums at q and q2 (same slots, same order);
+// without a pair anywhere in the wave the second sum is not formed.
+template <int PPC, int GS>
+__device__ __forceinline__ void window_sums(const float4* W, F3 q, F3 q2, bool two, int gl,
+                                            float psi_k, float& acc, float& acc2) {
+  if (!two) {
+    acc = window_sum<PPC, GS>(W, q, gl, psi_k);
+    acc2 = acc;
+    return;
+  }
+  acc = 0.0f;
+  acc2 = 0.0f;
+#pragma unroll
+  for (int sl = gl; sl < 27 * PPC; sl += GS) {
+    const float4 im = W[sl];
+    const float ex = q.x - im.x, ey = q.y - im.y, ez = q.z - im.z;
+    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+    acc = fmaf(im.w, __builtin_amdgcn_exp2f(r2 * psi_k), acc);
+    const float fx = q2.x - im.x, fy = q2.y - im.y, fz = q2.z - im.z;
+    const float s2 = fmaf(fx, fx, fmaf(fy, fy, fz * fz));
+    acc2 = fmaf(im.w, __builtin_amdgcn_exp2f(s2 * psi_k), acc2);
+  }
+}
+
+// Sphere mean: t lies where no classification can decide the sign.
+__device__ __forceinline__ bool uncertain_t(const SkipInfo& k, float t) {
+  return t >= k.ev0 && t <= k.ev1 && !(t > k.in0 && t < k.in1);
+}
+
+// The grid point after t when the march reaches it and it must be evaluated too
+// (sphere means); q2 receives it.
+__device__ __forceinline__ bool next_uncertain(const DevScene& S, const SkipInfo& k, F3 d, F3 o,
+                                               float t, float dt, float ft, F3& q2) {
+  const float t2 = t + dt;
+  if (!(t2 > t) || !(t2 < ft) || k.kind != 1) return false;
+  q2 = madd(d, t2, o);
+  if (uncertain_t(k, t2)) return true;
+  bool fin;
+  return classify_point<false>(S, k, q2, d, &fin) == 0;
 }
 
 // The window marcher: GPMedium::sampleDistance / transmittance
@@ -1124,6 +1187,10 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     bool do_eval = false;
     F3 q = mk(0, 0, 0);
     int qins = -1;  // Mesh::inside(q) if known
+    // sphere means: the grid point after q, evaluated in the same pass when it is
+    // uncertain too and lies in q's cell (same window)
+    bool pair = false;
+    F3 q2 = mk(0, 0, 0);
     for (int round = 0; round < 2; ++round) {
       const bool need = phase == 0;
       const uint64_t m = __ballot(need && gl == 0);
@@ -1179,9 +1246,10 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
             break;
           }
           const F3 p = madd(d, t, o);
-          if (sk.kind == 1 && t >= sk.ev0 && t <= sk.ev1 && !(t > sk.in0 && t < sk.in1)) {
+          if (sk.kind == 1 && uncertain_t(sk, t)) {
             q = p;  // uncertain for any classification: evaluate
             do_eval = true;
+            if constexpr (!kMesh) pair = next_uncertain(scene(), sk, d, o, t, dt, ft, q2);
             break;
           }
           bool fin;
@@ -1192,6 +1260,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
             q = p;
             qins = ins;
             do_eval = true;
+            if constexpr (!kMesh) pair = next_uncertain(scene(), sk, d, o, t, dt, ft, q2);
             break;
           }
           ++nsteps;
@@ -1218,6 +1287,10 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
       cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
       cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
+      if (pair)
+        pair = cell_coord(q2.x, G, cc.inv_kr, cc.kr_pow2) == cx &&
+               cell_coord(q2.y, G, cc.inv_kr, cc.kr_pow2) == cy &&
+               cell_coord(q2.z, G, cc.inv_kr, cc.kr_pow2) == cz;
       if (!wvalid) {
         newcells = 27;
       } else {
@@ -1250,28 +1323,42 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     }
     __syncthreads();
 
-    float acc = 0.0f;
+    // Window sums at q (and q2): each lane takes the same slots in the same order
+    // for either point, so a point's sum does not depend on whether it was paired.
+    const bool any_pair = !kMesh && __ballot(pair) != 0;
+    if (!pair) q2 = q;
+    float acc = 0.0f, acc2 = 0.0f;
     if (do_eval) {
       if constexpr (kMulti) {
         switch (ppc) {  // the compile-time slot count, per ray
-          case 1: acc = window_sum<1, GS>(W, q, gl, cc.psi_k); break;
-          case 2: acc = window_sum<2, GS>(W, q, gl, cc.psi_k); break;
-          case 3: acc = window_sum<3, GS>(W, q, gl, cc.psi_k); break;
-          case 4: acc = window_sum<4, GS>(W, q, gl, cc.psi_k); break;
-          case 5: acc = window_sum<5, GS>(W, q, gl, cc.psi_k); break;
-          case 6: acc = window_sum<6, GS>(W, q, gl, cc.psi_k); break;
-          case 7: acc = window_sum<7, GS>(W, q, gl, cc.psi_k); break;
-          default: acc = window_sum<8, GS>(W, q, gl, cc.psi_k); break;
+          case 1: window_sums<1, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 2: window_sums<2, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 3: window_sums<3, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 4: window_sums<4, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 5: window_sums<5, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 6: window_sums<6, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          case 7: window_sums<7, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
+          default: window_sums<8, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
         }
       } else {
-        acc = window_sum<PPC, GS>(W, q, gl, cc.psi_k);
+        window_sums<PPC, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2);
       }
     }
-    acc = group_sum<GS>(acc);
+    // lanes gl < GS/2 finish q's sum, the others q2's (the same total as
+    // group_sum gives, addition being commutative)
+    const bool lo = gl < GS / 2;
+    acc = any_pair ? group_sum2<GS>(lo ? acc : acc2, lo ? acc2 : acc) : group_sum<GS>(acc);
 
     if (do_eval) {
-      const float f = cached_field<kMesh>(scene(), G, q, acc * cc.psi_scale, c, cc.psi_k, qins);
-      if (gl == 0) ++evals;
+      const F3 qm = lo ? q : q2;
+      const float fm = cached_field<kMesh>(scene(), G, qm, acc * cc.psi_scale, c, cc.psi_k, qins);
+      float f = fm, f2 = fm;
+      if (any_pair) {
+        const float fo = swizzle_xor<GS / 2>(fm);
+        f = lo ? fm : fo;
+        f2 = lo ? fo : fm;
+      }
+      if (gl == 0) evals += pair ? 2 : 1;
       if (phase == 1) {
         fprev = f;
         sp = f > 0.0f;
@@ -1288,6 +1375,16 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
         } else {
           t += dt;
           ++gi;
+          if (pair) {  // the next grid point, as the next pass would take it
+            ++nsteps;
+            if ((f2 > 0.0f) != sp && fabsf(fprev - f2) > 1e-6f) {
+              finish(true, q2);
+            } else if (!(t + dt > t)) {
+              finish(false, q2);
+            } else {
+              t += dt;
+            }
+          }
         }
       }
     }
