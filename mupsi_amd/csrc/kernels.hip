@@ -1120,7 +1120,12 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
                                                      const CachedConsts* ccs) {
   static_assert(GS == 8 || GS == 16 || GS == 32, "a ray's group is 8, 16 or 32 lanes");
   constexpr uint64_t kGroupMask = (1ull << GS) - 1ull;
-  __shared__ float4 win[64 / GS][27 * PPC];
+  constexpr int NG = 64 / GS;  // rays (groups) per wave
+  __shared__ float4 win[NG][27 * PPC];
+  // a group's window regeneration for the wave's pooled pass: cell (x, y, z), seed;
+  // entered-cell pattern | window residues | points per cell; cell size
+  __shared__ int4 regen_cell[NG];
+  __shared__ int2 regen_aux[NG];
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
   const int grp = lane / GS;
@@ -1301,21 +1306,40 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
         }
       }
     }
+    // Regeneration of the entered cells of every group of the wave as one pool of
+    // items over the 64 lanes (item j of group g: impulse i of entered cell k),
+    // written to the same window slots a group's own lanes would write.
+    const int items = newcells >= 0 ? kNewCells.n[newcells] * ppc : 0;
+    if (gl == 0 && newcells >= 0) {
+      regen_cell[grp] = make_int4(cx, cy, cz, (int)seed);
+      regen_aux[grp] = make_int2(newcells | mod3(cx) << 5 | mod3(cy) << 7 | mod3(cz) << 9 | ppc << 11,
+                                 __float_as_int(G.kr));
+    }
     __syncthreads();  // the previous evaluation's reads of the window are complete
+    int start[NG + 1];  // first pooled item of each group (wave-uniform)
+    start[0] = 0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) start[g + 1] = start[g] + __builtin_amdgcn_readlane(items, g * GS);
+    for (int j = lane; j < start[NG]; j += 64) {
+      int g = 0;
+#pragma unroll
+      for (int h = 1; h < NG; ++h) g += j >= start[h] ? 1 : 0;
+      const int4 rc = regen_cell[g];
+      const int2 ra = regen_aux[g];
+      const int gppc = ra.x >> 11;
+      const int it = j - start[g];
+      const int k = it / gppc;
+      const int i = it - k * gppc;
+      const int ni = kNewCells.idx[ra.x & 31][k];
+      const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
+      int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
+      rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
+      ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
+      rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
+      win[g][(rx * 9 + ry * 3 + rz) * gppc + i] =
+          make_impulse(__int_as_float(ra.y), rc.x + ox, rc.y + oy, rc.z + oz, (uint32_t)rc.w, i);
+    }
     if (newcells >= 0) {
-      const int items = kNewCells.n[newcells] * ppc;
-      const int mx = mod3(cx), my = mod3(cy), mz = mod3(cz);  // residues of the window centre
-      for (int it = gl; it < items; it += GS) {
-        const int k = it / ppc;
-        const int i = it - k * ppc;
-        const int ni = kNewCells.idx[newcells][k];
-        const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
-        int rx = mx + ox, ry = my + oy, rz = mz + oz;
-        rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
-        ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
-        rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
-        W[(rx * 9 + ry * 3 + rz) * ppc + i] = make_impulse(G, cx + ox, cy + oy, cz + oz, seed, i);
-      }
       wx = cx;
       wy = cy;
       wz = cz;

@@ -97,9 +97,10 @@ __device__ __forceinline__ int cell_coord(float q, const DevGP& g, float inv_kr,
   return (int)(q / g.kr);
 }
 
-// Impulse i of cell (nx, ny, nz) of realization seed as (position, weight).
-__device__ __forceinline__ float4 make_impulse(const DevGP& g, int nx, int ny, int nz,
-                                               uint32_t seed, int i) {
+// Impulse i of cell (nx, ny, nz) of realization seed as (position, weight); kr is
+// the cell size.
+__device__ __forceinline__ float4 make_impulse(float kr, int nx, int ny, int nz, uint32_t seed,
+                                               int i) {
   const uint32_t cs = make_seed4((uint32_t)nx, (uint32_t)ny, (uint32_t)nz, seed);
   uint64_t s = (uint32_t)(cs + 1u);
   s = s * kJump.a[i] + kJump.c[i];
@@ -107,8 +108,12 @@ __device__ __forceinline__ float4 make_impulse(const DevGP& g, int nx, int ny, i
   const uint32_t r1 = pcg_next(s);
   const uint32_t r2 = pcg_next(s);
   const uint32_t r3 = pcg_next(s);
-  return make_float4(fmaf(u01(r2), g.kr, (float)nx * g.kr), fmaf(u01(r1), g.kr, (float)ny * g.kr),
-                     fmaf(u01(r0), g.kr, (float)nz * g.kr), (r3 >> 31) ? 1.0f : -1.0f);
+  return make_float4(fmaf(u01(r2), kr, (float)nx * kr), fmaf(u01(r1), kr, (float)ny * kr),
+                     fmaf(u01(r0), kr, (float)nz * kr), (r3 >> 31) ? 1.0f : -1.0f);
+}
+__device__ __forceinline__ float4 make_impulse(const DevGP& g, int nx, int ny, int nz,
+                                               uint32_t seed, int i) {
+  return make_impulse(g.kr, nx, ny, nz, seed, i);
 }
 
 }  // namespace dev
