@@ -1326,12 +1326,12 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       for (int h = 1; h < NG; ++h) g += j >= start[h] ? 1 : 0;
       const int4 rc = regen_cell[g];
       const int2 ra = regen_aux[g];
-      const int gppc = ra.x >> 11;
+      const int gppc = kMulti ? ra.x >> 11 : PPC;  // a compile-time divisor when it can be
       const int it = j - start[g];
       const int k = it / gppc;
       const int i = it - k * gppc;
-      const int ni = kNewCells.idx[ra.x & 31][k];
-      const int ox = ni / 9 - 1, oy = (ni / 3) % 3 - 1, oz = ni % 3 - 1;  // offsets in {-1,0,1}
+      const int no = kNewCells.off[ra.x & 31][k];
+      const int ox = (no & 3) - 1, oy = ((no >> 2) & 3) - 1, oz = (no >> 4) - 1;  // in {-1,0,1}
       int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
       rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
       ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);

@@ -57,10 +57,13 @@ __constant__ constexpr LcgJump kJump = make_jumps();
 
 // New cells of the window when its centre moves by delta (each |delta_a| <= 1):
 // {d in {-1,0,1}^3 : some axis a has delta_a != 0 and d_a == delta_a}.
-// Entry [delta index][k] is the neighbour index 9(dx+1) + 3(dy+1) + (dz+1).
+// Entry [delta index][k] is the neighbour index 9(dx+1) + 3(dy+1) + (dz+1);
+// off[delta index][k] the same neighbour's offsets packed as (dx+1) | (dy+1) << 2 |
+// (dz+1) << 4.
 struct NewCells {
   uint8_t n[28];
   uint8_t idx[28][27];
+  uint8_t off[28][27];
 };
 constexpr NewCells make_new_cells() {
   NewCells t{};
@@ -73,7 +76,10 @@ constexpr NewCells make_new_cells() {
         const int nn[3] = {ni / 9 - 1, (ni / 3) % 3 - 1, ni % 3 - 1};
         for (int a = 0; a < 3; ++a) add = add || (dd[a] != 0 && nn[a] == dd[a]);
       }
-      if (add) t.idx[di][cnt++] = (uint8_t)ni;
+      if (add) {
+        t.idx[di][cnt] = (uint8_t)ni;
+        t.off[di][cnt++] = (uint8_t)((ni / 9) | ((ni / 3) % 3) << 2 | (ni % 3) << 4);
+      }
     }
     t.n[di] = (uint8_t)cnt;
   }
