@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mupsi.h"
@@ -761,6 +762,7 @@ struct MarchArgs {
   const DevScene* scenes;     // batched launch: per-scene constants, indexed by sid[ray]
   const uint16_t* sid;
 };
+static_assert(std::is_standard_layout<MarchArgs>::value, "MarchArgs fields are read by offset");
 
 // A field of a kernel's FIRST argument, read from the kernarg segment where it is
 // used.  Pointers that a persistent kernel needs only when it takes or retires a
@@ -773,6 +775,8 @@ __device__ __forceinline__ T kernarg_field() {
   const CP base = (CP)__builtin_amdgcn_kernarg_segment_ptr();
   return *(const volatile T __attribute__((address_space(4)))*)(base + kOff);
 }
+// MARCH_ARG is valid only in kernels whose first parameter is the MarchArgs
+// (kernel arguments are laid out from offset 0 in declaration order).
 #define MARCH_ARG(field) kernarg_field<MarchArgs, decltype(MarchArgs::field), offsetof(MarchArgs, field)>()
 
 // GPMedium::sampleDistance / transmittance (gpmedium.cpp:46-129), lane per ray.
@@ -1114,6 +1118,7 @@ __device__ __forceinline__ bool next_uncertain(const DevScene& S, const SkipInfo
 //   kMesh    mean meshes possible: inside() at the next 32 grid points of a ray
 //            is evaluated at once, one point per lane, and kept as a bit mask
 //   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
+// The MarchArgs stay the first parameter: MARCH_ARG reads them at offset 0.
 template <int PPC, bool kMesh, bool kMulti, int GS = kGroup>
 __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
                                                      CachedConsts cc0,
