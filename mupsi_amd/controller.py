@@ -224,7 +224,7 @@ class Controller:
             img = tonemap(self._front) if tone else self._front.copy()
         if dst is None:
             return img
-        np.copyto(dst.reshape(img.shape), img)
+        np.copyto(dst, img.reshape(dst.shape))  # dst: (H, W, 3) or H*W*3 floats, any strides
         return dst
 
     def copy_display_to(self, dst: Optional[np.ndarray] = None) -> Optional[np.ndarray]:

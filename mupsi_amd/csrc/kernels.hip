@@ -2569,6 +2569,7 @@ extern "C" mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t me
   if (scene_in_live_batch(scene->ctx, scene))
     return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_gp_medium: scene belongs to a live batch");
   HIP_TRY(hipSetDevice(scene->ctx->device));
+  HIP_TRY(hipStreamSynchronize(scene->ctx->stream));  // no launch still reads the old mean
   HostScene& H = scene->host;
   const bool had_mesh = H.has_mean_mesh;
   HostMesh old_mesh = std::move(H.mean_mesh);
@@ -2591,7 +2592,6 @@ extern "C" mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t me
       return st;
     }
   }
-  HIP_TRY(hipStreamSynchronize(scene->ctx->stream));  // no launch still reads the old mean
   DevScene& D = scene->ds;
   D.gp = H.ds.gp;
   D.mean_type = H.ds.mean_type;
