@@ -1201,7 +1201,10 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     // uncertain too and lies in q's cell (same window)
     bool pair = false;
     F3 q2 = mk(0, 0, 0);
-    for (int round = 0; round < 2; ++round) {
+    // One fetch per pass: a group whose ray ended in this pass's stepping takes its
+    // next ray in the next pass (a second fetch round cost more than the pass it
+    // saves such a group).
+    {
       const bool need = phase == 0;
       const uint64_t m = __ballot(need && gl == 0);
       if (m) {
@@ -1286,7 +1289,6 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
           ++gi;
         }
       }
-      if (__ballot(phase == 0) == 0) break;
     }
     if (__ballot(phase != 3) == 0) break;
 
