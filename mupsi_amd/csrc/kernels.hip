@@ -1176,22 +1176,9 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     return (int)((imask >> (gi - gi0)) & 1u);
   };
 
-  auto finish = [&](bool hit, F3 p) {
-    if (gl == 0) {
-      MARCH_ARG(hit)[rid] = hit ? 1 : 0;
-      if (hit) {
-        MARCH_ARG(t)[rid] = t;
-        MARCH_ARG(px)[rid] = p.x;
-        MARCH_ARG(py)[rid] = p.y;
-        MARCH_ARG(pz)[rid] = p.z;
-      }
-      int32_t* const steps = MARCH_ARG(steps);
-      if (steps) steps[rid] = hit ? nsteps : -1;
-      refsteps += hit ? (uint32_t)nsteps
-                      : nsteps + (uint32_t)fmaxf(0.0f, ceilf((ft - t) / dt));
-    }
-    phase = 0;
-  };
+  // A ray ends in this pass (phase 4: miss, 5: hit at the current t); its outputs
+  // are written once, at the end of the pass.
+  auto finish = [&](bool hit, F3) { phase = hit ? 5 : 4; };
 
   while (true) {
     bool do_eval = false;
@@ -1418,6 +1405,24 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
           }
         }
       }
+    }
+    if (phase >= 4) {  // retire the ray: outputs at the final t, p = o + d t
+      if (gl == 0) {
+        const bool hit = phase == 5;
+        MARCH_ARG(hit)[rid] = hit ? 1 : 0;
+        if (hit) {
+          const F3 p = madd(d, t, o);
+          MARCH_ARG(t)[rid] = t;
+          MARCH_ARG(px)[rid] = p.x;
+          MARCH_ARG(py)[rid] = p.y;
+          MARCH_ARG(pz)[rid] = p.z;
+        }
+        int32_t* const steps = MARCH_ARG(steps);
+        if (steps) steps[rid] = hit ? nsteps : -1;
+        refsteps += hit ? (uint32_t)nsteps
+                        : nsteps + (uint32_t)fmaxf(0.0f, ceilf((ft - t) / dt));
+      }
+      phase = 0;
     }
   }
   const unsigned long long evals_w = wave_sum((unsigned long long)evals);
