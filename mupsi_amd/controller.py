@@ -36,6 +36,35 @@ from . import Context, MupsiError, Renderer, Scene
 from .framebuffer import save_framebuffer, tonemap
 
 
+def load_settings(j: dict, spp: int = 1) -> dict:
+    """What Controller::load keeps from a scene.json object besides the scene itself
+    (controller.cpp:40-137): spp (the previous value when absent), max_bounce,
+    output, gp_mode, and the gp_medium mean and kernel parameters that
+    applyGpMedium rebuilds from (kernel at the top level, as the reference reads it)."""
+    out = {"spp": int(j.get("spp", spp)), "max_bounce": int(j.get("max_bounce", 3)),
+           "output": j.get("output", "test.png"),
+           "gp_mode": j.get("gp_mode", "single_realization"),
+           "has_gp_medium": "gp_medium" in j}
+    if out["has_gp_medium"]:
+        gpm = j["gp_medium"]
+        out["mean_type"] = gpm.get("mean_type", "sphere")
+        if out["mean_type"] == "sphere":
+            out["mean_center"] = [float(x) for x in gpm["mean_center"]]
+            out["mean_radius"] = float(gpm.get("mean_radius", 70.0))
+        elif out["mean_type"] == "mesh":
+            t = gpm.get("transform", {}) or {}
+            out["mean_mesh"] = gpm.get("mesh", gpm.get("file"))
+            out["mean_mesh_pos"] = [float(x) for x in t.get("position", [0.0, 0.0, 0.0])]
+            out["mean_mesh_scale"] = float(t.get("scale", 1.0))
+        else:
+            raise ValueError(f"unknown mean_type: {out['mean_type']}")
+        kern = j.get("kernel", {}) or {}
+        out["sigma"] = float(kern.get("sigma", 1.0))
+        out["length_scale"] = float(kern.get("length_scale", 1.0))
+        out["points_per_cell"] = int(kern.get("points_per_cell", 3))
+    return out
+
+
 class Controller:
     def __init__(self, ctx: Optional[Context] = None, device: int = 0):
         self.ctx = ctx if ctx is not None else Context(device)
@@ -74,11 +103,11 @@ class Controller:
         self.stop()
         try:
             if isinstance(config, (str, os.PathLike)):
-                j = json.loads(Path(config).read_text())
+                st = load_settings(json.loads(Path(config).read_text()), self._spp)
                 scene = Scene(self.ctx, config)
                 self._config_path = str(config)
             else:
-                j = config
+                st = load_settings(config, self._spp)
                 scene = Scene(self.ctx, config)
                 self._config_path = ""
         except (OSError, ValueError, KeyError, MupsiError) as e:
@@ -87,26 +116,20 @@ class Controller:
         if self._scene is not None:
             self._scene.close()
         self._scene = scene
-        self._spp = int(j.get("spp", self._spp))
-        self._max_bounce = int(j.get("max_bounce", 3))
-        self._output_path = j.get("output", "test.png")
-        self._gp_mode = j.get("gp_mode", "single_realization")
-        self._has_gp_medium = "gp_medium" in j
+        self._spp, self._max_bounce = st["spp"], st["max_bounce"]
+        self._output_path, self._gp_mode = st["output"], st["gp_mode"]
+        self._has_gp_medium = st["has_gp_medium"]
         if self._has_gp_medium:
-            gpm = j["gp_medium"]
-            self._gp_mean_type = gpm.get("mean_type", "sphere")
+            self._gp_mean_type = st["mean_type"]
             if self._gp_mean_type == "sphere":
-                self._gp_mean_center = [float(x) for x in gpm["mean_center"]]
-                self._gp_mean_radius = float(gpm.get("mean_radius", 70.0))
+                self._gp_mean_center, self._gp_mean_radius = st["mean_center"], st["mean_radius"]
             else:
-                t = gpm.get("transform", {}) or {}
-                self._gp_mean_mesh = gpm.get("mesh", gpm.get("file"))
-                self._gp_mean_mesh_pos = [float(x) for x in t.get("position", [0.0, 0.0, 0.0])]
-                self._gp_mean_mesh_scale = float(t.get("scale", 1.0))
-            kern = j.get("kernel", {}) or {}
-            self._gp_kernel_sigma = float(kern.get("sigma", 1.0))
-            self._gp_kernel_length = float(kern.get("length_scale", 1.0))
-            self._gp_points_per_cell = int(kern.get("points_per_cell", 3))
+                self._gp_mean_mesh = st["mean_mesh"]
+                self._gp_mean_mesh_pos = st["mean_mesh_pos"]
+                self._gp_mean_mesh_scale = st["mean_mesh_scale"]
+            self._gp_kernel_sigma = st["sigma"]
+            self._gp_kernel_length = st["length_scale"]
+            self._gp_points_per_cell = st["points_per_cell"]
         return True
 
     # --- render control (controller.cpp:268-345) ---
