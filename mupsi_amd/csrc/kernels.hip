@@ -1239,6 +1239,11 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       if (phase == 1) {
         q = o;
         do_eval = true;
+      } else if (phase == 2 && !do_eval && t < ft && sk.kind == 1 && uncertain_t(sk, t)) {
+        // the common case, outside the stepping loop: a point no classification decides
+        q = madd(d, t, o);
+        do_eval = true;
+        if constexpr (!kMesh) pair = next_uncertain(scene(), sk, d, o, t, dt, ft, q2);
       } else if (phase == 2 && !do_eval) {
         while (true) {
           if (!(t < ft)) {
