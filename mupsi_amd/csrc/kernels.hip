@@ -1351,7 +1351,9 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     const bool any_pair = !kMesh && __ballot(pair) != 0;
     if (!pair) q2 = q;
     float acc = 0.0f, acc2 = 0.0f;
-    if (do_eval) {
+    // sphere means: every group sums (an idle group's result is not used), which
+    // saves the exec-mask bookkeeping; mesh means keep their BVH work guarded
+    if (!kMesh || do_eval) {
       if constexpr (kMulti) {
         switch (ppc) {  // the compile-time slot count, per ray
           case 1: window_sums<1, GS>(W, q, q2, any_pair, gl, cc.psi_k, acc, acc2); break;
@@ -1372,15 +1374,16 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     const bool lo = gl < GS / 2;
     acc = any_pair ? group_sum2<GS>(lo ? acc : acc2, lo ? acc2 : acc) : group_sum<GS>(acc);
 
+    const F3 qm = lo ? q : q2;
+    float fm = 0.0f;
+    if (!kMesh || do_eval) fm = cached_field<kMesh>(scene(), G, qm, acc * cc.psi_scale, c, cc.psi_k, qins);
+    float f = fm, f2 = fm;
+    if (any_pair) {
+      const float fo = swizzle_xor<GS / 2>(fm);
+      f = lo ? fm : fo;
+      f2 = lo ? fo : fm;
+    }
     if (do_eval) {
-      const F3 qm = lo ? q : q2;
-      const float fm = cached_field<kMesh>(scene(), G, qm, acc * cc.psi_scale, c, cc.psi_k, qins);
-      float f = fm, f2 = fm;
-      if (any_pair) {
-        const float fo = swizzle_xor<GS / 2>(fm);
-        f = lo ? fm : fo;
-        f2 = lo ? fo : fm;
-      }
       if (gl == 0) evals += pair ? 2 : 1;
       if (phase == 1) {
         fprev = f;
