@@ -1014,9 +1014,9 @@ struct CachedConsts {
   float psi_scale;  // sigma / sqrt(Var): psi = psi_scale * (impulse sum)
 };
 
-// Sum over the 32 lanes of a group by an xor butterfly (16, 8, 4, 2, 1), every
-// lane ending with the same total: ds_swizzle in bit mode (and 0x1f, xor k)
-// exchanges within 32-lane groups without address arithmetic.
+// Sum over the GS lanes of a group by an xor butterfly (16, 8, 4, 2, 1 for 32
+// lanes), every lane ending with the same total: ds_swizzle in bit mode
+// (and 0x1f, xor k) exchanges within 32-lane halves without address arithmetic.
 // Smaller groups start the butterfly at xor GS / 2.
 template <int GS = 32>
 __device__ __forceinline__ float group_sum(float v) {
@@ -1108,14 +1108,16 @@ __device__ __forceinline__ bool next_uncertain(const DevScene& S, const SkipInfo
 }
 
 // The window marcher: GPMedium::sampleDistance / transmittance
-// (gpmedium.cpp:46-129) with one 32-lane group per ray (two rays per wave).  The
-// 27 x ppc impulses of the 3x3x3 cell window around the evaluation point live in
-// LDS as a toroidal window indexed by cell mod 3; when the march point enters a
-// new cell only the 9/3/1 newly entered cells are regenerated (one PCG jump-ahead
-// per impulse, lanes in parallel).  Each lane sums its slots; a butterfly
-// finishes the sum.
+// (gpmedium.cpp:46-129) with one group of GS lanes per ray (64 / GS rays per
+// wave).  The 27 x ppc impulses of the 3x3x3 cell window around the evaluation
+// point live in LDS as a toroidal window indexed by cell mod 3; when the march
+// point enters a new cell only the 9/3/1 newly entered cells are regenerated
+// (one PCG jump-ahead per impulse, the wave's entered cells as one pool over its
+// lanes).  Each lane sums its slots; a butterfly finishes the sum.  Per pass a
+// group evaluates its next uncertain grid point and, for sphere means, the one
+// after it when that lies in the same cell.
 //   PPC      points per cell (kMulti: the LDS capacity, 8; each ray uses its own)
-//   kMesh    mean meshes possible: inside() at the next 32 grid points of a ray
+//   kMesh    mean meshes possible: inside() at the next GS grid points of a ray
 //            is evaluated at once, one point per lane, and kept as a bit mask
 //   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
 // The MarchArgs stay the first parameter: MARCH_ARG reads them at offset 0.
@@ -1145,7 +1147,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     if constexpr (kMulti) return a.scenes[sidx];
     else return S0;
   };
-  // ray state: identical in the 32 lanes of a group
+  // ray state: identical in the GS lanes of a group
   int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
   int rid = 0;
   F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
@@ -1160,7 +1162,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
-  // mesh means: Mesh::inside at grid points gi0 .. gi0 + 31 of the current ray
+  // mesh means: Mesh::inside at grid points gi0 .. gi0 + GS - 1 of the current ray
   int gi = 0, gi0 = -1;
   uint32_t imask = 0;
   const auto inside_here = [&]() -> int {

@@ -7,13 +7,13 @@
 // from one step to the next, so the impulses are generated once per cell visit
 // and evaluated from LDS at every step.
 //
-// Mapping: 32 lanes (a half-wave "group") march one ray; a 64-lane workgroup holds
-// two groups.  Each group owns a window of 27 cells x ppc impulses in LDS, slot
-// (n_x mod 3, n_y mod 3, n_z mod 3) x ppc + i, so moving the window centre by one
-// cell along an axis overwrites exactly the slab of cells that left it.  The
-// impulses of a cell are written as float4 (position, weight); evaluation reads
-// them with one ds_read_b128 per slot, 27*ppc slots over 32 lanes, and reduces
-// the group's partial sums with cross-lane shuffles.
+// Mapping: a group of 8 or 16 lanes marches one ray; a 64-lane workgroup (one
+// wave) holds 8 or 4 groups.  Each group owns a window of 27 cells x ppc impulses
+// in LDS, slot (n_x mod 3, n_y mod 3, n_z mod 3) x ppc + i, so moving the window
+// centre by one cell along an axis overwrites exactly the slab of cells that left
+// it.  The impulses of a cell are written as float4 (position, weight);
+// evaluation reads them with one ds_read_b128 per slot, 27*ppc slots over the
+// group's lanes, and reduces the group's partial sums with cross-lane swizzles.
 //
 // Results (hit flag, hit t bits) are the reference's sampleDistance/transmittance
 // decisions; only the summation order of psi differs from the reference's
@@ -24,7 +24,7 @@
 namespace mupsi {
 namespace dev {
 
-constexpr int kGroup = 32;        // lanes per ray
+constexpr int kGroup = 32;        // default lanes per ray of the templates (the launches pick below)
 // Lanes per ray of the per-scene instantiations.  Fewer lanes per ray put more
 // rays in a wave (less per-ray control work, more BVH loads in flight for mesh
 // means) but more window slots on each lane; measured best: sphere means 16
