@@ -38,6 +38,9 @@
  *   mupsi_bvh_build       <- BVH::build / buildRecursive src/bvh/bvh.cpp:115-193
  *   mupsi_scene_load_json <- Controller::load (scene.json schema)
  *                                                     src/controller/controller.cpp:40-264
+ *   mupsi_scene_set_gp_medium <- Controller::applyGpMedium  src/controller/controller.cpp:397-426
+ *   mupsi_scene_set_settings  <- the RenderContext snapshot of Controller::startProgressive
+ *                                                     src/controller/controller.cpp:293-345
  *
  * Conventions
  *   - Every function returns mupsi_status (0 = OK).  No C++ exception crosses
@@ -299,9 +302,11 @@ mupsi_status mupsi_mesh_inside(mupsi_scene* scene, int32_t prim, const float* px
  * src/bvh/bvh.cpp:302-353, where both children of an expanded node are
  * tested), Mesh::inside -> BVH::intersect (kind 1; mesh.cpp:341-362,
  * bvh.cpp:197-250, every re-cast included), or distToPoint started from the
- * distance-grid bound as mupsi_mesh_closest runs it on a mean mesh (kind 2).  The reference keeps no such
- * counters; they give the traversal's algorithmic bytes (32 per node, 52 per
- * triangle: index, face and three vertices). */
+ * distance-grid bound as mupsi_mesh_closest runs it on a mean mesh (kind 2),
+ * or Mesh::inside as mupsi_mesh_inside runs it: one traversal collecting the
+ * +x ray's hits, the reference's casts where that cannot decide (kind 3).  The
+ * reference keeps no such counters; they give the traversal's algorithmic
+ * bytes (32 per node, 52 per triangle: index, face and three vertices). */
 mupsi_status mupsi_mesh_query_cost(mupsi_scene* scene, int32_t prim, int32_t kind,
                                    const float* px, const float* py, const float* pz,
                                    int64_t n, uint32_t* nodes, uint32_t* tris);

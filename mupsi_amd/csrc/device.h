@@ -553,9 +553,62 @@ __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cn
   return norm(sub(out, p));
 }
 // Mesh::inside (mesh.cpp:341-362): parity of +x crossings, restarting 1e-4 past each hit.
-template <class C = NoCount>
+//
+// The reference casts the +x ray once per crossing, each cast from 1e-4 past the
+// previous hit.  One traversal collecting every hit gives the same count when
+// (a) each hit t lies in its leaf's x-slab [bmin.x - p.x, bmax.x - p.x] as the
+// slab test computes it, so no cast culls the leaf of the hit it must find (its
+// ancestors' slabs contain the leaf's); (b) no two hits are within 2e-4, so each
+// cast's nearT = h + 1e-4 lies strictly between consecutive hits and the cast
+// finds exactly the next one; (c) the stack never ran out (the casts, culling
+// more, then never run out either).  Otherwise the reference's loop of casts
+// runs as written.
+// kOneTraversal false: the reference's loop of casts only (its traversal cost).
+template <bool kOneTraversal = true, class C = NoCount>
 __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
   const F3 pl = xf_to_local(m.xf, p);
+  if (m.n_nodes == 0) return false;
+  if constexpr (kOneTraversal) {
+    const Ray r = make_ray(pl, mk(1, 0, 0));
+    constexpr int kHits = 8;
+    float hs[kHits];
+    int nh = 0;
+    bool ok = true;
+    uint32_t st[kStack];
+    int sp = 0;
+    st[sp++] = m.root;
+    while (sp > 0 && ok) {
+      const uint32_t ni = st[--sp];
+      const BvhNode nd = load_node(m.nodes, ni);
+      cnt.node();
+      if (!aabb_hit(nd, r)) continue;
+      if (nd.right & kLeafFlag) {
+        const float lo = nd.bmin[0] - pl.x, hi = nd.bmax[0] - pl.x;  // the slab test's x range
+        const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+        for (uint32_t i = s0; i < s0 + c; ++i) {
+          cnt.tri();
+          F3 v0, v1, v2;
+          uint32_t id;
+          leaf_tri(m, i, v0, v1, v2, id);
+          float t, uu, vv;
+          if (ray_tri(r, v0, v1, v2, t, uu, vv)) {
+            ok = ok && t >= lo && t <= hi && nh < kHits;
+#pragma unroll
+            for (int j = 0; j < kHits; ++j) {
+              if (j < nh && fabsf(t - hs[j]) <= 2e-4f + 1e-6f * fabsf(t)) ok = false;
+              if (j == nh) hs[j] = t;
+            }
+            ++nh;
+          }
+        }
+      } else if (sp <= kStack - 2) {
+        push_children(m, nd, ni, r, st, sp);
+      } else {
+        ok = false;
+      }
+    }
+    if (ok) return (nh & 1) == 1;
+  }
   Ray r = make_ray(pl, mk(1, 0, 0));
   int hits = 0;
   uint32_t ti;

@@ -2292,8 +2292,10 @@ __global__ __launch_bounds__(256) void k_mesh_cost(DevMesh m, int kind, const fl
     F3 o = mk(0, 0, 0);
     if (kind == 0) (void)mesh_dist(m, p, ti, o, c);
     else (void)mesh_dist_grid(m, p, ti, o, c);
+  } else if (kind == 1) {
+    (void)mesh_inside<false>(m, p, c);  // the reference's casts
   } else {
-    (void)mesh_inside(m, p, c);
+    (void)mesh_inside<true>(m, p, c);   // as the library runs it
   }
   nodes[i] = c.nodes;
   tris[i] = c.tris;
@@ -3736,7 +3738,7 @@ extern "C" mupsi_status mupsi_mesh_query_cost(mupsi_scene* sc, int32_t prim, int
                                               int64_t n, uint32_t* nodes, uint32_t* tris) {
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: no such mesh");
-  if (kind < 0 || kind > 2) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: kind must be 0, 1 or 2");
+  if (kind < 0 || kind > 3) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: kind must be 0 to 3");
   if (n <= 0) return MUPSI_OK;
   if (!px || !py || !pz || !nodes || !tris)
     return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: null buffer");

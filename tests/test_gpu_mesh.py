@@ -116,3 +116,8 @@ def test_traversal_cost_equals_oracle(ctx, dev, prim):
         assert tris.sum() > 20000
     batch.mesh_closest(s, prim, pt)
     assert batch.last_kernel_ms(ctx) > 0.0
+    # inside() as the library runs it: one traversal collecting the hits (kind 3)
+    # tests fewer nodes in total than the reference's loop of casts (kind 1)
+    n3, t3 = (x.cpu().numpy().astype(np.int64) for x in batch.mesh_query_cost(s, prim, 3, pt))
+    n1, t1 = (x.cpu().numpy().astype(np.int64) for x in batch.mesh_query_cost(s, prim, 1, pt))
+    assert 0 < n3.sum() < n1.sum() and t3.sum() > 0
