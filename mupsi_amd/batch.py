@@ -166,7 +166,7 @@ def mesh_query_cost(scene, prim: int, kind: int, p):
     """Per point: BVH nodes tested and triangles tested by distToPoint (kind 0),
     inside as the reference's loop of casts (kind 1), distToPoint from the
     distance-grid bound (kind 2) or inside as the library runs it (kind 3);
-    int32 tensors (n,)."""
+    int64 tensors (n,)."""
     torch = _torch()
     n = p.shape[0]
     px, py, pz = (p[:, i].contiguous() for i in range(3))
