@@ -1133,6 +1133,19 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   // entered-cell pattern | window residues | points per cell; cell size
   __shared__ int4 regen_cell[NG];
   __shared__ int2 regen_aux[NG];
+  // The regeneration tables, copied from constant memory once per block where
+  // the window is small: a global load in every regenerating pass put two memory
+  // round trips on the latency of a pass.  With the larger windows (mesh means,
+  // batched launches) the extra LDS would cost resident blocks instead.
+  constexpr bool kLdsTables = !kMesh && !kMulti && sizeof(win) <= 8192;
+  __shared__ uint8_t new_off[kLdsTables ? 28 * 27 : 1];
+  __shared__ ulonglong2 jump[kLdsTables ? kMaxPpcCached : 1];
+  if constexpr (kLdsTables) {
+    for (int i = threadIdx.x; i < 28 * 27; i += 64) new_off[i] = kNewCells.off[i / 27][i % 27];
+    if (threadIdx.x < kMaxPpcCached)
+      jump[threadIdx.x] = make_ulonglong2(kJump.a[threadIdx.x], kJump.c[threadIdx.x]);
+    // (the first pass's barrier orders these writes before any read)
+  }
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
   const int grp = lane / GS;
@@ -1287,7 +1300,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     if (__ballot(phase != 3) == 0) break;
 
     // Window: cells entering the 27-cell neighbourhood of the evaluation point.
-    int newcells = -1;
+    int newcells = -1, nnew = 0;
     int cx = 0, cy = 0, cz = 0;
     if (do_eval) {
       cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
@@ -1299,18 +1312,20 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
                cell_coord(q2.z, G, cc.inv_kr, cc.kr_pow2) == cz;
       if (!wvalid) {
         newcells = 27;
+        nnew = 27;
       } else {
         const int ex = cx - wx, ey = cy - wy, ez = cz - wz;
         if (ex | ey | ez) {
-          newcells = (abs(ex) <= 1 && abs(ey) <= 1 && abs(ez) <= 1) ? (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1)
-                                                                   : 27;
+          const bool adj = abs(ex) <= 1 && abs(ey) <= 1 && abs(ez) <= 1;
+          newcells = adj ? (ex + 1) * 9 + (ey + 1) * 3 + (ez + 1) : 27;
+          nnew = adj ? new_cell_count(ex, ey, ez) : 27;
         }
       }
     }
     // Regeneration of the entered cells of every group of the wave as one pool of
     // items over the 64 lanes (item j of group g: impulse i of entered cell k),
     // written to the same window slots a group's own lanes would write.
-    const int items = newcells >= 0 ? kNewCells.n[newcells] * ppc : 0;
+    const int items = nnew * ppc;
     if (gl == 0 && newcells >= 0) {
       regen_cell[grp] = make_int4(cx, cy, cz, (int)seed);
       regen_aux[grp] = make_int2(newcells | mod3(cx) << 5 | mod3(cy) << 7 | mod3(cz) << 9 | ppc << 11,
@@ -1331,14 +1346,15 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       const int it = j - start[g];
       const int k = it / gppc;
       const int i = it - k * gppc;
-      const int no = kNewCells.off[ra.x & 31][k];
+      const int no = kLdsTables ? new_off[(ra.x & 31) * 27 + k] : kNewCells.off[ra.x & 31][k];
       const int ox = (no & 3) - 1, oy = ((no >> 2) & 3) - 1, oz = (no >> 4) - 1;  // in {-1,0,1}
       int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
       rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
       ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
       rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
-      win[g][(rx * 9 + ry * 3 + rz) * gppc + i] =
-          make_impulse(__int_as_float(ra.y), rc.x + ox, rc.y + oy, rc.z + oz, (uint32_t)rc.w, i);
+      const ulonglong2 jm = kLdsTables ? jump[i] : make_ulonglong2(kJump.a[i], kJump.c[i]);
+      win[g][(rx * 9 + ry * 3 + rz) * gppc + i] = make_impulse(
+          __int_as_float(ra.y), rc.x + ox, rc.y + oy, rc.z + oz, (uint32_t)rc.w, jm.x, jm.y);
     }
     if (newcells >= 0) {
       wx = cx;

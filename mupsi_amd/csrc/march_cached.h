@@ -87,6 +87,19 @@ constexpr NewCells make_new_cells() {
 }
 __constant__ constexpr NewCells kNewCells = make_new_cells();
 
+// Number of cells entered for centre move delta: 27 minus the cells that stay,
+// 2 per axis that moves and 3 per axis that does not (27 for a refill).
+constexpr int new_cell_count(int ex, int ey, int ez) {
+  return 27 - (ex ? 2 : 3) * (ey ? 2 : 3) * (ez ? 2 : 3);
+}
+constexpr bool new_cell_counts_match() {
+  const NewCells t = make_new_cells();
+  for (int di = 0; di < 27; ++di)
+    if (new_cell_count(di / 9 - 1, (di / 3) % 3 - 1, di % 3 - 1) != t.n[di]) return false;
+  return t.n[27] == 27;
+}
+static_assert(new_cell_counts_match(), "entered-cell count formula");
+
 __device__ __forceinline__ int mod3(int n) {
   const int r = n % 3;
   return r < 0 ? r + 3 : r;
@@ -103,23 +116,20 @@ __device__ __forceinline__ int cell_coord(float q, const DevGP& g, float inv_kr,
   return (int)(q / g.kr);
 }
 
-// Impulse i of cell (nx, ny, nz) of realization seed as (position, weight); kr is
-// the cell size.
+// Impulse of cell (nx, ny, nz) of realization seed as (position, weight); kr is
+// the cell size, (ja, jc) the jump to the impulse's first draw (kJump.a[i],
+// kJump.c[i] for impulse i).
 __device__ __forceinline__ float4 make_impulse(float kr, int nx, int ny, int nz, uint32_t seed,
-                                               int i) {
+                                               uint64_t ja, uint64_t jc) {
   const uint32_t cs = make_seed4((uint32_t)nx, (uint32_t)ny, (uint32_t)nz, seed);
   uint64_t s = (uint32_t)(cs + 1u);
-  s = s * kJump.a[i] + kJump.c[i];
+  s = s * ja + jc;
   const uint32_t r0 = pcg_next(s);
   const uint32_t r1 = pcg_next(s);
   const uint32_t r2 = pcg_next(s);
   const uint32_t r3 = pcg_next(s);
   return make_float4(fmaf(u01(r2), kr, (float)nx * kr), fmaf(u01(r1), kr, (float)ny * kr),
                      fmaf(u01(r0), kr, (float)nz * kr), (r3 >> 31) ? 1.0f : -1.0f);
-}
-__device__ __forceinline__ float4 make_impulse(const DevGP& g, int nx, int ny, int nz,
-                                               uint32_t seed, int i) {
-  return make_impulse(g.kr, nx, ny, nz, seed, i);
 }
 
 }  // namespace dev
