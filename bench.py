@@ -41,7 +41,7 @@ FP32_PEAK_TFLOPS = 157.3
 # WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`, FETCH_SIZE scaled by
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
 # profiles/k_march_window_sphere_renewal_256x256x64.md holds the table.
-MARCH_TRAFFIC_BYTES = 94_062_926
+MARCH_TRAFFIC_BYTES = 92_911_236
 CFG = 2
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "
