@@ -42,7 +42,7 @@ FP32_PEAK_TFLOPS = 157.3
 # 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
 # profiles/k_march_window_sphere_renewal_256x256x64.md holds the table.
 MARCH_TRAFFIC_BYTES = 92_911_236
-CFG = 2
+PRESET = "sphere_renewal_256"
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "
             "mean sphere r70 @ (0,0,-500) in a null r130 sphere, seeded 64x32 env map")
@@ -117,7 +117,7 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
     import numpy as np
     import torch
     from mupsi_amd import Guide, Scene, scenes
-    sc = scenes.config_scene(5)
+    sc = scenes.preset_scene("guided_sphere_quad_1024")
     W, H = sc["width"], sc["height"]
     plain = Scene(ctx, sc)
     plain.render_passes(0, 1)                         # allocations outside the timing
@@ -265,12 +265,13 @@ def traversal_bench(ctx, sc: dict, n: int = 1 << 22) -> dict:
     return out
 
 
-def configs_bench(ctx, cfg3_passes: int = 4, cfg4_div: int = 16) -> dict:
-    """Throughput on the other configurations: 1 (single_realization, 128x128x16,
-    every pass), 3 (mesh mean, 512x512, the first cfg3_passes of 64 passes: per-
-    path cost does not depend on the pass), 4 (64 generated scenes, the first
-    spp/cfg4_div passes of each, rendered scene by scene and as one autotuned
-    batch; the two must give identical images)."""
+def configs_bench(ctx, mesh_passes: int = 4, batch_div: int = 16) -> dict:
+    """Throughput on the other workloads: sphere_single_128 (single_realization,
+    128x128x16, every pass), icosphere_renewal_512 (mesh mean, 512x512, the first
+    mesh_passes of 64 passes: per-path cost does not depend on the pass) and
+    generated_batch (64 generated scenes, the first spp/batch_div passes of each,
+    rendered scene by scene and as one autotuned batch; the two must give
+    identical images)."""
     import numpy as np
     import torch
     from mupsi_amd import Batch, Scene, scenes
@@ -283,38 +284,38 @@ def configs_bench(ctx, cfg3_passes: int = 4, cfg4_div: int = 16) -> dict:
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    sc1 = scenes.config_scene(1)
+    sc1 = scenes.preset_scene("sphere_single_128")
     s1 = Scene(ctx, sc1)
     s1.render_passes(100, 1)
     s1.reset()
     n1 = sc1["width"] * sc1["height"] * sc1["spp"]
     secs = timed(lambda: s1.render_passes(0, sc1["spp"]))
-    out["cfg1"] = {"workload": "sphere mean, single_realization, 128x128, 16 spp",
+    out["sphere_single_128"] = {"workload": "sphere mean, single_realization, 128x128, 16 spp",
                    "paths": n1, "s": secs, "Mpaths_s": n1 / secs / 1e6}
     s1.close()
-    print(f"bench: cfg1 {out['cfg1']}", file=sys.stderr, flush=True)
+    print(f"bench: sphere_single_128 {out['sphere_single_128']}", file=sys.stderr, flush=True)
 
-    sc3 = scenes.config_scene(3)
+    sc3 = scenes.preset_scene("icosphere_renewal_512")
     t0 = time.perf_counter()
     s3 = Scene(ctx, sc3)
     upload = time.perf_counter() - t0
     s3.render_passes(1000, 1)
     s3.reset()
-    n3 = sc3["width"] * sc3["height"] * cfg3_passes
-    secs = timed(lambda: s3.render_passes(0, cfg3_passes))
+    n3 = sc3["width"] * sc3["height"] * mesh_passes
+    secs = timed(lambda: s3.render_passes(0, mesh_passes))
     st = s3.stats()
-    out["cfg3"] = {"workload": "displaced icosphere mean (327,680 triangles), renewal_plus, "
-                               f"512x512, passes 0..{cfg3_passes - 1} of 64",
+    out["icosphere_renewal_512"] = {"workload": "displaced icosphere mean (327,680 triangles), renewal_plus, "
+                               f"512x512, passes 0..{mesh_passes - 1} of 64",
                    "paths": n3, "s": secs, "Mpaths_s": n3 / secs / 1e6,
                    "upload_s": upload, "march_kernel_ms": st["march_kernel_ms"],
                    "field_evals": st["field_evals"]}
     s3.close()
-    print(f"bench: cfg3 {out['cfg3']}", file=sys.stderr, flush=True)
+    print(f"bench: icosphere_renewal_512 {out['icosphere_renewal_512']}", file=sys.stderr, flush=True)
     out["bvh_traversal"] = traversal_bench(ctx, sc3)
     print(f"bench: bvh {out['bvh_traversal']}", file=sys.stderr, flush=True)
 
-    scs = scenes.config_batch()
-    kc = [max(1, sc["spp"] // cfg4_div) for sc in scs]
+    scs = scenes.generated_batch()
+    kc = [max(1, sc["spp"] // batch_div) for sc in scs]
     n4 = sum(sc["width"] * sc["height"] * k for sc, k in zip(scs, kc))
     t0 = time.perf_counter()
     seq = [Scene(ctx, sc) for sc in scs]
@@ -334,8 +335,8 @@ def configs_bench(ctx, cfg3_passes: int = 4, cfg4_div: int = 16) -> dict:
     b.close()
     for x in seq:
         x.close()
-    out["cfg4"] = {"workload": f"64 generated scenes (sphere or icosphere means, kr logU[0.05,5], "
-                               f"ppc 1..8, 64..512 px), the first spp/{cfg4_div} passes of each",
+    out["generated_batch"] = {"workload": f"64 generated scenes (sphere or icosphere means, kr logU[0.05,5], "
+                               f"ppc 1..8, 64..512 px), the first spp/{batch_div} passes of each",
                    "paths": n4, "upload_s": upload,
                    "sequential": {"s": secs_seq, "Mpaths_s": n4 / secs_seq / 1e6},
                    "batched": {"s": secs_b, "Mpaths_s": n4 / secs_b / 1e6,
@@ -344,7 +345,7 @@ def configs_bench(ctx, cfg3_passes: int = 4, cfg4_div: int = 16) -> dict:
                                "march_blocks_per_cu": tun["march_blocks_per_cu"],
                                "use_graph": tun["use_graph"]},
                    "batched_equals_sequential": bool(same)}
-    print(f"bench: cfg4 {out['cfg4']}", file=sys.stderr, flush=True)
+    print(f"bench: generated_batch {out['generated_batch']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -356,7 +357,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the configuration 1 / 3 / 4 and BVH traversal measurements")
+                    help="skip the sphere_single_128, icosphere_renewal_512, generated_batch and BVH traversal measurements")
     ap.add_argument("--only", choices=["guiding", "configs", "bvh"], default=None,
                     help="run one section alone and print its JSON (for profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -380,12 +381,12 @@ def main() -> int:
         elif args.only == "configs":
             out = configs_bench(ctx)
         else:
-            out = traversal_bench(ctx, scenes.config_scene(3))
+            out = traversal_bench(ctx, scenes.preset_scene("icosphere_renewal_512"))
         print(json.dumps(out), flush=True)
         ctx.close()
         return 0
 
-    scene = scenes.config_scene(CFG)
+    scene = scenes.preset_scene(PRESET)
     W, H, spp = scene["width"], scene["height"], scene["spp"]
     ctx = Context(0)
     s = Scene(ctx, scene)

@@ -2,8 +2,8 @@
 // directional mixture, shared by the MLP kernels (guide.hip) and the tracer
 // (kernels.hip).
 //
-// The reference has no guiding; this is the north-star addition on top of its
-// trace loop.  At a guided vertex (a GP-medium hit, whose BSDF is the fixed
+// The reference has no guiding; this is an addition on top of its trace loop
+// (the plain tracer stays available and unchanged).  At a guided vertex (a GP-medium hit, whose BSDF is the fixed
 // Lambertian 0.8 of gpmedium.h:46-48) the next direction is drawn by one-sample
 // MIS from
 //     q(w) = alpha * p_guide(w | x) + (1 - alpha) * cos(w, n) / pi,

@@ -1,14 +1,16 @@
-"""Procedural inputs of the benchmark configurations (seeded, bit-reproducible).
+"""Procedural scenes of the benchmark workloads (seeded, bit-reproducible).
 
 - `env_map`: the 64x32 FP32 environment map: a sky gradient plus one Gaussian
   sun lobe of peak 50.
 - `icosphere`: a radius-1 icosphere subdivided n times, optionally displaced
   along the vertex normals by seeded 3-octave value noise.
-- `config_batch(seed)`: the 64 uneven scenes of the batched configuration.
-- `config_scene(i)`: the scene dicts of configurations 1-3 and 5 (sphere mean,
-  single realization / renewal_plus; displaced-icosphere mesh mean; the
-  renewal_plus sphere scene with a mirror sphere and a small emissive quad,
-  the learned-guiding workload).
+- `generated_batch(seed)`: 64 uneven generated scenes for the batched renderer.
+- `preset_scene(name)`: the named scene dicts (PRESETS): sphere_single_128 (sphere
+  mean, single realization, 128x128, 16 spp), sphere_renewal_256 (sphere mean,
+  renewal_plus, 256x256, 64 spp), icosphere_renewal_512 (displaced-icosphere mesh
+  mean, 512x512) and guided_sphere_quad_1024 (the renewal_plus sphere scene with a
+  mirror sphere and a small emissive quad at 1024x1024, the learned-guiding
+  workload).
 """
 from __future__ import annotations
 
@@ -130,12 +132,16 @@ def ground_quad(y: float, half: float = 40.0, z0: float = -60.0, z1: float = 10.
 CAMERA = {"pos": [0.0, 0.0, 0.0], "dir": [0.0, 0.0, -1.0], "up": [0.0, 1.0, 0.0], "fov": 45.0}
 
 
-def config_scene(cfg: int, *, width: int | None = None, height: int | None = None,
+PRESETS = ("sphere_single_128", "sphere_renewal_256", "icosphere_renewal_512",
+           "guided_sphere_quad_1024")
+
+
+def preset_scene(name: str, *, width: int | None = None, height: int | None = None,
                  mesh_subdivisions: int = 7) -> dict:
-    """Scene dicts of the benchmark configurations (spp is stored under "spp")."""
+    """Scene dict of a named benchmark workload (spp is stored under "spp")."""
     sky = env_map()
-    if cfg in (1, 2):
-        single = cfg == 1
+    if name in ("sphere_single_128", "sphere_renewal_256"):
+        single = name == "sphere_single_128"
         res = 128 if single else 256
         return {
             "width": width or res, "height": height or res,
@@ -149,7 +155,7 @@ def config_scene(cfg: int, *, width: int | None = None, height: int | None = Non
             "primitives": [{"type": "sphere", "center": [0, 0, -500], "radius": 130,
                             "bsdf": "null", "int_medium": True}],
         }
-    if cfg == 3:
+    if name == "icosphere_renewal_512":
         return {
             "width": width or 512, "height": height or 512, "spp": 64, "max_bounce": 10,
             "max_medium_bounce": 10, "gp_mode": "renewal_plus", "camera": dict(CAMERA),
@@ -165,8 +171,8 @@ def config_scene(cfg: int, *, width: int | None = None, height: int | None = Non
                  "texture": np.full((1, 1, 3), 0.7, np.float32)},
             ],
         }
-    if cfg == 5:
-        sc = config_scene(2, width=width or 1024, height=height or 1024)
+    if name == "guided_sphere_quad_1024":
+        sc = preset_scene("sphere_renewal_256", width=width or 1024, height=height or 1024)
         sc["spp"] = 256
         # mesh emission is an extension of the reference schema (it ignores the key)
         sc["extensions"] = {"mesh_emission": True}
@@ -179,7 +185,7 @@ def config_scene(cfg: int, *, width: int | None = None, height: int | None = Non
              "emission": [200.0, 200.0, 200.0]},
         ]
         return sc
-    raise ValueError(f"configuration {cfg} has no scene generator here")
+    raise ValueError(f"no preset scene named {name!r} (one of {PRESETS})")
 
 
 def sphere_sdf_grid(res: int, bmin, bmax, center, radius: float) -> np.ndarray:
@@ -195,12 +201,12 @@ def sphere_sdf_grid(res: int, bmin, bmax, center, radius: float) -> np.ndarray:
 
 def medium_scene(kind: str, *, width: int = 128, height: int = 128, mean: str = "sphere",
                  noise_variance: float = 25.0, res: int = 48) -> dict:
-    """The configuration-2 layout (enclosing null sphere r130 at z = -500, env
+    """The sphere_renewal_256 layout (enclosing null sphere r130 at z = -500, env
     map) with an SDFMedium ("sdf") or ExternalGPMedium ("grid") inside instead of
     the GP medium.  sdf: the mean sphere r70 (or, mean="mesh", a displaced
     icosphere of radius 70); grid: the sphere's signed distance on a res^3 grid
     with a constant variance grid (noise_scale 1)."""
-    sc = config_scene(2, width=width, height=height)
+    sc = preset_scene("sphere_renewal_256", width=width, height=height)
     sc.pop("gp_medium")
     sc.pop("kernel")
     sc["spp"] = 16
@@ -234,11 +240,11 @@ def emissive_quad(center, side: float) -> dict:
 BATCH_SEED = 4
 
 
-def config_batch(seed: int = BATCH_SEED, n: int = 64, max_res: int = 512,
-                 max_subdivisions: int = 7) -> list[dict]:
-    """The batched configuration: n seeded renewal_plus scenes, sphere means (the
-    configuration-2 geometry) and displaced-icosphere mesh means (the
-    configuration-3 geometry) alternating at random; square resolution uniform
+def generated_batch(seed: int = BATCH_SEED, n: int = 64, max_res: int = 512,
+                    max_subdivisions: int = 7) -> list[dict]:
+    """The batched workload: n seeded renewal_plus scenes, sphere means (the
+    sphere_renewal_256 geometry) and displaced-icosphere mesh means (the
+    icosphere_renewal_512 geometry) alternating at random; square resolution uniform
     in 64..max_res, spp uniform in 16..128, kernel length_scale (the cell size)
     log-uniform in [0.05, 5], sigma uniform in [0.1, 3], points_per_cell uniform
     in 1..8, icosphere subdivisions uniform in 2..max_subdivisions."""
@@ -253,7 +259,7 @@ def config_batch(seed: int = BATCH_SEED, n: int = 64, max_res: int = 512,
                   "length_scale": float(np.exp(rng.uniform(np.log(0.05), np.log(5.0)))),
                   "aniso": [1, 1, 1], "points_per_cell": int(rng.integers(1, 9))}
         if rng.uniform() < 0.5:
-            sc = config_scene(2, width=res, height=res)
+            sc = preset_scene("sphere_renewal_256", width=res, height=res)
             sc["kernel"] = kernel
         else:
             sub = int(rng.integers(2, max_subdivisions + 1))

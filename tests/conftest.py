@@ -30,13 +30,13 @@ def dev():
 
 
 def kernels():
-    """Kernel parameter sets of the benchmark configurations plus edge cases."""
+    """Kernel parameter sets of the benchmark workloads plus edge cases."""
     return [
-        {"sigma": 1.0, "length_scale": 1.0, "aniso": [1, 1, 1], "points_per_cell": 3},   # cfg 1
-        {"sigma": 1.5, "length_scale": 2.0, "aniso": [1, 1, 4], "points_per_cell": 3},   # cfg 2/5
-        {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},   # cfg 3
-        {"sigma": 2.7, "length_scale": 0.37, "aniso": [1, 1, 1], "points_per_cell": 8},  # cfg 4 range
-        {"sigma": 0.1, "length_scale": 4.6, "aniso": [1, 1, 1], "points_per_cell": 1},   # cfg 4 range
+        {"sigma": 1.0, "length_scale": 1.0, "aniso": [1, 1, 1], "points_per_cell": 3},   # sphere_single_128
+        {"sigma": 1.5, "length_scale": 2.0, "aniso": [1, 1, 4], "points_per_cell": 3},   # sphere_renewal_256, guided_sphere_quad_1024
+        {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},   # icosphere_renewal_512
+        {"sigma": 2.7, "length_scale": 0.37, "aniso": [1, 1, 1], "points_per_cell": 8},  # generated_batch range
+        {"sigma": 0.1, "length_scale": 4.6, "aniso": [1, 1, 1], "points_per_cell": 1},   # generated_batch range
     ]
 
 

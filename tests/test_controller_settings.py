@@ -15,7 +15,7 @@ def test_defaults_of_an_empty_scene_object():
 
 
 def test_sphere_gp_medium_and_kernel_keys():
-    sc = scenes.config_scene(2)
+    sc = scenes.preset_scene("sphere_renewal_256")
     st = load_settings(sc)
     assert st["spp"] == 64 and st["max_bounce"] == 10 and st["gp_mode"] == "renewal_plus"
     assert st["mean_type"] == "sphere" and st["mean_center"] == [0.0, 0.0, -500.0]

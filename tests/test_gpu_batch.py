@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _small_batch():
-    scs = scenes.config_batch(seed=11, n=6, max_res=48, max_subdivisions=3)
+    scs = scenes.generated_batch(seed=11, n=6, max_res=48, max_subdivisions=3)
     for i, sc in enumerate(scs):       # cover ppc 1..8 on sphere and mesh means
         sc["kernel"]["points_per_cell"] = [1, 8, 3, 5, 2, 7][i]
     return scs

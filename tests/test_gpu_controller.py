@@ -34,8 +34,8 @@ def _edit(sc, kernel=None, **top):
     return out
 
 
-def _cfg2(res=48):
-    return scenes.config_scene(2, width=res, height=res)
+def _sphere_scene(res=48):
+    return scenes.preset_scene("sphere_renewal_256", width=res, height=res)
 
 
 def _controller(ctx, sc):
@@ -48,7 +48,7 @@ def _controller(ctx, sc):
 def test_renderer_calls_back_after_every_pass(ctx):
     """Renderer::startRenderProgressive: on_spp(k) after pass k with the k-pass mean."""
     from mupsi_amd import Renderer, Scene
-    sc = _cfg2(32)
+    sc = _sphere_scene(32)
     s = Scene(ctx, sc)
     r = Renderer(s)
     r.prepare_render()
@@ -60,7 +60,7 @@ def test_renderer_calls_back_after_every_pass(ctx):
 
 
 def test_progressive_applies_gp_medium_with_isotropic_kernel(ctx):
-    sc = _cfg2()
+    sc = _sphere_scene()
     assert sc["kernel"]["aniso"] != [1, 1, 1]
     c = _controller(ctx, sc)
     assert c.has_gp_medium() and c.gp_mode() == "renewal_plus" and c.gp_points_per_cell() == 3
@@ -91,7 +91,7 @@ def test_progressive_applies_gp_medium_with_isotropic_kernel(ctx):
 def test_full_render_keeps_the_loaded_medium_until_a_progressive_run(ctx):
     """start() renders spp passes without applyGpMedium; the medium a progressive
     run applied stays on the scene for the next full render."""
-    sc = _cfg2(40)
+    sc = _sphere_scene(40)
     sc["spp"] = 3
     c = _controller(ctx, sc)
     assert c.get_spp() == 3
@@ -109,7 +109,7 @@ def test_full_render_keeps_the_loaded_medium_until_a_progressive_run(ctx):
 
 
 def test_cancel_keeps_the_last_completed_pass(ctx):
-    sc = _cfg2(64)
+    sc = _sphere_scene(64)
     c = _controller(ctx, sc)
     c.start_progressive(1_000_000)
     t0 = time.time()
@@ -125,7 +125,7 @@ def test_cancel_keeps_the_last_completed_pass(ctx):
 
 
 def test_gp_setters_and_skip_medium_apply_on_the_next_run(ctx):
-    sc = _cfg2()
+    sc = _sphere_scene()
     c = _controller(ctx, sc)
     c.set_gp_kernel_sigma(0.8)
     c.set_gp_kernel_length(3.0)
@@ -147,7 +147,7 @@ def test_gp_setters_and_skip_medium_apply_on_the_next_run(ctx):
 def test_mesh_mean_from_scene_json_is_reloaded(ctx, tmp_path):
     """applyGpMedium re-reads the mean mesh file and rebuilds its device data."""
     from mupsi_amd.scene import write_scene_files
-    sc = scenes.config_scene(3, width=40, height=40, mesh_subdivisions=4)
+    sc = scenes.preset_scene("icosphere_renewal_512", width=40, height=40, mesh_subdivisions=4)
     path = write_scene_files(sc, tmp_path)
     c = _controller(ctx, str(path))
     assert c.config_path() == str(path)
@@ -162,7 +162,7 @@ def test_mesh_mean_from_scene_json_is_reloaded(ctx, tmp_path):
 def test_set_gp_medium_switches_mean_kind(ctx):
     """mupsi_scene_set_gp_medium from a sphere mean to a mesh mean and back."""
     from mupsi_amd import Scene
-    sc = _cfg2(40)
+    sc = _sphere_scene(40)
     mesh = scenes.icosphere(3, displacement=0.05)
     tf = {"scale": 60.0, "position": [0.0, 0.0, -500.0]}
     s = Scene(ctx, sc)
@@ -180,7 +180,7 @@ def test_set_gp_medium_switches_mean_kind(ctx):
 
 def test_set_gp_medium_refusals_leave_the_scene_unchanged(ctx):
     from mupsi_amd import Batch, MupsiError, Scene
-    sc = _cfg2(32)
+    sc = _sphere_scene(32)
     s = Scene(ctx, sc)
     with pytest.raises(MupsiError):
         s.set_gp_medium("sphere", [0, 0, -500], 70.0, kernel=dict(sc["kernel"], points_per_cell=0))
@@ -208,7 +208,7 @@ def test_display_files_equal_the_cli_output(ctx, tmp_path):
     from mupsi_amd.scene import write_scene_files
     exe = ROOT / "mupsi_amd" / "mupsi"
     assert exe.exists(), "the mupsi CLI is built with the library"
-    sc = _cfg2(40)
+    sc = _sphere_scene(40)
     sc["spp"] = 3
     path = write_scene_files(sc, tmp_path)
     for ext in ("png", "pfm"):

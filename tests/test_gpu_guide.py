@@ -152,7 +152,7 @@ def test_training_learns_a_directional_distribution(ctx, dev):
 
 
 def _scene5(res):
-    return scenes.config_scene(5, width=res, height=res)
+    return scenes.preset_scene("guided_sphere_quad_1024", width=res, height=res)
 
 
 def _chunk_means(s, mask, chunks, spp, k0=0):
@@ -195,7 +195,7 @@ def test_guided_tracer_with_alpha_zero_weights_like_the_plain_tracer(ctx):
     """alpha = 0: every guided sample is the cosine sample with weight albedo, so the
     image converges to the plain one; with the same passes the sky pixels agree."""
     from mupsi_amd import Guide, Scene
-    sc = scenes.config_scene(2, width=32, height=32)
+    sc = scenes.preset_scene("sphere_renewal_256", width=32, height=32)
     s = Scene(ctx, sc)
     g = Guide.for_scene(ctx, sc, alpha=0.0, train_passes=0)
     s.set_guide(g)

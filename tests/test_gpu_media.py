@@ -114,7 +114,7 @@ def test_batch_with_sdf_and_grid_media_equals_single_renders(ctx):
     from mupsi_amd import Batch, Scene
     scs = [scenes.medium_scene("sdf", width=48, height=48),
            scenes.medium_scene("grid", width=40, height=40),
-           scenes.config_scene(2, width=32, height=32)]
+           scenes.preset_scene("sphere_renewal_256", width=32, height=32)]
     passes = [3, 2, 4]
     single = [Scene(ctx, sc) for sc in scs]
     for s, k in zip(single, passes):

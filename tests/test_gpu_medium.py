@@ -8,8 +8,8 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _scene(cfg):
-    sc = scenes.config_scene(cfg, width=32, height=32)
+def _scene(name):
+    sc = scenes.preset_scene(name, width=32, height=32)
     return sc
 
 
@@ -32,14 +32,14 @@ def _cond_dev(cond, dev):
             "g": c[:, 5:8].contiguous()}
 
 
-@pytest.mark.parametrize("cfg", [1, 2])
-def test_field_eval_conditioned(ctx, dev, cfg):
+@pytest.mark.parametrize("name,k", [("sphere_single_128", 1), ("sphere_renewal_256", 2)])
+def test_field_eval_conditioned(ctx, dev, name, k):
     import torch
     from mupsi_amd import Scene, batch
-    sc = _scene(cfg)
+    sc = _scene(name)
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
-    rng = np.random.default_rng(cfg)
+    rng = np.random.default_rng(k)
     n = 8000
     dirs = rng.normal(size=(n, 3))
     dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
@@ -77,18 +77,18 @@ def _rays_into_medium(rng, n, center, r_enclose, r_mean):
     return rays
 
 
-@pytest.mark.parametrize("cfg", [1, 2])
-def test_march_hits_bit_exact(ctx, dev, cfg):
+@pytest.mark.parametrize("name,k", [("sphere_single_128", 1), ("sphere_renewal_256", 2)])
+def test_march_hits_bit_exact(ctx, dev, name, k):
     import torch
     from mupsi_amd import Scene, batch
-    sc = _scene(cfg)
+    sc = _scene(name)
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
-    rng = np.random.default_rng(10 + cfg)
+    rng = np.random.default_rng(10 + k)
     n = 600
     rays = _rays_into_medium(rng, n, [0, 0, -500.0], 130.0, 70.0)
     seeds = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
-    cond = _cond_states(rng, n, [0, 0, -500.0], 70.0, sc["kernel"]) if cfg == 2 else None
+    cond = _cond_states(rng, n, [0, 0, -500.0], 70.0, sc["kernel"]) if name == "sphere_renewal_256" else None
     ref32 = osc.march(rays, seeds, cond, fp64=False, track=False)
     ref64 = osc.march(rays, seeds, cond, fp64=True, track=True)
     got = batch.march(s, torch.from_numpy(rays).to(dev), torch.from_numpy(seeds.view(np.int32)).to(dev),
@@ -100,7 +100,7 @@ def test_march_hits_bit_exact(ctx, dev, cfg):
     # zero (relative to |mu| + L1 mass).  Cell indices are FP32 decisions on both sides
     # (the FP64 oracle shares the FP32 grid points), so cell boundaries are not ambiguous.
     unamb = ref64["amb"] > 2e-6
-    print(f"cfg{cfg}: unambiguous {unamb.mean():.3f}, hits {int(hit.sum())}")
+    print(f"{name}: unambiguous {unamb.mean():.3f}, hits {int(hit.sum())}")
     # decisions: identical to the FP64 oracle on every unambiguous ray
     same_hit = hit == ref64["hit"]
     same_t = (~hit.astype(bool)) | (t.view(np.uint32) == ref64["t"].view(np.uint32))
@@ -121,7 +121,7 @@ def test_march_hits_bit_exact(ctx, dev, cfg):
 def test_renewal_solve_fp64(ctx, dev):
     import torch
     from mupsi_amd import Scene, batch
-    sc = _scene(2)
+    sc = _scene("sphere_renewal_256")
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
     rng = np.random.default_rng(5)
@@ -146,7 +146,7 @@ def test_renewal_invariants_on_gpu(ctx, dev):
     """grad f_cond(C) = n and f_cond(C) = mu(C)(1 - 1/sigma) for the GPU solve + field."""
     import torch
     from mupsi_amd import Scene, batch
-    sc = _scene(2)
+    sc = _scene("sphere_renewal_256")
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
     rng = np.random.default_rng(6)

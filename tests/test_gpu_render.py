@@ -18,26 +18,28 @@ def _medium_pixels(W, H, rng, n):
     return x, y
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3, 5])
-def test_path_replay_matches_oracle(ctx, cfg):
-    """Whole paths against the oracle; configuration 5 adds the emissive quad, so
-    every medium hit takes a light sample whose shadow ray the GPU marches in a
-    separate queue (the oracle marches it serially); configuration 3 is the
-    mesh mean (a 3-times subdivided icosphere keeps the oracle's faithful
+@pytest.mark.parametrize("name,k", [("sphere_single_128", 1), ("sphere_renewal_256", 2),
+                                    ("icosphere_renewal_512", 3), ("guided_sphere_quad_1024", 5)])
+def test_path_replay_matches_oracle(ctx, name, k):
+    """Whole paths against the oracle; guided_sphere_quad_1024 adds the emissive
+    quad, so every medium hit takes a light sample whose shadow ray the GPU marches
+    in a separate queue (the oracle marches it serially); icosphere_renewal_512 is
+    the mesh mean (a 3-times subdivided icosphere keeps the oracle's faithful
     BVH-per-grid-point march fast enough)."""
     from mupsi_amd import Scene
-    sc = scenes.config_scene(cfg, mesh_subdivisions=3) if cfg == 3 else scenes.config_scene(cfg)
+    mesh = name == "icosphere_renewal_512"
+    sc = scenes.preset_scene(name, mesh_subdivisions=3) if mesh else scenes.preset_scene(name)
     s = Scene(ctx, sc)
     osc = O.OracleScene(sc)
-    rng = np.random.default_rng(40 + cfg)
-    n = 300 if cfg == 3 else 1500
+    rng = np.random.default_rng(40 + k)
+    n = 300 if mesh else 1500
     x, y = _medium_pixels(sc["width"], sc["height"], rng, n)
     k = rng.integers(0, 64, n).astype(np.int32)
     got = s.trace_paths(x, y, k)
     ref = osc.trace_paths(x, y, k)
     close = np.all(np.abs(got - ref) <= 1e-4 * (1 + np.abs(ref)), axis=1)
     frac = close.mean()
-    print(f"cfg{cfg}: {close.sum()}/{n} paths agree; nonzero {np.count_nonzero(ref.sum(1))}")
+    print(f"{name}: {close.sum()}/{n} paths agree; nonzero {np.count_nonzero(ref.sum(1))}")
     assert frac >= 0.97, frac
 
 
@@ -58,7 +60,7 @@ def test_sky_only_scene_is_exact(ctx):
 
 def test_render_matches_oracle_small(ctx):
     from mupsi_amd import Scene
-    sc = scenes.config_scene(2, width=40, height=40)
+    sc = scenes.preset_scene("sphere_renewal_256", width=40, height=40)
     s = Scene(ctx, sc)
     s.render_passes(0, 4)
     sums, cnt = s.read_accum()
@@ -73,7 +75,7 @@ def test_render_matches_oracle_small(ctx):
 
 def test_render_deterministic_and_chunked(ctx):
     from mupsi_amd import Scene
-    sc = scenes.config_scene(2, width=64, height=64)
+    sc = scenes.preset_scene("sphere_renewal_256", width=64, height=64)
     s = Scene(ctx, sc)
     s.render_passes(0, 8, passes_per_launch=8)
     a, _ = s.read_accum()
@@ -92,7 +94,7 @@ def test_render_deterministic_and_chunked(ctx):
 def test_abort_flag_keeps_last_pass(ctx):
     import ctypes
     from mupsi_amd import Scene, MupsiError
-    sc = scenes.config_scene(2, width=32, height=32)
+    sc = scenes.preset_scene("sphere_renewal_256", width=32, height=32)
     s = Scene(ctx, sc)
     s.render_passes(0, 2)
     before, n0 = s.read_accum()
@@ -106,7 +108,7 @@ def test_abort_flag_keeps_last_pass(ctx):
 
 def test_scene_json_matches_dict(ctx, tmp_path):
     from mupsi_amd import Scene, write_scene_files
-    sc = scenes.config_scene(2, width=24, height=24)
+    sc = scenes.preset_scene("sphere_renewal_256", width=24, height=24)
     path = write_scene_files(sc, tmp_path)
     a = Scene(ctx, sc)
     b = Scene(ctx, path)
@@ -124,7 +126,7 @@ def test_context_close_with_live_scene_exits_cleanly():
     code = ("import sys; sys.path.insert(0, %r)\n"
             "from mupsi_amd import Context, Scene, scenes\n"
             "c = Context(0)\n"
-            "s = Scene(c, scenes.config_scene(2, width=8, height=8))\n"
+            "s = Scene(c, scenes.preset_scene('sphere_renewal_256', width=8, height=8))\n"
             "s.render_passes(0, 1)\n"
             "c.close()\n"
             "del s\n"

@@ -84,7 +84,7 @@ def test_lambertian_furnace_with_russian_roulette():
 
 
 def test_render_sums_equal_trace_paths():
-    sc = scenes.config_scene(2, width=12, height=10)
+    sc = scenes.preset_scene("sphere_renewal_256", width=12, height=10)
     osc = O.OracleScene(sc)
     sums, paths, _ = osc.render(0, 3)
     assert paths == 12 * 10 * 3

@@ -169,7 +169,7 @@ def test_grid_march_without_noise_finds_the_sampled_sphere():
 
 def test_medium_priority_of_the_scene_lowering():
     """int_medium takes gp_medium, else sdf_medium, else external_gp_medium."""
-    gp = scenes.config_scene(2)
+    gp = scenes.preset_scene("sphere_renewal_256")
     sdf = scenes.medium_scene("sdf")
     grid = scenes.medium_scene("grid")
     both = dict(gp)

@@ -9,8 +9,8 @@ from oracle import oracle as O
 from tests.test_oracle_psi import f32fma
 
 
-def _scene(cfg):
-    return scenes.config_scene(cfg, width=16, height=16)
+def _scene(name):
+    return scenes.preset_scene(name, width=16, height=16)
 
 
 def scan(osc, ray, seed, cond=None, mode=0):
@@ -52,14 +52,16 @@ def _rays(rng, n, far=False):
     return rays
 
 
-@pytest.mark.parametrize("cfg,mode,far", [(1, 0, False), (2, 0, False), (2, 1, True)])
-def test_march_equals_grid_scan(cfg, mode, far):
-    osc = O.OracleScene(_scene(cfg))
-    rng = np.random.default_rng(cfg * 10 + mode)
+@pytest.mark.parametrize("name,k,mode,far", [("sphere_single_128", 1, 0, False),
+                                             ("sphere_renewal_256", 2, 0, False),
+                                             ("sphere_renewal_256", 2, 1, True)])
+def test_march_equals_grid_scan(name, k, mode, far):
+    osc = O.OracleScene(_scene(name))
+    rng = np.random.default_rng(k * 10 + mode)
     rays = _rays(rng, 12, far)
     seeds = rng.integers(0, 2**32, 12, dtype=np.uint64).astype(np.uint32)
     cond = None
-    if cfg == 2:
+    if name == "sphere_renewal_256":
         cond = np.zeros((12, 8), np.float32)
         cond[:, 0] = 1
         cond[:, 1:4] = rays[:, :3] + rng.normal(0, 2, (12, 3))
@@ -78,7 +80,7 @@ def test_march_equals_grid_scan(cfg, mode, far):
 
 def test_march_origin_value_is_fixed_and_zero_is_negative():
     """f_prev is f(origin) for the whole march; sign(0) is false (gpmedium.cpp:43,59)."""
-    sc = _scene(1)
+    sc = _scene("sphere_single_128")
     osc = O.OracleScene(sc)
     # a ray that starts inside the mean sphere and leaves it: f_prev < 0, hit where f > 0
     rays = np.array([[0, 0, -500.0, 1, 0, 0, 0.0015, 3.4e38]], np.float32)
@@ -92,7 +94,7 @@ def test_march_origin_value_is_fixed_and_zero_is_negative():
 def test_renewal_plus_invariants(fp64):
     """With the conditioning state of sampleCondition at C and the next seed:
     grad f_cond(C) = n exactly and f_cond(C) = mu(C)(1 - 1/sigma)."""
-    sc = _scene(2)
+    sc = _scene("sphere_renewal_256")
     osc = O.OracleScene(sc)
     rng = np.random.default_rng(4)
     n = 200
@@ -117,7 +119,7 @@ def test_renewal_plus_invariants(fp64):
 
 
 def test_single_realization_never_conditions():
-    sc = _scene(1)
+    sc = _scene("sphere_single_128")
     osc = O.OracleScene(sc)
     C3 = np.array([[0, 0, -430.0]], np.float32)
     u, g = osc.condition(C3, np.array([[0, 0, 1.0]], np.float32), np.array([1], np.uint32), fp64=False)
@@ -125,7 +127,7 @@ def test_single_realization_never_conditions():
 
 
 def test_sphere_mean_definition():
-    osc = O.OracleScene(_scene(2))
+    osc = O.OracleScene(_scene("sphere_renewal_256"))
     rng = np.random.default_rng(1)
     p = (np.array([0, 0, -500.0]) + rng.normal(0, 80, (100, 3))).astype(np.float32)
     m = osc.mean_eval(p)
