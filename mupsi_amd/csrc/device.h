@@ -116,6 +116,10 @@ struct DevMesh {
   const int32_t* face_tex;   // or null
   const float* verts;        // local vertex positions (light sampling)
   const float* face_cdf;     // normalised cumulative face areas (light sampling)
+  // per node, the next node of a left-first pre-order walk once its subtree is
+  // done or skipped (-1: the walk ends), or null when the tree is too deep for
+  // the reference's 64-entry stack to be provably never full
+  const int32_t* skip;
   float total_area;          // local-space area (Mesh::prepareForRender)
   int n_nodes, n_faces;
   uint32_t root;
@@ -574,12 +578,25 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
     float hs[kHits];
     int nh = 0;
     bool ok = true;
+    // With skip links the walk needs no stack: the next node is the left child
+    // of a hit internal node, else the node's skip link (loaded beside the node).
+    // Both walks test the same nodes (every child of a hit internal node).
+    const bool threaded = m.skip != nullptr;
     uint32_t st[kStack];
     int sp = 0;
     st[sp++] = m.root;
-    while (sp > 0 && ok) {
-      const uint32_t ni = st[--sp];
+    int32_t next = (int32_t)m.root;
+    int steps = 0;
+    while ((threaded ? next >= 0 : sp > 0) && ok) {
+      const uint32_t ni = threaded ? (uint32_t)next : st[--sp];
       const BvhNode nd = load_node(m.nodes, ni);
+      if (threaded) {
+        next = m.skip[ni];
+        if (++steps > m.n_nodes) {  // the links are checked on the host; a bound all the same
+          ok = false;
+          break;
+        }
+      }
       cnt.node();
       if (!aabb_hit(nd, r)) continue;
       if (nd.right & kLeafFlag) {
@@ -601,6 +618,8 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
             ++nh;
           }
         }
+      } else if (threaded) {
+        next = (int32_t)nd.left;
       } else if (sp <= kStack - 2) {
         push_children(m, nd, ni, r, st, sp);
       } else {

@@ -2491,8 +2491,42 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
   return MUPSI_OK;
 }
 
+// Skip links of a left-first pre-order walk (the next node once a node's subtree
+// is done or skipped; -1 ends the walk), if the tree allows the stackless inside
+// walk: every node reachable once, indices in range, and a depth at which the
+// reference's near/far traversal (at most one waiting sibling per level) can
+// never fill its 64-entry stack, so no cast of the reference drops a subtree.
+static bool bvh_skip_links(const BvhResult& b, std::vector<int32_t>& skip) {
+  const size_t n = b.nodes.size();
+  if (n == 0 || n > (size_t)INT32_MAX || b.root >= n) return false;
+  skip.assign(n, -2);
+  struct Item {
+    uint32_t node;
+    int32_t skip;
+    int depth;
+  };
+  std::vector<Item> st{{b.root, -1, 0}};
+  while (!st.empty()) {
+    const Item e = st.back();
+    st.pop_back();
+    if (e.node >= n || skip[e.node] != -2 || e.depth > dev::kStack - 4) return false;
+    skip[e.node] = e.skip;
+    const BvhNode& nd = b.nodes[e.node];
+    if (!(nd.right & kLeafFlag)) {
+      if (nd.left >= n || nd.right >= n) return false;
+      st.push_back({nd.right, e.skip, e.depth + 1});
+      st.push_back({nd.left, (int32_t)nd.right, e.depth + 1});
+    }
+  }
+  for (int32_t& s : skip)
+    if (s == -2) s = -1;  // unreachable nodes are never visited
+  return true;
+}
+
 static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh* out) {
   DevMesh m{};
+  std::vector<int32_t> skip;
+  if (bvh_skip_links(hm.bvh, skip)) HIP_TRY(mem.upload(skip.data(), skip.size(), &m.skip));
   const BvhNode* nodes;
   HIP_TRY(mem.upload(hm.bvh.nodes.data(), hm.bvh.nodes.size(), &nodes));
   m.nodes = nodes;
