@@ -565,8 +565,10 @@ __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cn
 // ancestors' slabs contain the leaf's); (b) no two hits are within 2e-4, so each
 // cast's nearT = h + 1e-4 lies strictly between consecutive hits and the cast
 // finds exactly the next one; (c) the stack never ran out (the casts, culling
-// more, then never run out either).  Otherwise the reference's loop of casts
-// runs as written.
+// more, then never run out either); with skip links (m.skip) the walk has no
+// stack and the host built them only for trees shallow enough that no cast's
+// 64-entry stack can fill.  Otherwise the reference's loop of casts runs as
+// written.
 // kOneTraversal false: the reference's loop of casts only (its traversal cost).
 template <bool kOneTraversal = true, class C = NoCount>
 __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
