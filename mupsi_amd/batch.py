@@ -28,6 +28,29 @@ def _dev(t):
     return C.c_void_p(t.data_ptr())
 
 
+_KINDS = {"f32": ("float32",), "i32": ("int32", "uint32"), "u8": ("uint8", "bool")}
+
+
+def _check(t, kind: str, n: int, cols: int | None, device, name: str):
+    """Refuses a tensor the kernels would misread: a dtype of another width or
+    kind (f32: float32; i32: 32-bit integers; u8: uint8), fewer or more rows
+    than n, a row width other than cols, or another device.  The entry points
+    read n (x cols) elements through raw pointers, so a short or narrower
+    array would be read past its end."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    dt = str(t.dtype).replace("torch.", "")
+    if dt not in _KINDS[kind]:
+        raise TypeError(f"{name}: dtype {dt} where {'/'.join(_KINDS[kind])} is required")
+    shape = (n,) if cols is None else (n, cols)
+    if tuple(t.shape) != shape:
+        raise ValueError(f"{name}: shape {tuple(t.shape)} where {shape} is required")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name}: on {t.device}, the other arrays are on {device}")
+    return t
+
+
 def gp_params(kernel: dict) -> A.GpParams:
     g = A.GpParams()
     g.sigma = float(kernel.get("sigma", 1.0))
@@ -37,11 +60,15 @@ def gp_params(kernel: dict) -> A.GpParams:
     return g
 
 
-def _cond(cond):
+def _cond(cond, n: int, device):
     """cond: None or dict of device tensors active(u8), C(n,3), u(n), g(n,3)."""
     if cond is None:
         return None, []
     torch = _torch()
+    _check(cond["active"], "u8", n, None, device, "cond.active")
+    _check(cond["C"], "f32", n, 3, device, "cond.C")
+    _check(cond["u"], "f32", n, None, device, "cond.u")
+    _check(cond["g"], "f32", n, 3, device, "cond.g")
     keep = [cond["active"].to(torch.uint8).contiguous()]
     Cc = cond["C"].float()
     g = cond["g"].float()
@@ -56,6 +83,8 @@ def _cond(cond):
 def psi_eval(ctx, kernel: dict, p, seed):
     torch = _torch()
     n = p.shape[0]
+    _check(p, "f32", n, 3, None, "p")
+    _check(seed, "i32", n, None, p.device, "seed")
     px, py, pz = (p[:, i].contiguous() for i in range(3))
     out = torch.empty((n, 4), dtype=torch.float32, device=p.device)
     l1 = torch.empty(n, dtype=torch.float32, device=p.device)
@@ -68,6 +97,8 @@ def psi_eval(ctx, kernel: dict, p, seed):
 def cell_impulses(ctx, kernel: dict, cells, seed):
     torch = _torch()
     n = cells.shape[0]
+    _check(cells, "i32", n, 3, None, "cells")
+    _check(seed, "i32", n, None, cells.device, "seed")
     ppc = int(kernel.get("points_per_cell", 3))
     cs = torch.empty(n, dtype=torch.int32, device=cells.device)
     pos = torch.empty((n, ppc, 3), dtype=torch.float32, device=cells.device)
@@ -81,9 +112,11 @@ def cell_impulses(ctx, kernel: dict, cells, seed):
 def field_eval(scene, p, seed, cond=None):
     torch = _torch()
     n = p.shape[0]
+    _check(p, "f32", n, 3, None, "p")
+    _check(seed, "i32", n, None, p.device, "seed")
     px, py, pz = (p[:, i].contiguous() for i in range(3))
     out = torch.empty((n, 4), dtype=torch.float32, device=p.device)
-    cs, keep = _cond(cond)
+    cs, keep = _cond(cond, n, p.device)
     check(scene.lib.mupsi_field_eval(scene.handle, _dev(px), _dev(py), _dev(pz), _dev(seed),
                                      C.byref(cs) if cs is not None else None, n, _dev(out)),
           "mupsi_field_eval")
@@ -94,6 +127,8 @@ def march(scene, rays, seed, cond=None, mode: int = 0):
     """rays: (n, 8) device tensor {ox, oy, oz, dx, dy, dz, near_t, far_t}."""
     torch = _torch()
     n = rays.shape[0]
+    _check(rays, "f32", n, 8, None, "rays")
+    _check(seed, "i32", n, None, rays.device, "seed")
     cols = [rays[:, i].contiguous() for i in range(8)]
     dev = rays.device
     hit = torch.empty(n, dtype=torch.uint8, device=dev)
@@ -101,7 +136,7 @@ def march(scene, rays, seed, cond=None, mode: int = 0):
     p = torch.empty((n, 3), dtype=torch.float32, device=dev)
     nrm = torch.empty((n, 3), dtype=torch.float32, device=dev)
     steps = torch.empty(n, dtype=torch.int32, device=dev)
-    cs, keep = _cond(cond)
+    cs, keep = _cond(cond, n, dev)
     check(scene.lib.mupsi_march(scene.handle, mode, *[_dev(c) for c in cols], _dev(seed),
                                 C.byref(cs) if cs is not None else None, n, _dev(hit), _dev(t),
                                 _dev(p), _dev(nrm), _dev(steps)), "mupsi_march")
@@ -111,6 +146,9 @@ def march(scene, rays, seed, cond=None, mode: int = 0):
 def condition(scene, C3, N3, seed_next):
     torch = _torch()
     n = C3.shape[0]
+    _check(C3, "f32", n, 3, None, "C")
+    _check(N3, "f32", n, 3, C3.device, "n")
+    _check(seed_next, "i32", n, None, C3.device, "seed_next")
     cols = [C3[:, i].contiguous() for i in range(3)] + [N3[:, i].contiguous() for i in range(3)]
     u = torch.empty(n, dtype=torch.float64, device=C3.device)
     g = torch.empty((n, 3), dtype=torch.float64, device=C3.device)
@@ -122,6 +160,7 @@ def condition(scene, C3, N3, seed_next):
 def mesh_intersect(scene, prim: int, rays):
     torch = _torch()
     n = rays.shape[0]
+    _check(rays, "f32", n, 8, None, "rays")
     cols = [rays[:, i].contiguous() for i in range(8)]
     tri = torch.empty(n, dtype=torch.int32, device=rays.device)
     t, u, v = (torch.empty(n, dtype=torch.float32, device=rays.device) for _ in range(3))
@@ -133,6 +172,7 @@ def mesh_intersect(scene, prim: int, rays):
 def mesh_occluded(scene, prim: int, rays):
     torch = _torch()
     n = rays.shape[0]
+    _check(rays, "f32", n, 8, None, "rays")
     cols = [rays[:, i].contiguous() for i in range(8)]
     occ = torch.empty(n, dtype=torch.uint8, device=rays.device)
     check(scene.lib.mupsi_mesh_occluded(scene.handle, prim, *[_dev(c) for c in cols], n, _dev(occ)),
@@ -143,6 +183,7 @@ def mesh_occluded(scene, prim: int, rays):
 def mesh_closest(scene, prim: int, p):
     torch = _torch()
     n = p.shape[0]
+    _check(p, "f32", n, 3, None, "p")
     px, py, pz = (p[:, i].contiguous() for i in range(3))
     d = torch.empty(n, dtype=torch.float32, device=p.device)
     tri = torch.empty(n, dtype=torch.int32, device=p.device)
@@ -155,6 +196,7 @@ def mesh_closest(scene, prim: int, p):
 def mesh_inside(scene, prim: int, p):
     torch = _torch()
     n = p.shape[0]
+    _check(p, "f32", n, 3, None, "p")
     px, py, pz = (p[:, i].contiguous() for i in range(3))
     out = torch.empty(n, dtype=torch.uint8, device=p.device)
     check(scene.lib.mupsi_mesh_inside(scene.handle, prim, _dev(px), _dev(py), _dev(pz), n, _dev(out)),
@@ -169,6 +211,7 @@ def mesh_query_cost(scene, prim: int, kind: int, p):
     int64 tensors (n,)."""
     torch = _torch()
     n = p.shape[0]
+    _check(p, "f32", n, 3, None, "p")
     px, py, pz = (p[:, i].contiguous() for i in range(3))
     nodes = torch.empty(n, dtype=torch.int32, device=p.device)
     tris = torch.empty(n, dtype=torch.int32, device=p.device)

@@ -3739,6 +3739,8 @@ extern "C" mupsi_status mupsi_mesh_intersect(mupsi_scene* sc, int32_t prim, cons
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_intersect: no such mesh");
   if (n <= 0) return MUPSI_OK;
+  if (!ox || !oy || !oz || !dx || !dy || !dz || !near_t || !far_t || !tri || !t || !u || !v)
+    return fail(MUPSI_ERR_INVALID, "mupsi_mesh_intersect: null buffer");
   RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
   return mesh_query_launch(sc, [&](hipStream_t st) {
     hipLaunchKernelGGL(k_mesh_intersect, dim3(grid_for(n)), dim3(256), 0, st, *m, rs, n, tri, t,
@@ -3753,6 +3755,8 @@ extern "C" mupsi_status mupsi_mesh_occluded(mupsi_scene* sc, int32_t prim, const
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_occluded: no such mesh");
   if (n <= 0) return MUPSI_OK;
+  if (!ox || !oy || !oz || !dx || !dy || !dz || !near_t || !far_t || !occ)
+    return fail(MUPSI_ERR_INVALID, "mupsi_mesh_occluded: null buffer");
   RaySoA rs{ox, oy, oz, dx, dy, dz, near_t, far_t};
   return mesh_query_launch(sc, [&](hipStream_t st) {
     hipLaunchKernelGGL(k_mesh_occluded, dim3(grid_for(n)), dim3(256), 0, st, *m, rs, n, occ);
@@ -3765,6 +3769,8 @@ extern "C" mupsi_status mupsi_mesh_closest(mupsi_scene* sc, int32_t prim, const 
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_closest: no such mesh");
   if (n <= 0) return MUPSI_OK;
+  if (!px || !py || !pz || !dist || !tri || !closest3)
+    return fail(MUPSI_ERR_INVALID, "mupsi_mesh_closest: null buffer");
   return mesh_query_launch(sc, [&](hipStream_t st) {
     hipLaunchKernelGGL(k_mesh_closest, dim3(grid_for(n)), dim3(256), 0, st, *m, px, py, pz, n,
                        dist, tri, closest3);
@@ -3777,6 +3783,8 @@ extern "C" mupsi_status mupsi_mesh_inside(mupsi_scene* sc, int32_t prim, const f
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_inside: no such mesh");
   if (n <= 0) return MUPSI_OK;
+  if (!px || !py || !pz || !inside)
+    return fail(MUPSI_ERR_INVALID, "mupsi_mesh_inside: null buffer");
   return mesh_query_launch(sc, [&](hipStream_t st) {
     hipLaunchKernelGGL(k_mesh_inside, dim3(grid_for(n)), dim3(256), 0, st, *m, px, py, pz, n,
                        inside);
