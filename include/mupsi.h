@@ -268,6 +268,23 @@ mupsi_status mupsi_march(mupsi_scene* scene, int32_t mode,
                          const mupsi_cond_soa* cond, int64_t n,
                          uint8_t* hit, float* t, float* p, float* nrm, int32_t* steps);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mupsi_march with the same arguments and results, whose
+ * scratch (the marcher's work counter and statistics, the normalized ray
+ * copies and hit points) is a fixed array in the library's device memory,
+ * 9 MiB per GPU, held until the process ends, instead of per-call allocations.
+ * n above MUPSI_MARCH_NOALLOC_MAX_RAYS is refused with MUPSI_ERR_INVALID.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one
+ * GPU (from two threads or two contexts) race. */
+enum { MUPSI_MARCH_NOALLOC_MAX_RAYS = 262144 };
+mupsi_status mupsi_march_noalloc(mupsi_scene* scene, int32_t mode,
+                                 const float* ox, const float* oy, const float* oz,
+                                 const float* dx, const float* dy, const float* dz,
+                                 const float* near_t, const float* far_t, const uint32_t* seed,
+                                 const mupsi_cond_soa* cond, int64_t n,
+                                 uint8_t* hit, float* t, float* p, float* nrm, int32_t* steps);
+
 /* GPMedium::sampleCondition for medium hits at C with normal nrm, next-realization
  * seed: the Renewal+ 4x4 system solved in FP64.  u_tilde[n], g_tilde[n*3] (double). */
 mupsi_status mupsi_condition(mupsi_scene* scene, const float* cx, const float* cy,

@@ -105,6 +105,7 @@ class BatchStats(C.Structure):
 
 
 GUIDE_NUM_PARAMS = 13097
+MARCH_NOALLOC_MAX_RAYS = 262144
 
 VP = C.c_void_p
 I32, I64 = C.c_int32, C.c_int64
@@ -134,6 +135,8 @@ SIGNATURES = {
     "mupsi_field_eval": (I32, [VP, VP, VP, VP, VP, C.POINTER(CondSoA), I64, VP]),
     "mupsi_march": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.POINTER(CondSoA), I64,
                           VP, VP, VP, VP, VP]),
+    "mupsi_march_noalloc": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, C.POINTER(CondSoA),
+                                  I64, VP, VP, VP, VP, VP]),
     "mupsi_condition": (I32, [VP, VP, VP, VP, VP, VP, VP, VP, I64, VP, VP]),
     "mupsi_mesh_intersect": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP, I64, VP, VP, VP, VP]),
     "mupsi_mesh_occluded": (I32, [VP, I32, VP, VP, VP, VP, VP, VP, VP, VP, I64, VP]),

@@ -123,8 +123,10 @@ def field_eval(scene, p, seed, cond=None):
     return out
 
 
-def march(scene, rays, seed, cond=None, mode: int = 0):
-    """rays: (n, 8) device tensor {ox, oy, oz, dx, dy, dz, near_t, far_t}."""
+def march(scene, rays, seed, cond=None, mode: int = 0, noalloc: bool = False):
+    """rays: (n, 8) device tensor {ox, oy, oz, dx, dy, dz, near_t, far_t}.
+    noalloc: run mupsi_march_noalloc (the library's fixed scratch, at most
+    MARCH_NOALLOC_MAX_RAYS rays) instead of mupsi_march."""
     torch = _torch()
     n = rays.shape[0]
     _check(rays, "f32", n, 8, None, "rays")
@@ -137,9 +139,10 @@ def march(scene, rays, seed, cond=None, mode: int = 0):
     nrm = torch.empty((n, 3), dtype=torch.float32, device=dev)
     steps = torch.empty(n, dtype=torch.int32, device=dev)
     cs, keep = _cond(cond, n, dev)
-    check(scene.lib.mupsi_march(scene.handle, mode, *[_dev(c) for c in cols], _dev(seed),
-                                C.byref(cs) if cs is not None else None, n, _dev(hit), _dev(t),
-                                _dev(p), _dev(nrm), _dev(steps)), "mupsi_march")
+    name = "mupsi_march_noalloc" if noalloc else "mupsi_march"
+    check(getattr(scene.lib, name)(scene.handle, mode, *[_dev(c) for c in cols], _dev(seed),
+                                   C.byref(cs) if cs is not None else None, n, _dev(hit), _dev(t),
+                                   _dev(p), _dev(nrm), _dev(steps)), name)
     return {"hit": hit, "t": t, "p": p, "n": nrm, "steps": steps}
 
 

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2317,6 +2318,38 @@ __global__ __launch_bounds__(256) void k_mesh_cost(DevMesh m, int kind, const fl
   tris[i] = c.tris;
 }
 
+// ====================================================================== no-allocation scratch
+
+// Device scratch of mupsi_march_noalloc, a file-scope array instead of per-call
+// allocations: a 256-byte head (the persistent marcher's work counter at byte 0,
+// its four 64-bit statistics at byte 8) and nine arrays of kNoAllocMaxRays
+// floats (normalized ray origin and direction, hit point).  Each GPU the library
+// runs on holds its own copy until the process ends.  Host code never passes
+// the name `scratch` to a kernel (on the host it is a stand-in, not a device
+// address): it uses the per-device address from noalloc_scratch().
+constexpr int64_t kNoAllocMaxRays = MUPSI_MARCH_NOALLOC_MAX_RAYS;
+constexpr size_t kScratchHead = 256;
+constexpr size_t kScratchBytes = kScratchHead + 9 * (size_t)kNoAllocMaxRays * sizeof(float);
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_addr;  // device address of `scratch`, per device index
+
+// Looks the array's address up once per device (the caller has made `device`
+// current) and keeps it; later calls return the kept address.
+static mupsi_status noalloc_scratch(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+  if (!g_scratch_addr[device]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MUPSI_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+    g_scratch_addr[device] = ptr;
+  }
+  *out = g_scratch_addr[device];
+  return MUPSI_OK;
+}
+
 // ====================================================================== host: context
 
 extern "C" const char* mupsi_status_string(mupsi_status s) {
@@ -2358,6 +2391,10 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false, sphere_group(3)>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
+  {
+    const mupsi_status st = noalloc_scratch(device, &ctx->scratch);
+    if (st != MUPSI_OK) return st;
+  }
   *out = ctx.release();
   return MUPSI_OK;
 }
@@ -3630,38 +3667,49 @@ extern "C" mupsi_status mupsi_field_eval(mupsi_scene* sc, const float* px, const
   return MUPSI_OK;
 }
 
-extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* ox,
-                                    const float* oy, const float* oz, const float* dx,
-                                    const float* dy, const float* dz, const float* near_t,
-                                    const float* far_t, const uint32_t* seed,
-                                    const mupsi_cond_soa* cond, int64_t n, uint8_t* hit, float* t,
-                                    float* p, float* nrm, int32_t* steps) {
-  if (!medium_ready(sc)) return fail(MUPSI_ERR_INVALID, "mupsi_march: scene has no medium");
-  if (mode != 0 && mode != 1) return fail(MUPSI_ERR_INVALID, "mupsi_march: mode must be 0 or 1");
+// Device buffers of one batch march beyond its inputs and outputs.
+struct MarchScratch {
+  int* work;                     // persistent marcher's ray counter
+  unsigned long long* stats;     // 4 counters
+  float *ox, *oy, *oz, *dx, *dy, *dz;  // normalized copies of the rays
+  float *px, *py, *pz;           // hit points before interleaving
+};
+
+static mupsi_status march_check(mupsi_scene* sc, int32_t mode, const float* ox, const float* oy,
+                                const float* oz, const float* dx, const float* dy,
+                                const float* dz, const float* near_t, const float* far_t,
+                                const uint32_t* seed, int64_t n, uint8_t* hit, float* t,
+                                float* p, const char* who) {
+  if (!medium_ready(sc)) return fail(MUPSI_ERR_INVALID, std::string(who) + ": scene has no medium");
+  if (mode != 0 && mode != 1) return fail(MUPSI_ERR_INVALID, std::string(who) + ": mode must be 0 or 1");
   if (n <= 0) return MUPSI_OK;
-  if (n > INT32_MAX) return fail(MUPSI_ERR_INVALID, "mupsi_march: too many rays");
+  if (n > INT32_MAX) return fail(MUPSI_ERR_INVALID, std::string(who) + ": too many rays");
   if (!ox || !oy || !oz || !dx || !dy || !dz || !near_t || !far_t || !seed || !hit || !t || !p)
-    return fail(MUPSI_ERR_INVALID, "mupsi_march: null array");
-  HIP_TRY(hipSetDevice(sc->ctx->device));
+    return fail(MUPSI_ERR_INVALID, std::string(who) + ": null array");
+  return MUPSI_OK;
+}
+
+// The march of mupsi_march / mupsi_march_noalloc on the context stream with the
+// given scratch; allocates nothing.  The counters are cleared by a fill kernel.
+static mupsi_status march_run(mupsi_scene* sc, int32_t mode, const float* ox, const float* oy,
+                              const float* oz, const float* dx, const float* dy, const float* dz,
+                              const float* near_t, const float* far_t, const uint32_t* seed,
+                              const mupsi_cond_soa* cond, int64_t n, uint8_t* hit, float* t,
+                              float* p, float* nrm, int32_t* steps, const MarchScratch& ws) {
   hipStream_t st = sc->ctx->stream;
-  DeviceBuffers tmp;
-  int* work;
-  unsigned long long* stats;
-  float *dox, *doy, *doz, *ddx, *ddy, *ddz, *px, *py, *pz;
-  HIP_TRY(tmp.alloc(1, &work));
-  HIP_TRY(tmp.alloc(4, &stats));
-  for (float** q : {&dox, &doy, &doz, &ddx, &ddy, &ddz, &px, &py, &pz}) HIP_TRY(tmp.alloc(n, q));
-  HIP_TRY(hipMemsetAsync(work, 0, sizeof(int), st));
-  HIP_TRY(hipMemsetAsync(stats, 0, 4 * sizeof(unsigned long long), st));
+  // work (1 int) and stats (4 x u64 from byte 8): 10 words from the work counter
+  static_assert(sizeof(unsigned long long) == 8, "stats words");
+  hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, ws.work, 1, 0, -1);
+  hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, (int*)ws.stats, 8, 0, -1);
   // Ray(o, d) normalizes the direction; the batch copies o and normalized d.
-  hipLaunchKernelGGL(k_normalize_rays, dim3(grid_for(n)), dim3(256), 0, st, ox, oy, oz, dx, dy, dz, n, dox,
-                     doy, doz, ddx, ddy, ddz);
+  hipLaunchKernelGGL(k_normalize_rays, dim3(grid_for(n)), dim3(256), 0, st, ox, oy, oz, dx, dy, dz,
+                     n, ws.ox, ws.oy, ws.oz, ws.dx, ws.dy, ws.dz);
   CondSoA cs = cond_soa(cond);
   MarchArgs ma{};
   ma.queue = nullptr;
   ma.count_dev = nullptr;
   ma.count_host = (int)n;
-  ma.ox = dox; ma.oy = doy; ma.oz = doz; ma.dx = ddx; ma.dy = ddy; ma.dz = ddz;
+  ma.ox = ws.ox; ma.oy = ws.oy; ma.oz = ws.oz; ma.dx = ws.dx; ma.dy = ws.dy; ma.dz = ws.dz;
   ma.nt = near_t;
   ma.ft = far_t;
   ma.seed = seed;
@@ -3670,19 +3718,66 @@ extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* 
   ma.cx = cs.cx; ma.cy = cs.cy; ma.cz = cs.cz; ma.cu = cs.u; ma.cgx = cs.gx; ma.cgy = cs.gy; ma.cgz = cs.gz;
   ma.hit = hit;
   ma.t = t;
-  ma.px = px; ma.py = py; ma.pz = pz;
+  ma.px = ws.px; ma.py = ws.py; ma.pz = ws.pz;
   ma.steps = steps;
-  ma.work = work;
-  ma.stats = stats;
+  ma.work = ws.work;
+  ma.stats = ws.stats;
   launch_march(sc->ctx, sc->ds, ma, st);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_interleave3, dim3(grid_for(n)), dim3(256), 0, st, px, py, pz, n, p);
+  hipLaunchKernelGGL(k_interleave3, dim3(grid_for(n)), dim3(256), 0, st, ws.px, ws.py, ws.pz, n, p);
   if (mode == 0 && nrm)
-    hipLaunchKernelGGL(k_hit_normals, dim3(grid_for(n)), dim3(256), 0, st, sc->ds, hit, px, py, pz,
-                       seed, cs, n, nrm);
+    hipLaunchKernelGGL(k_hit_normals, dim3(grid_for(n)), dim3(256), 0, st, sc->ds, hit, ws.px,
+                       ws.py, ws.pz, seed, cs, n, nrm);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_march(mupsi_scene* sc, int32_t mode, const float* ox,
+                                    const float* oy, const float* oz, const float* dx,
+                                    const float* dy, const float* dz, const float* near_t,
+                                    const float* far_t, const uint32_t* seed,
+                                    const mupsi_cond_soa* cond, int64_t n, uint8_t* hit, float* t,
+                                    float* p, float* nrm, int32_t* steps) {
+  const mupsi_status chk = march_check(sc, mode, ox, oy, oz, dx, dy, dz, near_t, far_t, seed, n,
+                                       hit, t, p, "mupsi_march");
+  if (chk != MUPSI_OK || n <= 0) return chk;
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  DeviceBuffers tmp;
+  MarchScratch ws{};
+  HIP_TRY(tmp.alloc(1, &ws.work));
+  HIP_TRY(tmp.alloc(4, &ws.stats));
+  for (float** q : {&ws.ox, &ws.oy, &ws.oz, &ws.dx, &ws.dy, &ws.dz, &ws.px, &ws.py, &ws.pz})
+    HIP_TRY(tmp.alloc(n, q));
+  return march_run(sc, mode, ox, oy, oz, dx, dy, dz, near_t, far_t, seed, cond, n, hit, t, p, nrm,
+                   steps, ws);
+}
+
+extern "C" mupsi_status mupsi_march_noalloc(mupsi_scene* sc, int32_t mode, const float* ox,
+                                            const float* oy, const float* oz, const float* dx,
+                                            const float* dy, const float* dz,
+                                            const float* near_t, const float* far_t,
+                                            const uint32_t* seed, const mupsi_cond_soa* cond,
+                                            int64_t n, uint8_t* hit, float* t, float* p,
+                                            float* nrm, int32_t* steps) {
+  const mupsi_status chk = march_check(sc, mode, ox, oy, oz, dx, dy, dz, near_t, far_t, seed, n,
+                                       hit, t, p, "mupsi_march_noalloc");
+  if (chk != MUPSI_OK || n <= 0) return chk;
+  if (n > kNoAllocMaxRays)
+    return fail(MUPSI_ERR_INVALID, "mupsi_march_noalloc: " + std::to_string(n) +
+                                       " rays exceed the scratch maximum of " +
+                                       std::to_string(kNoAllocMaxRays));
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  char* base = static_cast<char*>(sc->ctx->scratch);
+  if (!base) return fail(MUPSI_ERR_HIP, "mupsi_march_noalloc: no scratch address for the device");
+  float* arr = reinterpret_cast<float*>(base + kScratchHead);
+  MarchScratch ws{};
+  ws.work = reinterpret_cast<int*>(base);
+  ws.stats = reinterpret_cast<unsigned long long*>(base + 8);
+  float** slots[9] = {&ws.ox, &ws.oy, &ws.oz, &ws.dx, &ws.dy, &ws.dz, &ws.px, &ws.py, &ws.pz};
+  for (int i = 0; i < 9; ++i) *slots[i] = arr + (size_t)i * kNoAllocMaxRays;
+  return march_run(sc, mode, ox, oy, oz, dx, dy, dz, near_t, far_t, seed, cond, n, hit, t, p, nrm,
+                   steps, ws);
 }
 
 extern "C" mupsi_status mupsi_condition(mupsi_scene* sc, const float* cx, const float* cy,

@@ -24,6 +24,8 @@ struct mupsi_ctx {
   // device time of the last mesh-query kernel (mupsi_last_kernel_ms)
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   float last_kernel_ms = 0.0f;
+  // device address of the library's no-allocation scratch array on this device
+  void* scratch = nullptr;
 };
 
 namespace mupsi {
