@@ -188,6 +188,20 @@ def preset_scene(name: str, *, width: int | None = None, height: int | None = No
     raise ValueError(f"no preset scene named {name!r} (one of {PRESETS})")
 
 
+def sphere_light_scene(width: int = 256, height: int = 256) -> dict:
+    """The sphere_renewal_256 scene lit also by an emissive sphere, given by the
+    schema's own sphere "emission" key (the only emitter the reference's scene
+    parser creates): radius 40 at (0, 200, -500), above the medium and in view,
+    radiance 10.  Every GP-medium hit and Lambertian surface hit takes a light
+    sample of it (Sphere::sampleDirect), the medium's through a transmittance
+    march."""
+    sc = preset_scene("sphere_renewal_256", width=width, height=height)
+    sc["primitives"] = sc["primitives"] + [
+        {"type": "sphere", "center": [0.0, 200.0, -500.0], "radius": 40.0, "bsdf": "lambertian",
+         "emission": [10.0, 10.0, 10.0]}]
+    return sc
+
+
 def sphere_sdf_grid(res: int, bmin, bmax, center, radius: float) -> np.ndarray:
     """A GridMeanFunction payload: |p - center| - radius at the cell centres of a
     res^3 grid over [bmin, bmax], x-major (index (i * res + j) * res + k)."""

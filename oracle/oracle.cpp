@@ -1321,8 +1321,10 @@ static bool mesh_sample_direct(const Prim& pr, const Mesh& m, V3 p, Pcg& rng, Li
   return true;
 }
 
-// Scene::chooseLight (scene.cpp:50-61)
-static bool choose_light(const Scene& s, V3 p, Pcg& rng, LightSample& ls) {
+// Scene::chooseLight (scene.cpp:50-61); *chosen (if given) receives the
+// primitive index of the light drawn, -1 when none is.
+static bool choose_light(const Scene& s, V3 p, Pcg& rng, LightSample& ls, int* chosen = nullptr) {
+  if (chosen) *chosen = -1;
   if (s.prims.empty()) return false;
   int lights[64];
   int nl = 0;
@@ -1330,6 +1332,7 @@ static bool choose_light(const Scene& s, V3 p, Pcg& rng, LightSample& ls) {
     if (s.prims[i].has_em) lights[nl++] = (int)i;
   if (nl == 0) return false;
   int idx = (int)(rng.nextI() % (uint32_t)nl);
+  if (chosen) *chosen = lights[idx];
   const Prim& pr = s.prims[lights[idx]];
   if (pr.type == MUPSI_PRIM_SPHERE) return sphere_sample_direct(pr, p, rng, ls);
   return mesh_sample_direct(pr, s.meshes[pr.mesh], p, rng, ls);
@@ -1389,8 +1392,25 @@ static V3 sky_lookup(const Scene& s, V3 d, int* texel) {
   return s.sky.at(u, v, texel);
 }
 
-// PathTracer::trace (trace.cpp:118-213), handleSurface (:32-75), handleVolume (:77-116)
-static V3 trace(const Scene& s, int x, int y, int k, uint32_t seed) {
+// The decision margin of one march of a traced path, for *amb of trace(): the
+// same march in FP64 at the same FP32 t-grid points (march(..., fp64, track)),
+// whose amb is the smallest |f| / (1 + |mu| + l1) at a visited grid point; 0 when
+// the FP64 march decides otherwise than the FP32 one (the FP32 decision then
+// rests on rounding).  GP media only (the SDF and grid marches have no margin).
+static void track_margin(const Scene& s, int mode, const Ray& ray, uint32_t seed, const Cond& c,
+                         const MarchOut& mo, double* amb) {
+  if (!amb || s.medium_kind != MUPSI_MEDIUM_GP) return;
+  const MarchOut m64 = march(s, mode, ray, seed, c, true, true);
+  double a = m64.amb;
+  if (m64.hit != mo.hit || (mo.hit && m64.t != mo.t)) a = 0.0;
+  *amb = std::min(*amb, a);
+}
+
+// PathTracer::trace (trace.cpp:118-213), handleSurface (:32-75), handleVolume (:77-116).
+// amb (may be null): receives the smallest decision margin of the path's
+// marches (track_margin), +inf for a path with no GP-medium march.
+static V3 trace(const Scene& s, int x, int y, int k, uint32_t seed, double* amb = nullptr) {
+  if (amb) *amb = INFINITY;
   uint32_t pix_seed = make_seed4((uint32_t)x, (uint32_t)y, (uint32_t)k, seed);
   Pcg rng = pcg_ctor(pix_seed);
   Ray ray = make_ray(s.cam.pos, cam_dir(s.cam, x, y));
@@ -1407,6 +1427,7 @@ static V3 trace(const Scene& s, int x, int y, int k, uint32_t seed) {
       uint32_t mseed = single ? seed : make_seed5(x, y, k, seed, (uint32_t)bounce);
       uint32_t iseed = constant_sampler_i(mseed);
       MarchOut mo = march(s, 0, ray, iseed, cond, false, false);
+      track_margin(s, 0, ray, iseed, cond, mo, amb);
       if (mo.hit) {
         V3 P = mo.p, n = mo.n, wo = neg(ray.d);
         V3 alb = mk(kDefaultAlbedo, kDefaultAlbedo, kDefaultAlbedo);
@@ -1416,6 +1437,7 @@ static V3 trace(const Scene& s, int x, int y, int k, uint32_t seed) {
           float bias = ls.dist - s.eps * std::fabs(dot(ls.d, n));
           sr.farT = bias * 0.99f;
           MarchOut tr = march(s, 1, sr, iseed, cond, false, false);
+          track_margin(s, 1, sr, iseed, cond, tr, amb);
           float trans = tr.hit ? 0.0f : 1.0f;
           V3 f = bsdf_weight(MUPSI_BSDF_LAMBERTIAN, alb, wo, n, ls.d);
           float ac = std::fabs(dot(ls.d, n));
@@ -1881,6 +1903,42 @@ void orc_trace_paths(void* sc, const int32_t* x, const int32_t* y, const int32_t
     rgb[3 * i] = c.x;
     rgb[3 * i + 1] = c.y;
     rgb[3 * i + 2] = c.z;
+  }
+}
+
+void orc_trace_paths_amb(void* sc, const int32_t* x, const int32_t* y, const int32_t* k,
+                         int64_t n, float* rgb, double* amb, int32_t nthreads) {
+  const Scene& s = *(Scene*)sc;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t i = 0; i < n; ++i) {
+    V3 c = trace(s, x[i], y[i], k[i], 42u, &amb[i]);
+    rgb[3 * i] = c.x;
+    rgb[3 * i + 1] = c.y;
+    rgb[3 * i + 2] = c.z;
+  }
+}
+
+void orc_choose_light(void* sc, const float* p3, uint64_t* state, int64_t n, int32_t* light,
+                      float* w3, float* d3, float* dist, float* pdf) {
+  const Scene& s = *(Scene*)sc;
+  for (int64_t i = 0; i < n; ++i) {
+    Pcg rng{state[i]};
+    LightSample ls{};
+    int chosen = -1;
+    const bool ok = choose_light(s, mk(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]), rng, ls, &chosen);
+    state[i] = rng.s;
+    light[i] = ok ? chosen : -1 - (chosen >= 0 ? 1 : 0);
+    w3[3 * i] = ls.w.x;
+    w3[3 * i + 1] = ls.w.y;
+    w3[3 * i + 2] = ls.w.z;
+    d3[3 * i] = ls.d.x;
+    d3[3 * i + 1] = ls.d.y;
+    d3[3 * i + 2] = ls.d.z;
+    dist[i] = ls.dist;
+    pdf[i] = ls.pdf;
   }
 }
 

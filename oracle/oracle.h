@@ -86,6 +86,19 @@ void orc_sky_lookup(void* scene, const float* dir3, int64_t n, float* rgb, int32
 /* PathTracer::trace for each (x, y, k), seed 42. */
 void orc_trace_paths(void* scene, const int32_t* x, const int32_t* y, const int32_t* k,
                      int64_t n, float* rgb, int32_t nthreads);
+/* orc_trace_paths plus, per path, the smallest decision margin of its GP-medium
+ * marches (each march redone in FP64 at the same FP32 t grid: the smallest
+ * |f| / (1 + |mu| + l1) at a visited grid point, 0 where the FP64 decision
+ * differs from the FP32 one; +inf for a path with no such march). */
+void orc_trace_paths_amb(void* scene, const int32_t* x, const int32_t* y, const int32_t* k,
+                         int64_t n, float* rgb, double* amb, int32_t nthreads);
+/* Scene::chooseLight at points p3 (n x 3) with the PCG states state[i]
+ * (setState semantics; updated in place to the state after the draws).
+ * light[i]: primitive index of the light sampled; -1 when no light was drawn
+ * (no emitter: no draw at all), -2 when a light was drawn but its sampleDirect
+ * failed.  w3 = Le / pdf, d3 = direction, dist, pdf as LightSample. */
+void orc_choose_light(void* scene, const float* p3, uint64_t* state, int64_t n, int32_t* light,
+                      float* w3, float* d3, float* dist, float* pdf);
 /* Renderer::startRenderProgressive restated with OpenMP collapse(2), static
  * schedule, over passes k_begin..k_begin+k_count-1 and the pixels whose linear
  * index is congruent to pixel_offset mod pixel_stride.  Adds radiance into

@@ -42,6 +42,8 @@ _SIGS = {
     "orc_camera_rays": (None, [VP, VP, VP, I64, VP]),
     "orc_sky_lookup": (None, [VP, VP, I64, VP, VP]),
     "orc_trace_paths": (None, [VP, VP, VP, VP, I64, VP, I32]),
+    "orc_trace_paths_amb": (None, [VP, VP, VP, VP, I64, VP, VP, I32]),
+    "orc_choose_light": (None, [VP, VP, VP, I64, VP, VP, VP, VP, VP]),
     "orc_render": (C.c_double, [VP, I32, I32, I32, I32, I32, VP, VP]),
     "orc_max_threads": (I32, []),
 }
@@ -263,6 +265,34 @@ class OracleScene:
         out = np.zeros((x.size, 3), np.float32)
         lib().orc_trace_paths(self.h, _p(x), _p(y), _p(k), x.size, _p(out), nthreads)
         return out
+
+    def trace_paths_amb(self, x, y, k, nthreads=0):
+        """trace_paths plus each path's smallest GP-march decision margin
+        (FP64 |f| / (1 + |mu| + l1) at its visited grid points; 0 where the
+        FP64 march decides otherwise; inf without a GP march)."""
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        k = np.ascontiguousarray(k, np.int32)
+        out = np.zeros((x.size, 3), np.float32)
+        amb = np.zeros(x.size, np.float64)
+        lib().orc_trace_paths_amb(self.h, _p(x), _p(y), _p(k), x.size, _p(out), _p(amb), nthreads)
+        return out, amb
+
+    def choose_light(self, p3, states):
+        """Scene::chooseLight at points p3 (n, 3) from PCG states (n,) uint64
+        (setState semantics).  Returns a dict: light (primitive index; -1 no
+        draw, -2 sampleDirect failed), w (n, 3), d (n, 3), dist, pdf, state
+        (the PCG state after the draws)."""
+        p3 = np.ascontiguousarray(p3, np.float32).reshape(-1, 3)
+        n = p3.shape[0]
+        st = np.array(np.broadcast_to(np.asarray(states, np.uint64), (n,)))
+        light = np.zeros(n, np.int32)
+        w = np.zeros((n, 3), np.float32)
+        d = np.zeros((n, 3), np.float32)
+        dist = np.zeros(n, np.float32)
+        pdf = np.zeros(n, np.float32)
+        lib().orc_choose_light(self.h, _p(p3), _p(st), n, _p(light), _p(w), _p(d), _p(dist), _p(pdf))
+        return {"light": light, "w": w, "d": d, "dist": dist, "pdf": pdf, "state": st}
 
     def render(self, k_begin, k_count, pixel_stride=1, pixel_offset=0, nthreads=0):
         sums = np.zeros((self.height, self.width, 3), np.float64)

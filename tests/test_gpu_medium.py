@@ -107,14 +107,15 @@ def test_march_hits_bit_exact(ctx, dev, name, k):
     ok = same_hit & same_t
     assert ok[unamb].all(), f"{(~ok[unamb]).sum()} unambiguous rays differ"
     assert unamb.mean() > 0.95
-    # and with the FP32-faithful oracle on nearly all rays
+    # and with the FP32-faithful oracle on every unambiguous ray (its own FP32
+    # rounding can only move a decision whose FP64 margin is below the threshold)
     agree32 = (hit == ref32["hit"]) & ((~hit.astype(bool)) | (t == ref32["t"]))
-    assert agree32.mean() > 0.98
+    assert agree32[unamb].all(), f"{(~agree32[unamb]).sum()} unambiguous rays differ from FP32"
     # normals at common hits within 1e-5 of FP64 (normalized gradient)
     m = unamb & hit.astype(bool) & ref64["hit"].astype(bool)
     if m.any():
         err = np.abs(nrm[m] - ref64["n"][m]).max()
-        assert err < 1e-4, err
+        assert err < 1e-5, err
     assert hit.sum() > 10
 
 
