@@ -341,6 +341,17 @@ mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices, const in
                              int32_t n_faces, float* nodes_out, int32_t* n_nodes_out,
                              uint32_t* tri_index_out, uint32_t* root_out);
 
+/* Device layout of the BVH mupsi_bvh_build builds (no reference counterpart:
+ * it reports how the library stores and walks the reference's tree): the
+ * number of 64-byte records (one per internal node, both children's bounds
+ * side by side, breadth-first so that the top levels come first), the levels
+ * of internal nodes on the longest root-to-leaf path, and whether
+ * Mesh::inside runs as one stackless walk (trees of at most 60 levels, where
+ * the reference's 64-entry stack cannot fill) or as the reference's casts. */
+mupsi_status mupsi_bvh_layout(const float* vertices, int32_t n_vertices, const int32_t* faces,
+                              int32_t n_faces, int32_t* n_records, int32_t* depth,
+                              int32_t* stackless);
+
 /* ---- batched scenes ----
  * Several uploaded scenes of one context traced in shared wavefront launches
  * (the pass loop of Renderer::startRenderProgressive, renderer.cpp:22-49, for

@@ -144,6 +144,7 @@ SIGNATURES = {
     "mupsi_mesh_inside": (I32, [VP, I32, VP, VP, VP, I64, VP]),
     "mupsi_mesh_query_cost": (I32, [VP, I32, I32, VP, VP, VP, I64, VP, VP]),
     "mupsi_last_kernel_ms": (I32, [VP, C.POINTER(C.c_float)]),
+    "mupsi_bvh_layout": (I32, [VP, I32, VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "mupsi_bvh_build": (I32, [VP, I32, VP, I32, VP, C.POINTER(I32), VP, C.POINTER(C.c_uint32)]),
     "mupsi_batch_create": (I32, [VP, VP, I32, C.POINTER(VP)]),
     "mupsi_batch_free": (None, [VP]),

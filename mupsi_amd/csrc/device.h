@@ -106,8 +106,18 @@ struct DevTex {
   int w, h;
 };
 struct DevMesh {
-  const BvhNode* nodes;
-  const uint8_t* order;      // per internal node: axis | (lc<rc)<<2 | (lc>rc)<<3
+  // internal nodes as 64-byte records of both children's bounds, breadth-first
+  // from the root (record 0); the first ntop may be read from `top`, a copy in
+  // LDS that a kernel makes at its start (null where it makes none)
+  const BvhRec* recs;
+  const BvhRec* top;
+  uint32_t ntop, n_recs;
+  uint32_t root_ref;         // the root: record 0, or a leaf reference for <= 4 triangles
+  uint32_t occ_ref;          // node 0 of the reference's numbering (a leaf): BVH::occluded
+  float occ_min[3], occ_max[3];
+  // inside() may walk the tree once without a stack: it is at most 60 levels
+  // deep, so that the reference's 64-entry stack can never fill
+  int stackless;
   const float4* leaf_tris;   // 3 per leaf slot: (v0, id bits), (v1, 0), (v2, 0)
   const float* vnormals;     // per vertex, or null
   const int32_t* faces;
@@ -116,13 +126,8 @@ struct DevMesh {
   const int32_t* face_tex;   // or null
   const float* verts;        // local vertex positions (light sampling)
   const float* face_cdf;     // normalised cumulative face areas (light sampling)
-  // per node, the next node of a left-first pre-order walk once its subtree is
-  // done or skipped (-1: the walk ends), or null when the tree is too deep for
-  // the reference's 64-entry stack to be provably never full
-  const int32_t* skip;
   float total_area;          // local-space area (Mesh::prepareForRender)
-  int n_nodes, n_faces;
-  uint32_t root;
+  int n_nodes, n_faces;      // n_nodes: nodes of the reference's tree
   DevXf xf;
   float lbmin[3], lbmax[3];  // local-space bounds of the whole mesh
   // distance to the mesh from the centres of a grid of cubes (local space), or
@@ -265,24 +270,29 @@ __device__ __forceinline__ F3 xf_to_world(const DevXf& x, F3 p) {
 }
 
 // ------------------------------------------------------------------ BVH
-__device__ __forceinline__ bool aabb_hit(const BvhNode& n, const Ray& r) {
+// AABB::intersect (boudingbox.cpp:6-38) up to its farT clause: the slab entry
+// tmin where tmax >= tmin and tmax >= nearT, NaN where the box is missed
+// whatever farT is.  The box is hit iff slab_tmin(...) <= farT (NaN compares
+// false), so a traversal can test a box when it is first seen and complete the
+// test with the farT it has when it takes the box up.
+__device__ __forceinline__ float slab_tmin(const float* bmin, const float* bmax, const Ray& r) {
   float tmin = -INFINITY, tmax = INFINITY;
   const float o[3] = {r.o.x, r.o.y, r.o.z};
   const float d[3] = {r.d.x, r.d.y, r.d.z};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (fabsf(d[a]) > 1e-8f) {
-      const float t1 = (n.bmin[a] - o[a]) / d[a];
-      const float t2 = (n.bmax[a] - o[a]) / d[a];
+      const float t1 = (bmin[a] - o[a]) / d[a];
+      const float t2 = (bmax[a] - o[a]) / d[a];
       const float lo = t2 < t1 ? t2 : t1;  // std::min(t1, t2)
       const float hi = t1 < t2 ? t2 : t1;  // std::max(t1, t2)
       tmin = tmin < lo ? lo : tmin;
       tmax = hi < tmax ? hi : tmax;
-    } else if (o[a] < n.bmin[a] || o[a] > n.bmax[a]) {
-      return false;
+    } else if (o[a] < bmin[a] || o[a] > bmax[a]) {
+      return __builtin_nanf("");
     }
   }
-  return tmax >= tmin && tmax >= r.nearT && tmin <= r.farT;
+  return (tmax >= tmin && tmax >= r.nearT) ? tmin : __builtin_nanf("");
 }
 
 // Moller-Trumbore (bvh.cpp:82-111)
@@ -303,19 +313,85 @@ __device__ __forceinline__ bool ray_tri(const Ray& r, F3 v0, F3 v1, F3 v2, float
   return t > r.nearT && t < r.farT;
 }
 
-__device__ __forceinline__ BvhNode load_node(const BvhNode* nodes, uint32_t i) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + i);
-  const float4 a = p[0], b = p[1];
-  BvhNode n;
-  n.bmin[0] = a.x;
-  n.bmin[1] = a.y;
-  n.bmin[2] = a.z;
-  n.bmax[0] = a.w;
-  n.bmax[1] = b.x;
-  n.bmax[2] = b.y;
-  n.left = __float_as_uint(b.z);
-  n.right = __float_as_uint(b.w);
-  return n;
+// One record: from global memory (64 bytes, one cache line), or with kTop from
+// the LDS copy of the top levels where the kernel made one.
+template <bool kTop = false>
+__device__ __forceinline__ BvhRec load_rec(const DevMesh& m, uint32_t i) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  using LdsV4 = const V4 __attribute__((address_space(3)))*;
+  using GlobalV4 = const V4 __attribute__((address_space(1)))*;
+  V4 a, b, c, d;
+  if (kTop && i < m.ntop) {  // ds_read from the block's copy
+    const LdsV4 p = (LdsV4)(m.top + i);
+    a = p[0];
+    b = p[1];
+    c = p[2];
+    d = p[3];
+  } else {
+    const GlobalV4 p = (GlobalV4)(m.recs + i);
+    a = p[0];
+    b = p[1];
+    c = p[2];
+    d = p[3];
+  }
+  BvhRec R;
+  R.lmin[0] = a.x;
+  R.lmin[1] = a.y;
+  R.lmin[2] = a.z;
+  R.lmax[0] = a.w;
+  R.lmax[1] = b.x;
+  R.lmax[2] = b.y;
+  R.l = __float_as_uint(b.z);
+  R.order = __float_as_uint(b.w);
+  R.rmin[0] = c.x;
+  R.rmin[1] = c.y;
+  R.rmin[2] = c.z;
+  R.rmax[0] = c.w;
+  R.rmax[1] = d.x;
+  R.rmax[2] = d.y;
+  R.r = __float_as_uint(d.z);
+  R.next = __float_as_int(d.w);
+  return R;
+}
+
+// One child of record i (its 32-byte half): bounds, reference, and for the
+// right child the record's next.
+struct RecHalf {
+  float bmin[3], bmax[3];
+  uint32_t ref;
+  int32_t next;
+};
+template <bool kTop = false>
+__device__ __forceinline__ RecHalf load_half(const DevMesh& m, uint32_t i, bool right) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  using LdsV4 = const V4 __attribute__((address_space(3)))*;
+  using GlobalV4 = const V4 __attribute__((address_space(1)))*;
+  const uint32_t k = 4 * i + (right ? 2 : 0);
+  V4 a, b;
+  if (kTop && i < m.ntop) {
+    const LdsV4 p = (LdsV4)(m.top) + k;
+    a = p[0];
+    b = p[1];
+  } else {
+    const GlobalV4 p = (GlobalV4)(m.recs) + k;
+    a = p[0];
+    b = p[1];
+  }
+  RecHalf h;
+  h.bmin[0] = a.x;
+  h.bmin[1] = a.y;
+  h.bmin[2] = a.z;
+  h.bmax[0] = a.w;
+  h.bmax[1] = b.x;
+  h.bmax[2] = b.y;
+  h.ref = __float_as_uint(b.z);
+  h.next = __float_as_int(b.w);
+  return h;
+}
+__device__ __forceinline__ bool is_leaf(uint32_t ref) { return (ref & kLeafFlag) != 0; }
+__device__ __forceinline__ uint32_t leaf_first(uint32_t ref) { return ref & kLeafSlotMask; }
+__device__ __forceinline__ uint32_t leaf_count(uint32_t ref) {
+  return (ref & ~kLeafFlag) >> kLeafCountShift;
 }
 
 __device__ __forceinline__ void leaf_tri(const DevMesh& m, uint32_t slot, F3& v0, F3& v1, F3& v2,
@@ -328,20 +404,7 @@ __device__ __forceinline__ void leaf_tri(const DevMesh& m, uint32_t slot, F3& v0
   id = __float_as_uint(a.w);
 }
 
-constexpr int kStack = 64;
-
-__device__ __forceinline__ void push_children(const DevMesh& m, const BvhNode& nd, uint32_t ni,
-                                              const Ray& r, uint32_t* st, int& sp) {
-  const uint32_t o = m.order[ni];
-  const int axis = o & 3;
-  const bool nearFirst = (comp(r.d, axis) > 0) ? ((o >> 2) & 1) : ((o >> 3) & 1);
-  const uint32_t first = nearFirst ? nd.left : nd.right;
-  const uint32_t second = nearFirst ? nd.right : nd.left;
-  if (sp <= kStack - 2) {  // the reference's 64-entry stack; deeper trees are not traversed past it
-    st[sp++] = second;
-    st[sp++] = first;
-  }
-}
+constexpr int kStack = 64;  // the reference's traversal stack (bvh.cpp)
 
 // Traversal cost counters: a node is counted when its bounds are tested, a
 // triangle when its intersection or distance test runs.  NoCount compiles to
@@ -357,21 +420,28 @@ struct TravCount {
 };
 
 // BVH::intersect (bvh.cpp:197-250): front-to-back stack traversal from the root.
-template <class C = NoCount>
+// A node's slab test is made when its parent's record is read (both children
+// at once) and completed with the current farT when the node is popped, which
+// is the reference's test at the pop; the order, the counts and every decision
+// are the reference's.
+template <bool kTop = false, class C = NoCount>
 __device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, float& v,
                               C&& cnt = C()) {
   if (m.n_nodes == 0) return false;
   bool hit = false;
-  uint32_t st[kStack];
+  uint32_t sref[kStack];
+  float stmin[kStack];
   int sp = 0;
-  st[sp++] = m.root;
+  sref[sp] = m.root_ref;
+  stmin[sp++] = slab_tmin(m.lbmin, m.lbmax, r);
   while (sp > 0) {
-    const uint32_t ni = st[--sp];
-    const BvhNode nd = load_node(m.nodes, ni);
+    --sp;
+    const uint32_t ref = sref[sp];
+    const float tmin = stmin[sp];
     cnt.node();
-    if (!aabb_hit(nd, r)) continue;
-    if (nd.right & kLeafFlag) {
-      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+    if (!(tmin <= r.farT)) continue;
+    if (is_leaf(ref)) {
+      const uint32_t s0 = leaf_first(ref), c = leaf_count(ref);
       for (uint32_t i = s0; i < s0 + c; ++i) {
         cnt.tri();
         F3 v0, v1, v2;
@@ -386,43 +456,42 @@ __device__ bool bvh_intersect(const DevMesh& m, Ray& r, uint32_t& ti, float& u, 
           hit = true;
         }
       }
-    } else {
-      push_children(m, nd, ni, r, st, sp);
+    } else if (sp <= kStack - 2) {  // the reference's 64 entries; deeper trees stop there
+      const BvhRec R = load_rec<kTop>(m, ref);
+      const int axis = R.order & 3;
+      const bool nearFirst = (comp(r.d, axis) > 0) ? ((R.order >> 2) & 1) : ((R.order >> 3) & 1);
+      const float tl = slab_tmin(R.lmin, R.lmax, r), tr = slab_tmin(R.rmin, R.rmax, r);
+      // the far child first, the near one on top
+      sref[sp] = nearFirst ? R.r : R.l;
+      stmin[sp++] = nearFirst ? tr : tl;
+      sref[sp] = nearFirst ? R.l : R.r;
+      stmin[sp++] = nearFirst ? tl : tr;
     }
   }
   return hit;
 }
 
-// BVH::occluded (bvh.cpp:254-300): any hit, traversal starts at node 0.
+// BVH::occluded (bvh.cpp:254-300): any hit from node 0.  Node 0 of the
+// reference's post-order numbering is the first leaf it built, so the
+// traversal tests that leaf's bounds and its triangles only.
 __device__ bool bvh_occluded(const DevMesh& m, const Ray& r) {
   if (m.n_nodes == 0) return false;
-  uint32_t st[kStack];
-  int sp = 0;
-  st[sp++] = 0;
-  while (sp > 0) {
-    const uint32_t ni = st[--sp];
-    const BvhNode nd = load_node(m.nodes, ni);
-    if (!aabb_hit(nd, r)) continue;
-    if (nd.right & kLeafFlag) {
-      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
-      for (uint32_t i = s0; i < s0 + c; ++i) {
-        F3 v0, v1, v2;
-        uint32_t id;
-        leaf_tri(m, i, v0, v1, v2, id);
-        float t, uu, vv;
-        if (ray_tri(r, v0, v1, v2, t, uu, vv)) return true;
-      }
-    } else {
-      push_children(m, nd, ni, r, st, sp);
-    }
+  if (!(slab_tmin(m.occ_min, m.occ_max, r) <= r.farT)) return false;
+  const uint32_t s0 = leaf_first(m.occ_ref), c = leaf_count(m.occ_ref);
+  for (uint32_t i = s0; i < s0 + c; ++i) {
+    F3 v0, v1, v2;
+    uint32_t id;
+    leaf_tri(m, i, v0, v1, v2, id);
+    float t, uu, vv;
+    if (ray_tri(r, v0, v1, v2, t, uu, vv)) return true;
   }
   return false;
 }
 
-__device__ __forceinline__ float dist2_aabb(F3 p, const BvhNode& n) {
-  const float dx = fmaxf(fmaxf(n.bmin[0] - p.x, 0.0f), p.x - n.bmax[0]);
-  const float dy = fmaxf(fmaxf(n.bmin[1] - p.y, 0.0f), p.y - n.bmax[1]);
-  const float dz = fmaxf(fmaxf(n.bmin[2] - p.z, 0.0f), p.z - n.bmax[2]);
+__device__ __forceinline__ float dist2_aabb(F3 p, const float* bmin, const float* bmax) {
+  const float dx = fmaxf(fmaxf(bmin[0] - p.x, 0.0f), p.x - bmax[0]);
+  const float dy = fmaxf(fmaxf(bmin[1] - p.y, 0.0f), p.y - bmax[1]);
+  const float dz = fmaxf(fmaxf(bmin[2] - p.z, 0.0f), p.z - bmax[2]);
   return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 __device__ __forceinline__ float dist2_line(F3 p, F3 a, F3 b, F3& out) {
@@ -470,20 +539,26 @@ __device__ __forceinline__ float dist2_tri(F3 p, F3 a, F3 b, F3 c, F3& out) {
 // the minimum is still reached (every node above it is nearer than md0 and than
 // any distance found before it), so ti, out and the result are unchanged while
 // nodes farther than md0 are never visited.
-template <class C = NoCount>
+// A node's squared distance is computed when its parent's record is read and
+// kept beside it on the stack; the pop compares the kept value with the current
+// minimum, as the reference's recomputation at the pop does.
+template <bool kTop = false, class C = NoCount>
 __device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C(),
                              float md0 = 3.402823466e+38f) {
   if (m.n_nodes == 0) return 0.0f;
-  uint32_t st[kStack];
+  uint32_t sref[kStack];
+  float sd[kStack];
   int sp = 0;
-  st[sp++] = m.root;
+  sref[sp] = m.root_ref;
+  sd[sp++] = dist2_aabb(p, m.lbmin, m.lbmax);
   float md = md0;
   while (sp > 0) {
-    const BvhNode nd = load_node(m.nodes, st[--sp]);
+    --sp;
+    const uint32_t ref = sref[sp];
     cnt.node();
-    if (dist2_aabb(p, nd) >= md) continue;
-    if (nd.right & kLeafFlag) {
-      const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+    if (sd[sp] >= md) continue;
+    if (is_leaf(ref)) {
+      const uint32_t s0 = leaf_first(ref), c = leaf_count(ref);
       for (uint32_t i = s0; i < s0 + c; ++i) {
         cnt.tri();
         F3 v0, v1, v2;
@@ -498,17 +573,18 @@ __device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& 
         }
       }
     } else {
-      const float dl = dist2_aabb(p, load_node(m.nodes, nd.left));
-      const float dr = dist2_aabb(p, load_node(m.nodes, nd.right));
+      const BvhRec R = load_rec<kTop>(m, ref);
+      const float dl = dist2_aabb(p, R.lmin, R.lmax);
+      const float dr = dist2_aabb(p, R.rmin, R.rmax);
       cnt.node();
       cnt.node();
       if (sp > kStack - 2) continue;
       if (dl < dr) {
-        if (dr < md) st[sp++] = nd.right;
-        if (dl < md) st[sp++] = nd.left;
+        if (dr < md) { sref[sp] = R.r; sd[sp++] = dr; }
+        if (dl < md) { sref[sp] = R.l; sd[sp++] = dl; }
       } else {
-        if (dl < md) st[sp++] = nd.left;
-        if (dr < md) st[sp++] = nd.right;
+        if (dl < md) { sref[sp] = R.l; sd[sp++] = dl; }
+        if (dr < md) { sref[sp] = R.r; sd[sp++] = dr; }
       }
     }
   }
@@ -516,6 +592,7 @@ __device__ float bvh_closest(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& 
 }
 
 // Mesh::intersect (mesh.cpp:185-209): narrows r.farT on hit.
+template <bool kTop = false>
 __device__ bool mesh_intersect(const DevMesh& m, Ray& ray, uint32_t& ti, float& u, float& v) {
   Ray lr = ray;
   float tcorr = 1.0f;
@@ -528,7 +605,7 @@ __device__ bool mesh_intersect(const DevMesh& m, Ray& ray, uint32_t& ti, float& 
     lr.nearT = ray.nearT * len;
     lr.farT = ray.farT * len;
   }
-  const bool hit = bvh_intersect(m, lr, ti, u, v);
+  const bool hit = bvh_intersect<kTop>(m, lr, ti, u, v);
   if (hit) ray.farT = lr.farT * tcorr;
   return hit;
 }
@@ -544,7 +621,7 @@ __device__ bool mesh_occluded(const DevMesh& m, const Ray& ray) {
 }
 // Mesh::distToPoint (mesh.cpp:325-339)
 // bound_local: an upper bound of the local-space distance (INFINITY: none).
-template <class C = NoCount>
+template <bool kTop = false, class C = NoCount>
 __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C(),
                            float bound_local = INFINITY) {
   const F3 pl = xf_to_local(m.xf, p);
@@ -552,7 +629,7 @@ __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cn
   ti = 0;
   // widened so that it stays strictly above the rounded squared distance
   const float b = fmaf(bound_local, 1.0001f, 1e-5f);
-  bvh_closest(m, pl, ti, ol, cnt, fminf(b * b, 3.402823466e+38f));
+  bvh_closest<kTop>(m, pl, ti, ol, cnt, fminf(b * b, 3.402823466e+38f));
   out = xf_to_world(m.xf, ol);
   return norm(sub(out, p));
 }
@@ -564,46 +641,30 @@ __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cn
 // slab test computes it, so no cast culls the leaf of the hit it must find (its
 // ancestors' slabs contain the leaf's); (b) no two hits are within 2e-4, so each
 // cast's nearT = h + 1e-4 lies strictly between consecutive hits and the cast
-// finds exactly the next one; (c) the stack never ran out (the casts, culling
-// more, then never run out either); with skip links (m.skip) the walk has no
-// stack and the host built them only for trees shallow enough that no cast's
-// 64-entry stack can fill.  Otherwise the reference's loop of casts runs as
+// finds exactly the next one; (c) no cast's 64-entry stack fills, which holds
+// for trees of at most 60 levels (m.stackless: the near/far traversal keeps at
+// most one waiting sibling per level).  The single traversal is a left-first
+// pre-order walk without a stack: at a hit internal node it reads the node's
+// record and tests the left child, then the right one, and after a subtree it
+// continues with the right child of the record the subtree's `next` names.  It
+// tests the boxes the reference's casts would test first (both children of
+// every hit internal node).  Otherwise the reference's loop of casts runs as
 // written.
 // kOneTraversal false: the reference's loop of casts only (its traversal cost).
-template <bool kOneTraversal = true, class C = NoCount>
+template <bool kOneTraversal = true, bool kTop = false, class C = NoCount>
 __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
   const F3 pl = xf_to_local(m.xf, p);
   if (m.n_nodes == 0) return false;
   if constexpr (kOneTraversal) {
-    const Ray r = make_ray(pl, mk(1, 0, 0));
-    constexpr int kHits = 8;
-    float hs[kHits];
-    int nh = 0;
-    bool ok = true;
-    // With skip links the walk needs no stack: the next node is the left child
-    // of a hit internal node, else the node's skip link (loaded beside the node).
-    // Both walks test the same nodes (every child of a hit internal node).
-    const bool threaded = m.skip != nullptr;
-    uint32_t st[kStack];
-    int sp = 0;
-    st[sp++] = m.root;
-    int32_t next = (int32_t)m.root;
-    int steps = 0;
-    while ((threaded ? next >= 0 : sp > 0) && ok) {
-      const uint32_t ni = threaded ? (uint32_t)next : st[--sp];
-      const BvhNode nd = load_node(m.nodes, ni);
-      if (threaded) {
-        next = m.skip[ni];
-        if (++steps > m.n_nodes) {  // the links are checked on the host; a bound all the same
-          ok = false;
-          break;
-        }
-      }
-      cnt.node();
-      if (!aabb_hit(nd, r)) continue;
-      if (nd.right & kLeafFlag) {
-        const float lo = nd.bmin[0] - pl.x, hi = nd.bmax[0] - pl.x;  // the slab test's x range
-        const uint32_t s0 = nd.left, c = nd.right & ~kLeafFlag;
+    if (m.stackless) {
+      const Ray r = make_ray(pl, mk(1, 0, 0));
+      constexpr int kHits = 8;
+      float hs[kHits];
+      int nh = 0;
+      bool ok = true;
+      const auto leaf = [&](uint32_t ref, float bminx, float bmaxx) {
+        const float lo = bminx - pl.x, hi = bmaxx - pl.x;  // the slab test's x range
+        const uint32_t s0 = leaf_first(ref), c = leaf_count(ref);
         for (uint32_t i = s0; i < s0 + c; ++i) {
           cnt.tri();
           F3 v0, v1, v2;
@@ -620,26 +681,54 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
             ++nh;
           }
         }
-      } else if (threaded) {
-        next = (int32_t)nd.left;
-      } else if (sp <= kStack - 2) {
-        push_children(m, nd, ni, r, st, sp);
-      } else {
-        ok = false;
+      };
+      cnt.node();
+      if (slab_tmin(m.lbmin, m.lbmax, r) <= r.farT) {
+        if (is_leaf(m.root_ref)) {
+          leaf(m.root_ref, m.lbmin[0], m.lbmax[0]);
+        } else {
+          // the current child: record rec, left or right half
+          uint32_t rec = 0;
+          bool right = false;
+          RecHalf H = load_half<kTop>(m, rec, right);
+          uint32_t steps = 0;  // the links are checked on the host; a bound all the same
+          while (ok) {
+            cnt.node();
+            const bool h = slab_tmin(H.bmin, H.bmax, r) <= r.farT;
+            if (++steps > 2 * m.n_recs + 2) {
+              ok = false;
+            } else if (h && !is_leaf(H.ref)) {  // descend to the left child
+              rec = H.ref;
+              right = false;
+              H = load_half<kTop>(m, rec, false);
+            } else {
+              if (h) leaf(H.ref, H.bmin[0], H.bmax[0]);
+              if (!right) {  // the right child of the same record
+                right = true;
+                H = load_half<kTop>(m, rec, true);
+              } else if (H.next < 0) {
+                break;
+              } else {  // the right child of the record the subtree continues with
+                rec = (uint32_t)H.next;
+                H = load_half<kTop>(m, rec, true);
+              }
+            }
+          }
+        }
       }
+      if (ok) return (nh & 1) == 1;
     }
-    if (ok) return (nh & 1) == 1;
   }
   Ray r = make_ray(pl, mk(1, 0, 0));
   int hits = 0;
   uint32_t ti;
   float u, v;
-  if (bvh_intersect(m, r, ti, u, v, cnt)) {
+  if (bvh_intersect<kTop>(m, r, ti, u, v, cnt)) {
     ++hits;
     while (r.farT < INFINITY) {
       r.nearT = r.farT + 1e-4f;
       r.farT = INFINITY;
-      if (!bvh_intersect(m, r, ti, u, v, cnt)) break;
+      if (!bvh_intersect<kTop>(m, r, ti, u, v, cnt)) break;
       ++hits;
     }
   }
@@ -675,11 +764,11 @@ __device__ __forceinline__ bool mesh_in_parity_slab(const DevMesh& m, F3 pl) {
 
 // Mesh::distToPoint with the closest-point search started from the distance
 // grid's upper bound at p (same ti, closest point and distance; fewer nodes).
-template <class C = NoCount>
+template <bool kTop = false, class C = NoCount>
 __device__ float mesh_dist_grid(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cnt = C()) {
   float lo, hi;
   mesh_dist_bounds_local(m, xf_to_local(m.xf, p), lo, hi);
-  return mesh_dist(m, p, ti, out, cnt, hi);
+  return mesh_dist<kTop>(m, p, ti, out, cnt, hi);
 }
 
 __device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {

@@ -89,4 +89,95 @@ BvhResult build_bvh(const float* vertices, const int32_t* faces, uint32_t n_face
   return res;
 }
 
+namespace {
+uint32_t leaf_ref(const BvhNode& n) {
+  return kLeafFlag | (n.right & ~kLeafFlag) << kLeafCountShift | n.left;
+}
+}  // namespace
+
+std::vector<uint8_t> bvh_order(const BvhResult& b) {
+  const std::vector<BvhNode>& nodes = b.nodes;
+  std::vector<uint8_t> order(nodes.size(), 0);
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    const BvhNode& n = nodes[i];
+    if ((n.right & kLeafFlag) || n.left >= nodes.size() || n.right >= nodes.size()) continue;
+    const float dx = n.bmax[0] - n.bmin[0], dy = n.bmax[1] - n.bmin[1], dz = n.bmax[2] - n.bmin[2];
+    const int axis = (dx > dy && dx > dz) ? 0 : (dy > dz ? 1 : 2);
+    const BvhNode& l = nodes[n.left];
+    const BvhNode& r = nodes[n.right];
+    const float lc = (l.bmin[axis] + l.bmax[axis]) * 0.5f;
+    const float rc = (r.bmin[axis] + r.bmax[axis]) * 0.5f;
+    order[i] = (uint8_t)(axis | ((lc < rc) ? 4 : 0) | ((lc > rc) ? 8 : 0));
+  }
+  return order;
+}
+
+bool build_records(const BvhResult& b, const std::vector<uint8_t>& order, BvhRecords& out) {
+  out = BvhRecords{};
+  const size_t n = b.nodes.size();
+  if (n == 0) return true;
+  if (b.root >= n || order.size() != n || n > (size_t)INT32_MAX) return false;
+  for (const BvhNode& nd : b.nodes)
+    if ((nd.right & kLeafFlag) &&
+        ((nd.right & ~kLeafFlag) > 7u || nd.left > kLeafSlotMask ||
+         (nd.right & ~kLeafFlag) + (uint64_t)nd.left > (uint64_t)kLeafSlotMask + 1))
+      return false;
+  // node 0 is the first node the reference's post-order build creates: a leaf
+  if (!(b.nodes[0].right & kLeafFlag)) return false;
+  out.occ_ref = leaf_ref(b.nodes[0]);
+  for (int a = 0; a < 3; ++a) {
+    out.occ_min[a] = b.nodes[0].bmin[a];
+    out.occ_max[a] = b.nodes[0].bmax[a];
+  }
+  const BvhNode& root = b.nodes[b.root];
+  if (root.right & kLeafFlag) {
+    out.root_ref = leaf_ref(root);
+    return true;
+  }
+  // breadth-first numbering of the internal nodes
+  std::vector<int32_t> rec(n, -1);
+  std::vector<uint32_t> bfs{b.root};
+  std::vector<int> level{1};
+  rec[b.root] = 0;
+  for (size_t q = 0; q < bfs.size(); ++q) {
+    const BvhNode& nd = b.nodes[bfs[q]];
+    for (uint32_t c : {nd.left, nd.right}) {
+      if (c >= n || rec[c] != -1 || c == b.root) return false;
+      if (b.nodes[c].right & kLeafFlag) {
+        rec[c] = -2;  // reached once
+        continue;
+      }
+      rec[c] = (int32_t)bfs.size();
+      bfs.push_back(c);
+      level.push_back(level[q] + 1);
+    }
+  }
+  out.recs.resize(bfs.size());
+  for (size_t q = 0; q < bfs.size(); ++q) {
+    const BvhNode& nd = b.nodes[bfs[q]];
+    const BvhNode& l = b.nodes[nd.left];
+    const BvhNode& r = b.nodes[nd.right];
+    BvhRec& R = out.recs[q];
+    for (int a = 0; a < 3; ++a) {
+      R.lmin[a] = l.bmin[a];
+      R.lmax[a] = l.bmax[a];
+      R.rmin[a] = r.bmin[a];
+      R.rmax[a] = r.bmax[a];
+    }
+    R.l = (l.right & kLeafFlag) ? leaf_ref(l) : (uint32_t)rec[nd.left];
+    R.r = (r.right & kLeafFlag) ? leaf_ref(r) : (uint32_t)rec[nd.right];
+    R.order = order[bfs[q]];
+    out.depth = std::max(out.depth, level[q]);
+  }
+  // next: a left child continues with its parent's right child, a right child
+  // with whatever its parent continues with (breadth-first order visits parents first)
+  out.recs[0].next = -1;
+  for (size_t q = 0; q < bfs.size(); ++q) {
+    const BvhRec& R = out.recs[q];
+    if (!(R.l & kLeafFlag)) out.recs[R.l].next = (int32_t)q;
+    if (!(R.r & kLeafFlag)) out.recs[R.r].next = R.next;
+  }
+  return true;
+}
+
 }  // namespace mupsi

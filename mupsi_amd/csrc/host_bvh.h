@@ -29,4 +29,46 @@ struct BvhResult {
 // vertices: n_vertices x 3 floats; faces: n_faces x 3 vertex indices.
 BvhResult build_bvh(const float* vertices, const int32_t* faces, uint32_t n_faces);
 
+// Per node, the near/far order BVH::intersect pushes an internal node's children
+// in (bvh.cpp:228-245): axis | (lc < rc) << 2 | (lc > rc) << 3, with axis the
+// longest axis of the node's bounds and lc, rc the children's box centres on
+// it; 0 for leaves.
+std::vector<uint8_t> bvh_order(const BvhResult& b);
+
+// The device layout of the same tree: one 64-byte record per internal node
+// holding the bounds of both its children side by side, so that a traversal
+// step reads the two boxes it tests with one record load (a walk that tests one
+// child at a time reads the 32-byte half of that child).  Records are
+// numbered breadth-first from the root (record 0), so the top levels are the
+// first records (the part a kernel may copy into LDS).  A child reference is a
+// record index, or for a leaf kLeafFlag | count << 28 | first triangle slot.
+struct BvhRec {
+  // first half: the left child
+  float lmin[3], lmax[3];  // its bounds
+  uint32_t l;              // its reference
+  uint32_t order;          // axis | (lc < rc) << 2 | (lc > rc) << 3 (near/far order)
+  // second half: the right child
+  float rmin[3], rmax[3];
+  uint32_t r;
+  int32_t next;            // the record whose right child a left-first pre-order
+                           // walk visits once this subtree is done (-1: none)
+};
+static_assert(sizeof(BvhRec) == 64, "BvhRec must stay 64 bytes");
+
+constexpr uint32_t kLeafCountShift = 28;
+constexpr uint32_t kLeafSlotMask = (1u << kLeafCountShift) - 1u;
+
+struct BvhRecords {
+  std::vector<BvhRec> recs;
+  uint32_t root_ref = 0;   // reference of the root (a leaf for meshes of <= 4 triangles)
+  uint32_t occ_ref = 0;    // reference of node 0, where BVH::occluded starts (a leaf)
+  float occ_min[3] = {0, 0, 0}, occ_max[3] = {0, 0, 0};
+  int depth = 0;           // levels of internal nodes on the longest root-to-leaf path
+};
+
+// Records of b (order: the near/far byte of each internal node, indexed like
+// b.nodes).  False if b is not a tree rooted at b.root with every node reached
+// once, or a leaf slot does not fit the reference encoding.
+bool build_records(const BvhResult& b, const std::vector<uint8_t>& order, BvhRecords& out);
+
 }  // namespace mupsi

@@ -18,6 +18,7 @@ struct mupsi_ctx {
   int cus = 0;
   int march_blocks = 0;         // persistent grid of k_march
   int march_cached_blocks = 0;  // persistent grid of k_march_window
+  int query_blocks = 0;         // resident blocks of the LDS-staging mesh-query kernels
   std::vector<mupsi_scene*> scenes;  // live scenes, freed by mupsi_close
   std::vector<mupsi_guide*> guides;  // live guides, freed by mupsi_close
   std::vector<mupsi_batch*> batches; // live batches, freed by mupsi_close

@@ -113,19 +113,7 @@ bool lower_mesh(const mupsi_mesh_desc& d, HostMesh* m, std::string* err) {
     for (float& c : m->face_cdf) c /= total;
   m->total_area = total;
   m->bvh = build_bvh(d.vertices, d.faces, (uint32_t)d.n_faces);
-  const std::vector<BvhNode>& nodes = m->bvh.nodes;
-  m->order.assign(nodes.size(), 0);
-  for (size_t i = 0; i < nodes.size(); ++i) {
-    const BvhNode& n = nodes[i];
-    if (n.right & kLeafFlag) continue;
-    const float dx = n.bmax[0] - n.bmin[0], dy = n.bmax[1] - n.bmin[1], dz = n.bmax[2] - n.bmin[2];
-    const int axis = (dx > dy && dx > dz) ? 0 : (dy > dz ? 1 : 2);
-    const BvhNode& l = nodes[n.left];
-    const BvhNode& r = nodes[n.right];
-    const float lc = (l.bmin[axis] + l.bmax[axis]) * 0.5f;
-    const float rc = (r.bmin[axis] + r.bmax[axis]) * 0.5f;
-    m->order[i] = (uint8_t)(axis | ((lc < rc) ? 4 : 0) | ((lc > rc) ? 8 : 0));
-  }
+  m->order = bvh_order(m->bvh);
   m->leaf_tris.resize(3 * m->bvh.tri_index.size());
   for (size_t s = 0; s < m->bvh.tri_index.size(); ++s) {
     const uint32_t ti = m->bvh.tri_index[s];

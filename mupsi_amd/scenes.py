@@ -297,3 +297,20 @@ def generated_batch(seed: int = BATCH_SEED, n: int = 64, max_res: int = 512,
         out.append(sc)
     return out
 
+
+
+def chain_mesh(n: int = 70, ratio: float = 2.05, x0: float = 1e-9) -> dict:
+    """n flat triangles facing +x at x_i = x0 ratio^i, each spanning 0.1 x_i in y
+    and z: every centroid-midpoint split of the reference's BVH build peels off
+    the farthest triangle, so the tree is about n levels deep (a pathological
+    case for the 64-entry traversal stack)."""
+    x = x0 * ratio ** np.arange(n, dtype=np.float64)
+    v = np.zeros((3 * n, 3), np.float64)
+    v[0::3, 0] = x
+    v[1::3, 0] = x
+    v[1::3, 1] = 0.1 * x
+    v[2::3, 0] = x
+    v[2::3, 2] = 0.1 * x
+    f = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    return {"vertices": v.astype(np.float32), "faces": f, "normals": None, "texcoords": None,
+            "face_tex": None}

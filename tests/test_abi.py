@@ -86,3 +86,44 @@ def test_bvh_build_rejects_bad_faces():
     st = lib.mupsi_bvh_build(v.ctypes.data_as(C.c_void_p), 3, f.ctypes.data_as(C.c_void_p), 1,
                              None, C.byref(nn), None, None)
     assert st == 1
+
+
+def _layout(m):
+    lib = _abi.load()
+    v = np.ascontiguousarray(m["vertices"], np.float32)
+    f = np.ascontiguousarray(m["faces"], np.int32)
+    nr, depth, sl = C.c_int32(), C.c_int32(), C.c_int32()
+    st = lib.mupsi_bvh_layout(v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p),
+                              len(f), C.byref(nr), C.byref(depth), C.byref(sl))
+    assert st == 0
+    return nr.value, depth.value, sl.value
+
+
+def _ref_depth(nodes, root):
+    """Levels of internal nodes on the longest root-to-leaf path of a post-order tree."""
+    u = nodes.view(np.uint32)
+    best, st = 0, [(root, 0)]
+    while st:
+        i, d = st.pop()
+        if u[i, 7] & 0x80000000:
+            best = max(best, d)
+        else:
+            st += [(u[i, 6], d + 1), (u[i, 7], d + 1)]
+    return best
+
+
+@pytest.mark.parametrize("which", ["ico4", "ico7", "chain"])
+def test_bvh_device_layout(which):
+    """One 64-byte record per internal node of the reference tree; the walk
+    without a stack only for trees of at most 60 levels (a 70-triangle chain
+    built by the midpoint split is deeper and keeps the reference's casts)."""
+    m = scenes.chain_mesh() if which == "chain" else scenes.icosphere(int(which[3:]), displacement=0.05)
+    nodes, _, root = O.bvh_build(m["vertices"], m["faces"])
+    internal = int(np.count_nonzero((nodes.view(np.uint32)[:, 7] & 0x80000000) == 0))
+    nr, depth, sl = _layout(m)
+    assert nr == internal
+    assert depth == _ref_depth(nodes, root)
+    assert sl == (1 if depth <= 60 else 0)
+    assert (which == "chain") == (sl == 0)
+    if which == "chain":
+        assert depth > 60

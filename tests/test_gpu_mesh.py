@@ -121,3 +121,43 @@ def test_traversal_cost_equals_oracle(ctx, dev, prim):
     n3, t3 = (x.cpu().numpy().astype(np.int64) for x in batch.mesh_query_cost(s, prim, 3, pt))
     n1, t1 = (x.cpu().numpy().astype(np.int64) for x in batch.mesh_query_cost(s, prim, 1, pt))
     assert 0 < n3.sum() < n1.sum() and t3.sum() > 0
+
+
+def test_deep_tree_queries_bit_exact(ctx, dev):
+    """A BVH deeper than 60 levels (a 70-triangle chain): inside() runs the
+    reference's casts there (its 64-entry stacks may fill, and the walk without
+    a stack is used only where they cannot); inside bits, hit triangles and t
+    bits against the oracle, whose casts fill the same stack the same way."""
+    import torch
+    from mupsi_amd import Scene, batch
+    m = scenes.chain_mesh()
+    sc = {"width": 8, "height": 8, "camera": dict(scenes.CAMERA), "max_bounce": 2,
+          "primitives": [{"type": "mesh", "mesh": m, "bsdf": "lambertian"}]}
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(21)
+    n = 4000
+    x = m["vertices"][::3, 0].astype(np.float64)
+    px = x[rng.integers(0, len(x), n)] * rng.uniform(0.3, 1.2, n)
+    p = np.stack([px, rng.uniform(-0.01, 0.05, n) * px, rng.uniform(-0.01, 0.05, n) * px], 1)
+    p = p.astype(np.float32)
+    pt = torch.from_numpy(p).to(dev)
+    ins = batch.mesh_inside(s, 0, pt).cpu().numpy()
+    assert np.array_equal(ins, osc.mesh_inside(0, p))
+    assert 0.2 < ins.mean() < 0.8
+    n1, t1 = (a.cpu().numpy() for a in batch.mesh_query_cost(s, 0, 1, pt))
+    n3, t3 = (a.cpu().numpy() for a in batch.mesh_query_cost(s, 0, 3, pt))
+    on1, ot1 = osc.mesh_query_cost(0, 1, p)
+    assert np.array_equal(n1, on1) and np.array_equal(t1, ot1)
+    assert np.array_equal(n3, n1) and np.array_equal(t3, t1)   # casts: no single walk here
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = p
+    rays[:, 3] = 1.0
+    rays[:, 4:6] = rng.normal(0, 0.01, (n, 2))
+    rays[:, 7] = np.finfo(np.float32).max
+    tri, t, u, v = (a.cpu().numpy() for a in batch.mesh_intersect(s, 0, torch.from_numpy(rays).to(dev)))
+    otri, ot, ou, ov = osc.mesh_intersect(0, rays)
+    assert np.array_equal(tri, otri)
+    h = tri >= 0
+    assert h.sum() > n // 4
+    assert np.array_equal(t[h].view(np.uint32), ot[h].view(np.uint32))
