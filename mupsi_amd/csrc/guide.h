@@ -44,18 +44,18 @@ constexpr int kOffW4 = kOffB3 + kHid;          // 10432
 constexpr int kOffB4 = kOffW4 + kOut * kHid;   // 13056
 constexpr int kNumParams = kOffB4 + kOut;      // 13097
 
-// MFMA 16x16x16 BF16 fragments (64 lanes x 4 bf16 each).  Forward: A = W_l tiles
-// (out-tile mt, in-tile kt); transposed: A = W_l^T tiles (in-tile mt, out-tile kt)
-// for the backward pass of layers 2-4.
-constexpr int kFragW1 = 0;                 // 4 x 2
-constexpr int kFragW2 = kFragW1 + 8;       // 4 x 4
-constexpr int kFragW3 = kFragW2 + 16;      // 4 x 4
-constexpr int kFragW4 = kFragW3 + 16;      // 3 x 4 (rows 41..47 are zero)
-constexpr int kNumFragF = kFragW4 + 12;    // 52
-constexpr int kFragT2 = 0;                 // W2^T: 4 x 4
-constexpr int kFragT3 = kFragT2 + 16;      // W3^T: 4 x 4
-constexpr int kFragT4 = kFragT3 + 16;      // W4^T: 4 x 3 (columns 41..47 zero)
-constexpr int kNumFragT = kFragT4 + 12;    // 44
+// MFMA 16x16x32 BF16 fragments (64 lanes x 8 bf16 each).  Forward: A = W_l tiles
+// (out-tile mt of 16 rows, in-tile kt of 32 columns); transposed: A = W_l^T tiles
+// (in-tile mt, out-tile kt) for the backward pass of layers 2-4.
+constexpr int kFragW1 = 0;                 // 4 x 1
+constexpr int kFragW2 = kFragW1 + 4;       // 4 x 2
+constexpr int kFragW3 = kFragW2 + 8;       // 4 x 2
+constexpr int kFragW4 = kFragW3 + 8;       // 3 x 2 (rows 41..47 are zero)
+constexpr int kNumFragF = kFragW4 + 6;     // 26
+constexpr int kFragT2 = 0;                 // W2^T: 4 x 2
+constexpr int kFragT3 = kFragT2 + 8;       // W3^T: 4 x 2
+constexpr int kFragT4 = kFragT3 + 8;       // W4^T: 4 x 2 (columns 41..63 are zero)
+constexpr int kNumFragT = kFragT4 + 8;     // 24
 
 constexpr float kLogKappaMin = -3.0f;  // kappa in (0.0498, 2981)
 constexpr float kLogKappaMax = 8.0f;
@@ -65,6 +65,7 @@ __device__ __forceinline__ float log_kappa(float s) {
 }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));  // 8 BF16 bit patterns
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (inputs are finite).
@@ -90,17 +91,24 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
 }
 
 // Where W_l[o][i] lives in the fragment arrays (element index in units of bf16).
+// In a 32-wide k-tile, lane group g (lanes 16g .. 16g+15) holds k = 4g .. 4g+3
+// in elements 0-3 and k = 16+4g .. 16+4g+3 in elements 4-7: the rows that the
+// 16x16 D tiles 2kt and 2kt+1 of a layer leave in lane group g (a D tile puts
+// rows 4g .. 4g+3 of column s in lane s + 16g), so a layer's output tiles are
+// the next layer's B operand as they stand, with no data moved between lanes.
+// The B operand uses the same k order, so the products pair up as the sum
+// needs them.
 __host__ __device__ __forceinline__ int frag_elem(int frag_base, int nkt, int row, int col) {
-  const int mt = row >> 4, r = row & 15, kt = col >> 4, c = col & 15;
-  const int lane = r + 16 * (c >> 2);
-  return ((frag_base + mt * nkt + kt) * 64 + lane) * 4 + (c & 3);
+  const int mt = row >> 4, r = row & 15, kt = col >> 5, c = col & 31;
+  const int lane = r + 16 * ((c & 15) >> 2);
+  return ((frag_base + mt * nkt + kt) * 64 + lane) * 8 + (c & 3) + ((c >> 4) << 2);
 }
 
 // Device view of one network.
 struct NetDev {
   const float* theta;   // FP32 master parameters (biases are read from here)
-  const bf16x4* wf;     // kNumFragF x 64 forward fragments
-  const bf16x4* wt;     // kNumFragT x 64 transposed fragments
+  const bf16x8* wf;     // kNumFragF x 64 forward fragments
+  const bf16x8* wt;     // kNumFragT x 64 transposed fragments
 };
 
 // Position normalisation for the encoding: p_hat = clamp((p - c) / R, -1, 1).

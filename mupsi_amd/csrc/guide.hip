@@ -1,17 +1,19 @@
 // guide.hip — the guiding network on MFMA: inference, fused forward + backward
 // with MFMA weight gradients, Adam, and the C ABI of the guide object.
 //
-// Layout on the matrix cores (v_mfma_f32_16x16x16_bf16, 64 lanes; lane l,
-// s = l % 16, g = l / 16):
-//   A fragment (16 x 16): lane holds A[s][4g .. 4g+3]
-//   B fragment (16 x 16): lane holds B[4g .. 4g+3][s]
+// Layout on the matrix cores (v_mfma_f32_16x16x32_bf16 of gfx950, 64 lanes;
+// lane l, s = l % 16, g = l / 16):
+//   A fragment (16 x 32): lane holds A[s][k] for the 8 k of lane group g
+//   B fragment (32 x 16): lane holds B[k][s] for the same 8 k
 //   D (16 x 16, FP32):    lane holds D[4g .. 4g+3][s]
 // The forward pass computes Z^T = W X^T (weights as A, a 16-sample tile of
-// activations as B), so the D tile of layer l is, element for element, the B
-// fragment of layer l+1: activations stay in registers between layers.  The
-// backward pass does the same with W^T fragments.  Weight gradients
-// dW = dZ H^T take the batch as the contraction dimension; dZ and H are staged
-// in LDS as [feature][sample] BF16 and read back as A / B fragments.
+// activations as B).  A lane group's 8 k are rows 4g .. 4g+3 of two 16-row D
+// tiles (the fragment k order of guide.h's frag_elem), so the D tiles of layer
+// l, two at a time, are the B fragments of layer l+1: activations stay in
+// registers between layers.  The backward pass does the same with W^T
+// fragments.  Weight gradients dW = dZ H^T take the batch as the contraction
+// dimension; dZ and H are staged in LDS as [feature][sample] BF16 and read back
+// as A / B fragments of 8 consecutive samples per lane group.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -28,8 +30,14 @@ using namespace mupsi::guide;
 
 namespace {
 
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+typedef __bf16 vbf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vbf16x8, a),
+                                                 __builtin_bit_cast(vbf16x8, b), c, 0, 0, 0);
+}
+// Two 16-row tiles as one 32-wide k operand (guide.h's fragment k order).
+__device__ __forceinline__ bf16x8 cat(bf16x4 lo, bf16x4 hi) {
+  return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -41,17 +49,20 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   return x;
 }
 
-// One hidden layer of the forward pass: 64 outputs from NK k-tiles.
+// One hidden layer of the forward pass: 64 outputs from NK 16-row input tiles
+// (NK / 2 k-tiles of 32).
 template <int NK>
-__device__ __forceinline__ void hidden_layer(const bf16x4* __restrict__ wf, int frag0,
+__device__ __forceinline__ void hidden_layer(const bf16x8* __restrict__ wf, int frag0,
                                              const float* __restrict__ bias, const bf16x4* in,
                                              bf16x4* out, int lane) {
   const int g = lane >> 4;
+  constexpr int KT = NK / 2;
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < NK; ++kt) z = mfma(wf[(frag0 + mt * NK + kt) * 64 + lane], in[kt], z);
+    for (int kt = 0; kt < KT; ++kt)
+      z = mfma(wf[(frag0 + mt * KT + kt) * 64 + lane], cat(in[2 * kt], in[2 * kt + 1]), z);
     const int r = 16 * mt + 4 * g;
     out[mt] = pack4(fmaxf(z[0] + bias[r], 0.f), fmaxf(z[1] + bias[r + 1], 0.f),
                     fmaxf(z[2] + bias[r + 2], 0.f), fmaxf(z[3] + bias[r + 3], 0.f));
@@ -59,31 +70,56 @@ __device__ __forceinline__ void hidden_layer(const bf16x4* __restrict__ wf, int 
 }
 
 // Output layer: 3 tiles of 16 rows (rows 41..47 are padding and read as 0).
-__device__ __forceinline__ void output_layer(const bf16x4* __restrict__ wf,
-                                             const float* __restrict__ theta, const bf16x4* h3,
+__device__ __forceinline__ void output_layer(const bf16x8* __restrict__ wf,
+                                             const float* __restrict__ b4, const bf16x4* h3,
                                              f32x4* o, int lane) {
   const int g = lane >> 4;
 #pragma unroll
   for (int mt = 0; mt < 3; ++mt) {
     f32x4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) z = mfma(wf[(kFragW4 + mt * 4 + kt) * 64 + lane], h3[kt], z);
+    for (int kt = 0; kt < 2; ++kt)
+      z = mfma(wf[(kFragW4 + mt * 2 + kt) * 64 + lane], cat(h3[2 * kt], h3[2 * kt + 1]), z);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = 16 * mt + 4 * g + i;
-      o[mt][i] = r < kOut ? z[i] + theta[kOffB4 + r] : 0.0f;
+      o[mt][i] = r < kOut ? z[i] + b4[r] : 0.0f;
     }
   }
 }
 
-__device__ __forceinline__ void forward_tile(const NetDev& net, bf16x4 x0, bf16x4 x1,
-                                             bf16x4* h1, bf16x4* h2, bf16x4* h3, f32x4* o,
-                                             int lane) {
+// The network as a block keeps it in LDS: the forward fragments, the
+// transposed fragments (training only) and the biases b1 | b2 | b3 | b4.  A
+// layer then waits on LDS, not on a global load, for its weights.
+constexpr int kBiasLds = 3 * kHid + kOutPad;
+struct NetLds {
+  bf16x8 wf[kNumFragF * 64];
+  float bias[kBiasLds];
+};
+struct NetLdsT {
+  bf16x8 wf[kNumFragF * 64];
+  bf16x8 wt[kNumFragT * 64];
+  float bias[kBiasLds];
+};
+__device__ __forceinline__ void stage_bias(const NetDev& net, float* bias) {
+  for (int i = threadIdx.x; i < kBiasLds; i += blockDim.x) {
+    const int l = i / kHid, r = i % kHid;
+    bias[i] = l < 3 ? net.theta[(l == 0 ? kOffB1 : l == 1 ? kOffB2 : kOffB3) + r]
+                    : (r < kOut ? net.theta[kOffB4 + r] : 0.0f);
+  }
+}
+__device__ __forceinline__ void stage_frags(const bf16x8* src, bf16x8* dst, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__device__ __forceinline__ void forward_tile(const bf16x8* wf, const float* bias, bf16x4 x0,
+                                             bf16x4 x1, bf16x4* h1, bf16x4* h2, bf16x4* h3,
+                                             f32x4* o, int lane) {
   const bf16x4 x[2] = {x0, x1};
-  hidden_layer<2>(net.wf, kFragW1, net.theta + kOffB1, x, h1, lane);
-  hidden_layer<4>(net.wf, kFragW2, net.theta + kOffB2, h1, h2, lane);
-  hidden_layer<4>(net.wf, kFragW3, net.theta + kOffB3, h2, h3, lane);
-  output_layer(net.wf, net.theta, h3, o, lane);
+  hidden_layer<2>(wf, kFragW1, bias, x, h1, lane);
+  hidden_layer<4>(wf, kFragW2, bias + kHid, h1, h2, lane);
+  hidden_layer<4>(wf, kFragW3, bias + 2 * kHid, h2, h3, lane);
+  output_layer(wf, bias + 3 * kHid, h3, o, lane);
 }
 
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
@@ -101,6 +137,10 @@ struct InferArgs {
 
 __global__ __launch_bounds__(256) void k_mlp_infer(NetDev net, InferArgs a) {
   __shared__ float Ow[4][kOutPad][17];
+  __shared__ NetLds N;
+  stage_frags(net.wf, N.wf, kNumFragF * 64);
+  stage_bias(net, N.bias);
+  __syncthreads();
   const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
   // every wave of a block runs the same number of iterations (block tiles of 64
@@ -115,8 +155,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(NetDev net, InferArgs a) {
     }
     bf16x4 h1[4], h2[4], h3[4];
     f32x4 o[3];
-    forward_tile(net, pack4(f0.x, f0.y, f0.z, f0.w), pack4(f1.x, f1.y, f1.z, f1.w), h1, h2, h3, o,
-                 lane);
+    forward_tile(N.wf, N.bias, pack4(f0.x, f0.y, f0.z, f0.w), pack4(f1.x, f1.y, f1.z, f1.w), h1,
+                 h2, h3, o, lane);
 #pragma unroll
     for (int mt = 0; mt < 3; ++mt)
 #pragma unroll
@@ -197,30 +237,31 @@ __device__ __forceinline__ GradTile grad_tile(int t) {
   return r;
 }
 
-__device__ __forceinline__ bf16x4 lds_frag(const uint16_t (*L)[kLdsStride], int row, int col) {
-  const uint2 v = *reinterpret_cast<const uint2*>(&L[row][col]);
-  bf16x4 r;
-  r[0] = (short)(v.x & 0xffffu);
-  r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffffu);
-  r[3] = (short)(v.y >> 16);
-  return r;
+// 8 consecutive BF16 of an LDS row (16-byte aligned: col is a multiple of 8 and
+// a row is 144 bytes).
+__device__ __forceinline__ bf16x8 lds_frag(const uint16_t (*L)[kLdsStride], int row, int col) {
+  return *reinterpret_cast<const bf16x8*>(&L[row][col]);
 }
 __device__ __forceinline__ void lds_put4(uint16_t (*L)[kLdsStride], int row0, int col, bf16x4 v) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) L[row0 + i][col] = (uint16_t)v[i];
 }
 
-// Backward through one hidden layer: dZ_prev = (W^T dZ) masked by h_prev != 0.
+// Backward through one hidden layer: dZ_prev = (W^T dZ) masked by h_prev != 0;
+// NK 16-row tiles of dZ (the last k-tile of 32 is padded with zeros for odd NK).
 template <int NK>
-__device__ __forceinline__ void backward_layer(const bf16x4* __restrict__ wt, int frag0,
+__device__ __forceinline__ void backward_layer(const bf16x8* __restrict__ wt, int frag0,
                                                const bf16x4* dz, const bf16x4* hprev,
                                                bf16x4* dzprev, int lane) {
+  constexpr int KT = (NK + 1) / 2;
+  const bf16x4 zero4 = {0, 0, 0, 0};
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < NK; ++kt) z = mfma(wt[(frag0 + mt * NK + kt) * 64 + lane], dz[kt], z);
+    for (int kt = 0; kt < KT; ++kt)
+      z = mfma(wt[(frag0 + mt * KT + kt) * 64 + lane],
+               cat(dz[2 * kt], 2 * kt + 1 < NK ? dz[2 * kt + 1] : zero4), z);
     dzprev[mt] = pack4(hprev[mt][0] != 0 ? z[0] : 0.f, hprev[mt][1] != 0 ? z[1] : 0.f,
                        hprev[mt][2] != 0 ? z[2] : 0.f, hprev[mt][3] != 0 ? z[3] : 0.f);
   }
@@ -230,6 +271,11 @@ __global__ __launch_bounds__(256) void k_mlp_train(NetDev net, TrainArgs a) {
   __shared__ __align__(16) uint16_t L[kRowsLds][kLdsStride];
   __shared__ float Ow[4][kOutPad][17];
   __shared__ float lsum[4];
+  __shared__ NetLdsT N;
+  stage_frags(net.wf, N.wf, kNumFragF * 64);
+  stage_frags(net.wt, N.wt, kNumFragT * 64);
+  stage_bias(net, N.bias);
+  __syncthreads();
   const int n = a.n_dev ? *a.n_dev : a.n_host;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
   const int col = wave * 16 + s;
@@ -259,7 +305,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(NetDev net, TrainArgs a) {
     lds_put4(L, kRowX + 16 + 4 * g, col, x1);
     bf16x4 h1[4], h2[4], h3[4];
     f32x4 o[3];
-    forward_tile(net, x0, x1, h1, h2, h3, o, lane);
+    forward_tile(N.wf, N.bias, x0, x1, h1, h2, h3, o, lane);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       lds_put4(L, kRowH1 + 16 * mt + 4 * g, col, h1[mt]);
@@ -300,9 +346,9 @@ __global__ __launch_bounds__(256) void k_mlp_train(NetDev net, TrainArgs a) {
       lds_put4(L, kRowD4 + r, col, dO[kt]);
     }
     bf16x4 d3[4], d2[4], d1[4];
-    backward_layer<3>(net.wt, kFragT4, dO, h3, d3, lane);
-    backward_layer<4>(net.wt, kFragT3, d3, h2, d2, lane);
-    backward_layer<4>(net.wt, kFragT2, d2, h1, d1, lane);
+    backward_layer<3>(N.wt, kFragT4, dO, h3, d3, lane);
+    backward_layer<4>(N.wt, kFragT3, d3, h2, d2, lane);
+    backward_layer<4>(N.wt, kFragT2, d2, h1, d1, lane);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       lds_put4(L, kRowD3 + 16 * mt + 4 * g, col, d3[mt]);
@@ -315,18 +361,23 @@ __global__ __launch_bounds__(256) void k_mlp_train(NetDev net, TrainArgs a) {
     for (int tt = 0; tt < 13; ++tt) {
       const GradTile gt = grad_tile(wave + 4 * tt);
 #pragma unroll
-      for (int sg = 0; sg < 4; ++sg) {
-        const bf16x4 A = lds_frag(L, gt.arow + s, 16 * sg + 4 * g);
-        const bf16x4 B = lds_frag(L, gt.brow + s, 16 * sg + 4 * g);
+      for (int sg = 0; sg < 2; ++sg) {  // samples 32 sg + 8 g .. + 7 of the tile
+        const bf16x8 A = lds_frag(L, gt.arow + s, 32 * sg + 8 * g);
+        const bf16x8 B = lds_frag(L, gt.brow + s, 32 * sg + 8 * g);
         acc[tt] = mfma(A, B, acc[tt]);
       }
     }
     // bias gradients: rows D1 (64), D2 (64), D3 (64), D4 (41)
-    if (threadIdx.x < 3 * kHid + kOut) {
+    if (threadIdx.x < 3 * kHid + kOut) {  // 8 reads of 8 BF16 (in column order)
       const int t = threadIdx.x;
       const int row = t < 192 ? kRowD1 + t : kRowD4 + (t - 192);
       float sacc = 0.f;
-      for (int c = 0; c < 64; ++c) sacc += bf2f(L[row][c]);
+#pragma unroll
+      for (int c = 0; c < 64; c += 8) {
+        const bf16x8 v = lds_frag(L, row, c);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sacc += bf2f((uint16_t)v[e]);
+      }
       bacc += sacc;
     }
     __syncthreads();
@@ -357,19 +408,19 @@ __global__ __launch_bounds__(256) void k_mlp_train(NetDev net, TrainArgs a) {
 __device__ __forceinline__ void store_frags(int p, float th, uint16_t* wf, uint16_t* wt) {
   const uint16_t b = f2bf(th);
   if (p < kOffB1) {
-    wf[frag_elem(kFragW1, 2, p / kIn, p % kIn)] = b;
+    wf[frag_elem(kFragW1, 1, p / kIn, p % kIn)] = b;
   } else if (p >= kOffW2 && p < kOffB2) {
     const int q = p - kOffW2, r = q / kHid, c = q % kHid;
-    wf[frag_elem(kFragW2, 4, r, c)] = b;
-    wt[frag_elem(kFragT2, 4, c, r)] = b;
+    wf[frag_elem(kFragW2, 2, r, c)] = b;
+    wt[frag_elem(kFragT2, 2, c, r)] = b;
   } else if (p >= kOffW3 && p < kOffB3) {
     const int q = p - kOffW3, r = q / kHid, c = q % kHid;
-    wf[frag_elem(kFragW3, 4, r, c)] = b;
-    wt[frag_elem(kFragT3, 4, c, r)] = b;
+    wf[frag_elem(kFragW3, 2, r, c)] = b;
+    wt[frag_elem(kFragT3, 2, c, r)] = b;
   } else if (p >= kOffW4 && p < kOffB4) {
     const int q = p - kOffW4, r = q / kHid, c = q % kHid;
-    wf[frag_elem(kFragW4, 4, r, c)] = b;
-    wt[frag_elem(kFragT4, 3, c, r)] = b;
+    wf[frag_elem(kFragW4, 2, r, c)] = b;
+    wt[frag_elem(kFragT4, 2, c, r)] = b;
   }
 }
 
@@ -378,36 +429,50 @@ __global__ void k_pack(const float* theta, uint16_t* wf, uint16_t* wt) {
   if (p < kNumParams) store_frags(p, theta[p], wf, wt);
 }
 
-// Sum of the per-block partial gradients in block order, times inv_n.  The loads
-// are issued 16 at a time ahead of the (sequential, block-ordered) additions, so
-// a thread waits for memory once per 16 blocks instead of once per block.
-__device__ __forceinline__ float reduce_grad(const float* part, int nparts, int p, float inv_n) {
-  constexpr int kBatch = 16;
+// Sum of the per-block partial gradients, times inv_n, by kRedLanes lanes per
+// parameter: lane j adds the partials of blocks j, j + kRedLanes, ... in block
+// order (loads issued 8 at a time ahead of the additions), then an xor
+// butterfly over the lanes gives every lane the same total.  The order is
+// fixed, so a gradient's bits do not depend on the run.  Every lane of a
+// kRedLanes group must call it (p >= kNumParams reads nothing).
+constexpr int kRedLanes = 8;
+__device__ __forceinline__ float reduce_grad(const float* part, int nparts, int p, int j,
+                                             float inv_n) {
+  constexpr int kBatch = 8;
   float gsum = 0.f;
-  int b = 0;
-  for (; b + kBatch <= nparts; b += kBatch) {
-    float v[kBatch];
+  if (p < kNumParams) {
+    int b = j;
+    for (; b + (kBatch - 1) * kRedLanes < nparts; b += kBatch * kRedLanes) {
+      float v[kBatch];
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) v[j] = part[(size_t)(b + j) * kNumParams + p];
+      for (int i = 0; i < kBatch; ++i) v[i] = part[(size_t)(b + i * kRedLanes) * kNumParams + p];
 #pragma unroll
-    for (int j = 0; j < kBatch; ++j) gsum += v[j];
+      for (int i = 0; i < kBatch; ++i) gsum += v[i];
+    }
+    for (; b < nparts; b += kRedLanes) gsum += part[(size_t)b * kNumParams + p];
   }
-  for (; b < nparts; ++b) gsum += part[(size_t)b * kNumParams + p];
+  static_assert(kRedLanes == 8, "three butterfly steps");
+  gsum += __shfl_xor(gsum, 1);
+  gsum += __shfl_xor(gsum, 2);
+  gsum += __shfl_xor(gsum, 4);
   return gsum * inv_n;
 }
 
 __global__ void k_grad_reduce(const float* part, int nparts, float inv_n, float* out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < kNumParams) out[p] = reduce_grad(part, nparts, p, inv_n);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = t / kRedLanes, j = t % kRedLanes;
+  const float gr = reduce_grad(part, nparts, p, j, inv_n);
+  if (j == 0 && p < kNumParams) out[p] = gr;
 }
 
 // Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the reduced minibatch gradient, then
-// the BF16 fragments of the updated parameter.
+// the BF16 fragments of the updated parameter (one parameter per kRedLanes lanes).
 __global__ void k_adam(float* theta, float* m, float* v, uint16_t* wf, uint16_t* wt,
                        const float* part, int nparts, float inv_n, float lr, float bc1, float bc2) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNumParams) return;
-  const float gr = reduce_grad(part, nparts, p, inv_n);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = t / kRedLanes, j = t % kRedLanes;
+  const float gr = reduce_grad(part, nparts, p, j, inv_n);
+  if (j != 0 || p >= kNumParams) return;
   const float mm = 0.9f * m[p] + 0.1f * gr;
   const float vv = 0.999f * v[p] + 0.001f * gr * gr;
   m[p] = mm;
@@ -444,7 +509,7 @@ void launch_adam(mupsi_guide* g, int batch, hipStream_t st) {
   g->step += 1;
   const float bc1 = (float)(1.0 - std::pow(0.9, (double)g->step));
   const float bc2 = (float)(1.0 - std::pow(0.999, (double)g->step));
-  hipLaunchKernelGGL(k_adam, dim3(grid_for(kNumParams)), dim3(256), 0, st, g->theta, g->adam_m,
+  hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->theta, g->adam_m,
                      g->adam_v, reinterpret_cast<uint16_t*>(g->wf), reinterpret_cast<uint16_t*>(g->wt),
                      g->grad_part, train_blocks(batch), 1.0f / (float)batch, g->cfg.learning_rate,
                      bc1, bc2);
@@ -479,8 +544,9 @@ namespace guide {
 
 void launch_infer(const mupsi_guide* g, const float* feats, const int* count_dev, int count_host,
                   const int* q_pid, float* raw_out, float* mix_out, hipStream_t st) {
-  const int cap_blocks = count_dev ? g->ctx->cus * 8 : std::max(1, (count_host + 63) / 64);
-  const int blocks = std::max(1, std::min(cap_blocks, g->ctx->cus * 8));
+  // blocks that are resident at once (each keeps the network in 40 KiB of LDS)
+  const int cap_blocks = count_dev ? g->ctx->cus * 4 : std::max(1, (count_host + 63) / 64);
+  const int blocks = std::max(1, std::min(cap_blocks, g->ctx->cus * 4));
   InferArgs ia{feats, count_dev, count_host, q_pid, raw_out, mix_out, g->cfg.alpha};
   hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, g->net(), ia);
 }
@@ -538,8 +604,8 @@ extern "C" mupsi_status mupsi_guide_create(mupsi_ctx* ctx, const mupsi_guide_con
   HIP_TRY(g->mem.alloc(kNumParams, &g->grad_out));
   HIP_TRY(g->mem.alloc((size_t)kNumFragF * 64, &g->wf));
   HIP_TRY(g->mem.alloc((size_t)kNumFragT * 64, &g->wt));
-  HIP_TRY(hipMemset(g->wf, 0, (size_t)kNumFragF * 64 * sizeof(bf16x4)));
-  HIP_TRY(hipMemset(g->wt, 0, (size_t)kNumFragT * 64 * sizeof(bf16x4)));
+  HIP_TRY(hipMemset(g->wf, 0, (size_t)kNumFragF * 64 * sizeof(bf16x8)));
+  HIP_TRY(hipMemset(g->wt, 0, (size_t)kNumFragT * 64 * sizeof(bf16x8)));
   g->part_blocks = kMaxTrainBlocks;
   HIP_TRY(g->mem.alloc((size_t)kMaxTrainBlocks * kNumParams, &g->grad_part));
   HIP_TRY(g->mem.alloc(kMaxTrainBlocks, &g->loss_part));
@@ -610,7 +676,7 @@ extern "C" mupsi_status mupsi_guide_gradient(mupsi_guide* g, const float* feats,
   const int batch = (int)((n + 63) / 64 * 64);
   hipStream_t st = g->ctx->stream;
   launch_train(g, feats, dirs, weights, nullptr, (int)n, batch, 0, 0u, st);
-  hipLaunchKernelGGL(k_grad_reduce, dim3(grid_for(kNumParams)), dim3(256), 0, st, g->grad_part,
+  hipLaunchKernelGGL(k_grad_reduce, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->grad_part,
                      train_blocks(batch), 1.0f / (float)n, g->grad_out);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -638,7 +704,7 @@ extern "C" mupsi_status mupsi_guide_train_step(mupsi_guide* g, const float* feat
   g->step += 1;
   const float bc1 = (float)(1.0 - std::pow(0.9, (double)g->step));
   const float bc2 = (float)(1.0 - std::pow(0.999, (double)g->step));
-  hipLaunchKernelGGL(k_adam, dim3(grid_for(kNumParams)), dim3(256), 0, st, g->theta, g->adam_m,
+  hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->theta, g->adam_m,
                      g->adam_v, reinterpret_cast<uint16_t*>(g->wf), reinterpret_cast<uint16_t*>(g->wt),
                      g->grad_part, train_blocks(batch), 1.0f / (float)n, g->cfg.learning_rate, bc1, bc2);
   HIP_TRY(hipGetLastError());

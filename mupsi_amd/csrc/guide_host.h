@@ -11,8 +11,8 @@ struct mupsi_guide {
   float* theta = nullptr;  // FP32 master parameters
   float* adam_m = nullptr;
   float* adam_v = nullptr;
-  mupsi::guide::bf16x4* wf = nullptr;  // forward fragments
-  mupsi::guide::bf16x4* wt = nullptr;  // transposed fragments
+  mupsi::guide::bf16x8* wf = nullptr;  // forward fragments
+  mupsi::guide::bf16x8* wt = nullptr;  // transposed fragments
   float* grad_part = nullptr;          // per-block partial gradients
   float* loss_part = nullptr;
   float* grad_out = nullptr;           // reduced gradient (mupsi_guide_gradient)
