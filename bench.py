@@ -18,11 +18,10 @@ Reported next to it:
   guiding       the learned-guiding workload (the same sphere scene with a
                 mirror sphere and an emissive quad, 1024x1024): wall time for
                 the plain and the guided tracer to reach 1e-3 relative MSE
-                (mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)).
-                Both tracers are unbiased, so relMSE is the per-pixel variance of
-                the progressive average over R^2 + 0.01, measured from independent
-                renders against a 1024-pass plain R; the guided time includes its
-                16 online-training passes.
+                (mean over pixels and channels of (I - R)^2 / (R^2 + 0.01))
+                against a 6144-pass plain R, measured on progressive renders
+                (the guided time includes its 32 online-training passes), with
+                the figure extrapolated from per-pass variances beside it.
 """
 from __future__ import annotations
 
@@ -91,29 +90,55 @@ def _chunk_stats(s, k0: int, chunks: int, spp: int, R):
     return v, secs / (chunks * spp)
 
 
-def _passes_to_target(v_t: float, n_t: int, v_g: float, target: float) -> int:
-    """Smallest N >= n_t with (n_t v_t + (N - n_t) v_g) / N^2 <= target: the relMSE
-    of the progressive average whose first n_t passes have per-pass variance v_t."""
-    n = n_t
-    while (n_t * v_t + (n - n_t) * v_g) / (n * n) > target:
-        n += 1
-    return n
+def _relmse(img, R) -> float:
+    import numpy as np
+    return float(np.mean((img - R) ** 2 / (R ** 2 + 0.01)))
 
 
-def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs: int = 4,
-                  ref_passes: int = 1024) -> dict:
+def _crossing(s, R, k0: int, target: float, first: int, step: int, limit: int,
+              r_err: float) -> dict:
+    """Progressive render from an empty image until its relMSE against R, less
+    R's own estimated relMSE r_err (the two are independent), falls to target:
+    passes are added `first` then `step` at a time and the image is read back
+    between calls (outside the timed render calls).  Returns the pass count
+    and the summed wall time of the render calls at the first check at or
+    below target, the raw relMSE there, and the history of checks."""
+    import torch
+    s.reset()
+    n, secs, hist = 0, 0.0, []
+    while n < limit:
+        k = first if n == 0 else step
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.render_passes(k0 + n, k)
+        torch.cuda.synchronize()
+        secs += time.perf_counter() - t0
+        n += k
+        e = _relmse(s.image(), R)
+        hist.append((n, round(secs, 4), e))
+        if e - r_err <= target:
+            return {"passes": n, "time_s": secs, "relmse_raw": e, "relmse_less_reference": e - r_err,
+                    "crossed": True, "checks": len(hist)}
+    return {"passes": n, "time_s": secs, "relmse_raw": hist[-1][2], "crossed": False,
+            "checks": len(hist)}
+
+
+def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8,
+                  ref_passes: int = 6144) -> dict:
     """Time to `target` relMSE, plain vs guided, on the learned-guiding workload.
 
-    relMSE(I) = mean over pixels and channels of (I - R)^2 / (R^2 + 0.01) with R
-    the converged image; both tracers are unbiased, so the relMSE of an N-pass
-    progressive image is v / N with v the per-pass variance over R^2 + 0.01.  R
-    here is a ref_passes-pass plain render on its own pass range (its noise only
-    enters the denominators).  Plain: v_p from independent chunks -> v_p / target
-    passes.  Guided (progressive, as render_passes accumulates it): `runs`
-    independent online trainings give the training-phase variance (network
-    randomness included) and the training time; the trained network's v_g comes
-    from chunks; the time to target counts every pass of the progressive image.
-    The figure that discards the training passes' samples is reported too."""
+    relMSE(I) = mean over pixels and channels of (I - R)^2 / (R^2 + 0.01).  R is
+    a ref_passes-pass plain render on its own pass range (its relMSE, v_p /
+    ref_passes, is about 1e-4 and is subtracted from the measured figures).
+    Measured: the plain tracer and the guided tracers (the network trained
+    online on its first 32 passes, and on its first 16; and the network with
+    the reference's Russian roulette, rr_boost 1) render progressively until
+    their image crosses target; the wall time of the render calls up to that
+    pass is the time to target (training passes and Adam steps included).
+    Extrapolated beside it: both tracers are unbiased, so an N-pass image has
+    relMSE v / N with v the per-pass variance over R^2 + 0.01, estimated from
+    `chunks` independent renders of `spp` passes; time = seconds per pass x
+    v / target."""
     import numpy as np
     import torch
     from mupsi_amd import Guide, Scene, scenes
@@ -128,84 +153,79 @@ def guiding_bench(ctx, target: float = 1e-3, chunks: int = 8, spp: int = 8, runs
     torch.cuda.synchronize()
     ref_s = time.perf_counter() - t0
     R = plain.image()
+    print(f"bench: guiding reference {ref_passes} passes in {ref_s:.1f} s", file=sys.stderr, flush=True)
     v_p, t_p = _chunk_stats(plain, 1000, chunks, spp, R)
+    r_err = v_p / ref_passes
+    plain_x = _crossing(plain, R, 0, target, 64, 16, 4096, r_err)
+    print(f"bench: guiding plain crossing {plain_x}", file=sys.stderr, flush=True)
     plain.close()
     guided = Scene(ctx, sc)
-    warm = Guide.for_scene(ctx, sc)
-    guided.set_guide(warm)
-    # allocate the path, guide and training-ring buffers at their final sizes
-    guided.render_passes(0, warm.config["train_passes"] + spp)
-    warm.close()
-    n_t = 0
-    imgs, t_trains = [], []
-    g = None
-    for r in range(runs):
-        if g is not None:
-            g.close()
-        g = Guide.for_scene(ctx, sc, seed=1 + r)
+    out_runs = {}
+    for name, cfg in (("guided_train32", dict(train_passes=32)),
+                      ("guided_train16", dict(train_passes=16, passes_per_update=2)),
+                      ("network_only_reference_roulette", dict(train_passes=32, rr_boost=1.0))):
+        g = Guide.for_scene(ctx, sc, seed=7, **cfg)
         n_t = g.config["train_passes"]
-        guided.set_guide(g)                           # fresh network and training history
-        guided.reset()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        guided.render_passes(2000 + 100 * r, n_t)
-        torch.cuda.synchronize()
-        t_trains.append(time.perf_counter() - t0)
-        imgs.append(guided.image())
-    imgs = np.array(imgs)
-    v_t = float(np.mean(imgs.var(0, ddof=1) * n_t / (R ** 2 + 0.01)))
-    t_train = float(np.median(t_trains))
-    v_g, t_g = _chunk_stats(guided, 3000, chunks, spp, R)
-    st = g.stats()
-    # the same guided tracer with its roulette but without the network (alpha = 0)
+        if not out_runs:   # allocate the path, guide and ring buffers outside the timing
+            warm = Guide.for_scene(ctx, sc, seed=99, **cfg)
+            guided.set_guide(warm)
+            guided.render_passes(50000, n_t + 4)
+            warm.close()
+        guided.set_guide(g)                            # fresh network and training history
+        x = _crossing(guided, R, 2000 + 1000 * len(out_runs), target, n_t + 8, 4, 4096, r_err)
+        st = g.stats()
+        x.update({"train_passes": n_t, "rr_boost": g.config["rr_boost"],
+                  "train_steps": st["train_steps"], "train_ms": st["train_ms"],
+                  "fraction_of_plain_time": x["time_s"] / plain_x["time_s"]})
+        if name == "guided_train32":   # the trained network's per-pass variance and speed
+            v_g, t_g = _chunk_stats(guided, 3000, chunks, spp, R)
+            x["v_per_pass_trained"] = v_g
+            x["s_per_pass_trained"] = t_g
+            mlp = {"steps": st["train_steps"], "ms": st["train_ms"]}
+        out_runs[name] = x
+        print(f"bench: guiding {name} {x}", file=sys.stderr, flush=True)
+        g.close()
+    # extrapolated (per-pass variance x seconds per pass), as a cross-check
+    gx = out_runs["guided_train32"]
+    time_plain_x = t_p * v_p / target
+    n_t = gx["train_passes"]
+    train_s = gx["time_s"] - (gx["passes"] - n_t) * gx["s_per_pass_trained"]
+    time_guided_x = train_s + gx["s_per_pass_trained"] * gx["v_per_pass_trained"] / target
+    # the guided tracer's roulette without the network (alpha = 0)
     ctl = Guide.for_scene(ctx, sc, alpha=0.0, train_passes=0)
     guided.set_guide(ctl)
     v_c, _ = _chunk_stats(guided, 4000, chunks // 2, spp, R)
-    # the network with the reference's roulette
-    g1 = Guide.for_scene(ctx, sc, rr_boost=1.0)
-    guided.set_guide(g1)
-    guided.render_passes(5000, g1.config["train_passes"])
-    v_1, _ = _chunk_stats(guided, 6000, chunks // 2, spp, R)
+    ctl.close()
     guided.close()
-    time_plain = t_p * v_p / target
-    n_star = _passes_to_target(v_t, n_t, v_g, target)
-    time_guided = t_train + (n_star - n_t) * t_g
-    time_guided_discard = t_train + t_g * v_g / target
     paths = W * H
     # algorithmic GEMM flops per training sample: forward 2(64*32 + 2*64*64 + 41*64),
     # input gradients of layers 4..2 2(41*64 + 2*64*64), weight gradients as forward
-    mflop_step = 73088 * g.config["batch_size"] / 1e6
-    out = {
+    mflop_step = 73088 * Guide.DEFAULTS["batch_size"] / 1e6
+    return {
         "workload": f"sphere-mean GPIS renewal_plus + mirror sphere r0.5 + emissive quad "
-                    f"(area 0.01, radiance 200), {W}x{H}, guide trained online on the first "
-                    f"{n_t} passes",
+                    f"(area 0.01, radiance 200), {W}x{H}",
         "target_relmse": target,
-        "reference": {"passes": ref_passes, "s": ref_s,
-                      "relmse_of_reference": v_p / ref_passes},
-        "plain": {"v_per_pass": v_p, "s_per_pass": t_p, "Mpaths_s": paths / t_p / 1e6,
-                  "passes_to_target": v_p / target, "time_to_target_s": time_plain},
-        "guided": {"v_per_pass_trained": v_g, "v_per_pass_training_phase": v_t,
-                   "s_per_pass": t_g, "Mpaths_s": paths / t_g / 1e6,
-                   "train_s": t_train, "train_s_runs": t_trains,
-                   "passes_to_target": n_star, "time_to_target_s": time_guided,
-                   "time_to_target_discarding_training_passes_s": time_guided_discard,
-                   "train_steps": st["train_steps"], "train_records": st["train_samples"],
-                   "rr_boost": g.config["rr_boost"]},
-        "guided_time_fraction": time_guided / time_plain,
-        "guided_time_fraction_discarding_training_passes": time_guided_discard / time_plain,
-        "decomposition_v_per_pass": {
-            "plain": v_p, "network_with_reference_roulette": v_1,
-            "roulette_only_alpha0": v_c, "guided": v_g},
-        "mlp_train": {"steps": st["train_steps"], "ms": st["train_ms"],
-                      "TFLOP_s": (st["train_steps"] * mflop_step / 1e6) / (st["train_ms"] * 1e-3)
-                      if st["train_ms"] > 0 else 0.0,
+        "reference": {"passes": ref_passes, "s": ref_s, "relmse_of_reference": r_err},
+        "measured": {"plain": plain_x, **out_runs},
+        "guided_time_fraction": gx["fraction_of_plain_time"],
+        "network_only_time_fraction": out_runs["network_only_reference_roulette"]["fraction_of_plain_time"],
+        "guided_train16_time_fraction": out_runs["guided_train16"]["fraction_of_plain_time"],
+        "extrapolated": {
+            "plain": {"v_per_pass": v_p, "s_per_pass": t_p, "Mpaths_s": paths / t_p / 1e6,
+                      "time_to_target_s": time_plain_x},
+            "guided_train32": {"v_per_pass": gx["v_per_pass_trained"],
+                               "s_per_pass": gx["s_per_pass_trained"],
+                               "Mpaths_s": paths / gx["s_per_pass_trained"] / 1e6,
+                               "time_to_target_s": time_guided_x},
+            "guided_time_fraction": time_guided_x / time_plain_x,
+            "estimates": f"{chunks} x {spp} passes per chunk"},
+        "decomposition_v_per_pass": {"plain": v_p, "roulette_only_alpha0": v_c,
+                                     "guided": gx["v_per_pass_trained"]},
+        "mlp_train": {"steps": mlp["steps"], "ms": mlp["ms"],
+                      "TFLOP_s": (mlp["steps"] * mflop_step / 1e6) / (mlp["ms"] * 1e-3)
+                      if mlp["ms"] > 0 else 0.0,
                       "peak_bf16_TFLOP_s": 2500.0},
-        "estimates": f"{chunks} x {spp} passes per chunk estimate, {runs} independent trainings",
     }
-    g.close()
-    g1.close()
-    ctl.close()
-    return out
 
 
 HBM_PEAK_GBS = 8000.0

@@ -405,7 +405,7 @@ typedef struct {
   uint64_t seed;           /* parameter initialisation and minibatch draws */
   float learning_rate;     /* Adam step size (1e-3) */
   float alpha;             /* largest guide selection probability (0.9; 0 = never) */
-  int32_t train_passes;    /* passes of online training (16) */
+  int32_t train_passes;    /* passes of online training (32) */
   int32_t batch_size;      /* training minibatch, multiple of 64 (16384) */
   int32_t min_steps_per_pass; /* Adam steps per training pass: about four epochs of */
   int32_t max_steps_per_pass; /*   the pass's new records, within [min, max] */
@@ -414,7 +414,7 @@ typedef struct {
   float box_radius;        /*   p_hat = clamp((p - center) / radius, -1, 1) */
   float indirect_clamp;    /* training radiance: direct + min(indirect, clamp) (luminance) */
   float rr_boost;          /* guided roulette survives with min(1, rr_boost max(T)) (>= 1) */
-  int32_t passes_per_update; /* training passes traced per launch before the Adam steps (2) */
+  int32_t passes_per_update; /* training passes traced per launch before the Adam steps (4) */
 } mupsi_guide_config;
 
 typedef struct {

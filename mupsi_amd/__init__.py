@@ -204,10 +204,10 @@ class Guide:
     directions at GP-medium hits, trained online during the first train_passes
     passes of a render it is attached to (Scene.set_guide)."""
 
-    DEFAULTS = dict(seed=1, learning_rate=1e-3, alpha=0.9, train_passes=16, batch_size=16384,
+    DEFAULTS = dict(seed=1, learning_rate=1e-3, alpha=0.9, train_passes=32, batch_size=16384,
                     min_steps_per_pass=4, max_steps_per_pass=128, max_records=3,
                     box_center=(0.0, 0.0, 0.0), box_radius=1.0, indirect_clamp=1.0,
-                    rr_boost=4.0, passes_per_update=2)
+                    rr_boost=4.0, passes_per_update=4)
 
     def __init__(self, ctx: Context, **cfg):
         self.ctx = ctx
