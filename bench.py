@@ -375,6 +375,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the bounce loop's launches one by one instead of replaying a graph")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain measurement")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the sphere_single_128, icosphere_renewal_512, generated_batch and BVH traversal measurements")
@@ -410,6 +412,7 @@ def main() -> int:
     W, H, spp = scene["width"], scene["height"], scene["spp"]
     ctx = Context(0)
     s = Scene(ctx, scene)
+    s.set_graph(not args.no_graph)  # the bounce loop replays a captured hipGraph
     print(f"bench: {ctx.info()} scene {W}x{H} spp {spp}", file=sys.stderr, flush=True)
 
     if args.copy_probe_mb:
@@ -425,8 +428,6 @@ def main() -> int:
         s.render_passes(0, spp)
         print(f"bench: warmup {i} done", file=sys.stderr, flush=True)
 
-    march_ms = 0.0
-    launches = 0
     evals = 0
     steps_ref = 0
     torch.cuda.synchronize()
@@ -435,18 +436,28 @@ def main() -> int:
         s.reset()
         s.render_passes(0, spp)
         st = s.stats()
-        march_ms += st["march_kernel_ms"]
-        launches += st["march_kernel_launches"]
         evals += st["field_evals"]
         steps_ref += st["march_steps"]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     secs = t1 - t0
+    # the march kernel's launches timed one by one (HIP events on the library
+    # stream) on the same work, issued launch by launch: a replayed graph runs
+    # the same kernels but is not timed per launch
+    s.set_graph(False)
+    march_ms, launches, evals_t = 0.0, 0, 0
+    for i in range(min(args.steps, 3)):
+        s.reset()
+        s.render_passes(0, spp)
+        st = s.stats()
+        march_ms += st["march_kernel_ms"]
+        launches += st["march_kernel_launches"]
+        evals_t += st["field_evals"]
     paths = W * H * spp * args.steps
     ms_per_step = 1e3 * secs / args.steps
     ppc = scene["kernel"]["points_per_cell"]
     flops_per_eval = 27 * ppc * 16 + 9
-    achieved = evals * flops_per_eval / (march_ms * 1e-3) / 1e12 if march_ms > 0 else 0.0
+    achieved = evals_t * flops_per_eval / (march_ms * 1e-3) / 1e12 if march_ms > 0 else 0.0
     img = s.image()
     print(f"bench: {args.steps} steps in {secs:.3f} s; march kernel {march_ms:.1f} ms over "
           f"{launches} launches; {evals} field evals (reference grid points {steps_ref}); "
@@ -480,7 +491,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "width": W, "height": H, "spp": spp,
-                   "paths_per_step": W * H * spp, "tracer": "plain wavefront"},
+                   "paths_per_step": W * H * spp, "tracer": "plain wavefront",
+                   "bounce_loop": "launches issued one by one" if args.no_graph
+                   else "hipGraph replay"},
         "roofline": {
             "bound": "valu",
             "kernel": f"k_march_window<{ppc}, false, false, 16>",
@@ -489,7 +502,7 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": MARCH_TRAFFIC_BYTES,
-            "flops_per_launch": evals * flops_per_eval / max(1, launches),
+            "flops_per_launch": evals_t * flops_per_eval / max(1, launches),
             "avg_launch_ms": march_ms / max(1, launches),
         },
         "cpu_baseline": cpu,

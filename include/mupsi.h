@@ -214,6 +214,15 @@ mupsi_status mupsi_scene_set_settings(mupsi_scene* scene, int32_t max_bounce, in
  * passes_per_launch bounds how many passes run as one wavefront (0 = auto). */
 mupsi_status mupsi_render_passes(mupsi_scene* scene, int32_t k_begin, int32_t k_count,
                                  int32_t passes_per_launch, const volatile int32_t* abort_flag);
+/* No reference counterpart (how the pass loop is launched): with use_graph != 0,
+ * a plain render (no guide) called without an abort flag captures the bounce
+ * loop of a launch (init, marches, shades) into a hipGraph once and replays it
+ * for every later launch of the same size, the first pass read from device
+ * memory; the results are those of the launches issued one by one.  A capture
+ * is dropped when the path buffers grow or the scene's settings or GP medium
+ * change.  mupsi_scene_graph_builds counts the captures made. */
+mupsi_status mupsi_scene_set_graph(mupsi_scene* scene, int32_t use_graph);
+mupsi_status mupsi_scene_graph_builds(mupsi_scene* scene, int64_t* builds);
 mupsi_status mupsi_reset_accum(mupsi_scene* scene);
 /* rgb_sum: host, width*height*3 doubles (row-major, y = row); sample_count: passes accumulated. */
 mupsi_status mupsi_read_accum(mupsi_scene* scene, double* rgb_sum, int64_t* sample_count);

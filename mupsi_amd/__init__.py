@@ -107,6 +107,17 @@ class Scene:
     def reset(self) -> None:
         check(self.lib.mupsi_reset_accum(self.handle), "mupsi_reset_accum")
 
+    def set_graph(self, use_graph: bool = True) -> None:
+        """Replay a captured bounce loop for plain renders without an abort flag
+        (mupsi_scene_set_graph)."""
+        check(self.lib.mupsi_scene_set_graph(self.handle, 1 if use_graph else 0),
+              "mupsi_scene_set_graph")
+
+    def graph_builds(self) -> int:
+        n = C.c_int64()
+        check(self.lib.mupsi_scene_graph_builds(self.handle, C.byref(n)), "mupsi_scene_graph_builds")
+        return int(n.value)
+
     def set_gp_medium(self, mean_type: str = "sphere", mean_center=(0.0, 0.0, 0.0),
                       mean_radius: float = 70.0, mesh=None, transform: dict | None = None,
                       kernel: dict | None = None) -> None:

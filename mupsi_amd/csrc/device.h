@@ -780,9 +780,10 @@ __device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {
 // mu and grad mu (meanfunction.cpp:13-39).  For a mesh mean the reference calls
 // inside() and distToPoint() once for the value and again for the gradient;
 // both calls return the same bits, so one pair serves both here.
-template <bool kGrad>
+// kMesh false: the mean is known to be a sphere (no BVH code is compiled).
+template <bool kGrad, bool kMesh = true>
 __device__ __forceinline__ float mean_eval(const DevScene& S, F3 p, F3* grad) {
-  if (S.mean_type == 0) {
+  if (!kMesh || S.mean_type == 0) {
     const F3 r = sub(p, mk(S.mean_c[0], S.mean_c[1], S.mean_c[2]));
     if (kGrad) *grad = normalized(r);
     return norm(r) - S.mean_r;
@@ -882,7 +883,7 @@ __device__ __forceinline__ float grid_medium_eval(const DevScene& S, F3 p, uint3
 
 // GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).  mean: the
 // mean's value and gradient at p when the caller has them already.
-template <bool kGrad>
+template <bool kGrad, bool kMesh = true>
 __device__ __forceinline__ float field_eval(const DevScene& S, F3 p, uint32_t seed, const Cond& c,
                                             F3* grad, const MeanAt* mean = nullptr) {
   F3 mg = mk(0, 0, 0), pg = mk(0, 0, 0);
@@ -891,7 +892,7 @@ __device__ __forceinline__ float field_eval(const DevScene& S, F3 p, uint32_t se
     mu = mean->mu;
     mg = mean->grad;
   } else {
-    mu = mean_eval<kGrad>(S, p, &mg);
+    mu = mean_eval<kGrad, kMesh>(S, p, &mg);
   }
   float pv;
   psi_eval<kGrad>(S.gp, p, seed, pv, pg, nullptr);

@@ -87,6 +87,7 @@ struct LaunchPlan {
 // batched launch's plan (with the path's scene id).
 struct PathSrc {
   int W, npix, k_begin;
+  const int* k_begin_dev;   // graph replays: the first pass, read from device memory
   const int32_t *xs, *ys, *ks;
   const LaunchPlan* plan;   // batched launch, or null
   int nplan;
@@ -111,7 +112,7 @@ __device__ __forceinline__ void path_xyk(const PathSrc& s, int pid, int& x, int&
     const int pix = pid % s.npix;
     x = pix % s.W;
     y = pix / s.W;
-    k = s.k_begin + pid / s.npix;
+    k = (s.k_begin_dev ? *s.k_begin_dev : s.k_begin) + pid / s.npix;
   }
 }
 
@@ -317,6 +318,8 @@ __device__ __forceinline__ void sphere_local(const DevPrim& pr, const Ray& r, F3
 }
 
 // Scene::intersect (scene.cpp:25-39) + intersectInfo of the closest primitive.
+// kTri false: the scene is known to hold spheres only (no BVH code is compiled).
+template <bool kTri = true>
 __device__ __forceinline__ int scene_intersect(const DevScene& S, Ray& ray, F3& P, F3& Ng, float& uu, float& vv) {
   float mint = 3.402823466e+38f;
   int hitp = -1;
@@ -327,7 +330,7 @@ __device__ __forceinline__ int scene_intersect(const DevScene& S, Ray& ray, F3& 
     bool h = false;
     uint32_t ti = 0;
     float u = 0, v = 0;
-    if (pr.type == MUPSI_PRIM_SPHERE) {
+    if (!kTri || pr.type == MUPSI_PRIM_SPHERE) {
       F3 O, D;
       sphere_local(pr, ray, O, D);
       const float t = unit_sphere_t(ray.nearT, ray.farT, O, D);
@@ -349,7 +352,7 @@ __device__ __forceinline__ int scene_intersect(const DevScene& S, Ray& ray, F3& 
   if (hitp < 0) return -1;
   P = madd(ray.d, ray.farT, ray.o);
   const DevPrim& pr = S.prims[hitp];
-  if (pr.type == MUPSI_PRIM_SPHERE) {
+  if (!kTri || pr.type == MUPSI_PRIM_SPHERE) {
     const F3 pl = pr.xf.identity ? P : xf_to_local(pr.xf, P);
     const F3 ln = normalized(pl);
     Ng = pr.xf.identity ? ln : normalized(mul(ln, pr.xf.inv_s));
@@ -386,10 +389,11 @@ __device__ __forceinline__ int scene_intersect(const DevScene& S, Ray& ray, F3& 
 }
 
 // Scene::occluded (scene.cpp:41-48)
+template <bool kTri = true>
 __device__ __forceinline__ bool scene_occluded(const DevScene& S, const Ray& r) {
   for (int i = 0; i < S.n_prims; ++i) {
     const DevPrim& pr = S.prims[i];
-    if (pr.type == MUPSI_PRIM_SPHERE) {
+    if (!kTri || pr.type == MUPSI_PRIM_SPHERE) {
       F3 O, D;
       sphere_local(pr, r, O, D);
       if (unit_sphere_t(r.nearT, r.farT, O, D) >= 0.0f) return true;
@@ -472,11 +476,12 @@ __device__ bool mesh_sample_direct(const DevPrim& pr, const DevMesh& m, F3 p, Pc
 }
 
 // Scene::chooseLight (scene.cpp:50-61): no draw when the scene has no emitter.
+template <bool kTri = true>
 __device__ __forceinline__ bool choose_light(const DevScene& S, F3 p, Pcg& rng, LightSample& ls) {
   if (S.n_prims == 0 || S.n_lights == 0) return false;
   const int idx = (int)(rng.nextI() % (uint32_t)S.n_lights);
   const DevPrim& pr = S.prims[S.lights[idx]];
-  if (pr.type == MUPSI_PRIM_SPHERE) return sphere_sample_direct(pr, p, rng, ls);
+  if (!kTri || pr.type == MUPSI_PRIM_SPHERE) return sphere_sample_direct(pr, p, rng, ls);
   return mesh_sample_direct(pr, S.meshes[pr.mesh], p, rng, ls);
 }
 
@@ -639,20 +644,21 @@ __device__ __forceinline__ float std_max(float a, float b) { return a < b ? b : 
 // change (f_prev * f < 0) is refined by 16 bisection steps and the hit is hi.
 // iters: steps taken.  Transmittance takes the same decision without the
 // bisection.
+template <bool kMesh = true>
 __device__ __forceinline__ bool sdf_march(const DevScene& S, F3 o, F3 d, float nt, float ft, float& t_hit,
                           F3& p_hit, int& iters) {
   float t = nt;
-  float fc = mean_eval<false>(S, o, nullptr);
+  float fc = mean_eval<false, kMesh>(S, o, nullptr);
   int iter = 0;
   while (t < ft && iter < 512) {
     const F3 p = madd(d, t, o);
     const float fp = fc, tp = t;
-    fc = mean_eval<false>(S, p, nullptr);
+    fc = mean_eval<false, kMesh>(S, p, nullptr);
     if (fp * fc < 0.0f) {
       float lo = tp, hi = t, flo = fp;
       for (int b = 0; b < 16; ++b) {
         const float mid = (lo + hi) * 0.5f;
-        const float fm = mean_eval<false>(S, madd(d, mid, o), nullptr);
+        const float fm = mean_eval<false, kMesh>(S, madd(d, mid, o), nullptr);
         if (flo * fm <= 0.0f) {
           hi = mid;
         } else {
@@ -709,36 +715,40 @@ __device__ __forceinline__ bool grid_march(const DevScene& S, F3 o, F3 d, float 
 // The march of a medium other than the GP medium (hit flag, t, point).  Every
 // function taking the scene by reference is inlined: an out-of-line call needs
 // the scene's address, and the kernel would copy its scene argument to scratch.
+// kMesh false: the mean is known to be a sphere.
+template <bool kMesh = true>
 __device__ __forceinline__ bool alt_march(const DevScene& S, F3 o, F3 d, float nt, float ft,
                                        uint32_t seed, float& t_hit, F3& p_hit, int& iters) {
-  if (S.medium_kind == MUPSI_MEDIUM_SDF) return sdf_march(S, o, d, nt, ft, t_hit, p_hit, iters);
+  if (S.medium_kind == MUPSI_MEDIUM_SDF) return sdf_march<kMesh>(S, o, d, nt, ft, t_hit, p_hit, iters);
   return grid_march(S, o, d, nt, ft, seed, t_hit, p_hit, iters);
 }
 // Gradient of a medium other than the GP medium at p.
+template <bool kMesh = true>
 __device__ __forceinline__ F3 alt_grad(const DevScene& S, F3 p, uint32_t seed) {
   F3 g = mk(0, 0, 0);
-  if (S.medium_kind == MUPSI_MEDIUM_SDF) (void)mean_eval<true>(S, p, &g);
+  if (S.medium_kind == MUPSI_MEDIUM_SDF) (void)mean_eval<true, kMesh>(S, p, &g);
   else (void)grid_medium_eval<true>(S, p, seed, &g);
   return g;
 }
 
 // Serial march of one ray (used for NEE shadow rays in k_shade): the same
 // decisions as k_march without the certain-sign skip.
+template <bool kMesh = true>
 __device__ __forceinline__ bool march_serial(const DevScene& S, const Ray& ray, uint32_t seed,
                                              const Cond& c) {
   if (S.medium_kind != MUPSI_MEDIUM_GP) {
     float th;
     F3 ph;
     int it;
-    return alt_march(S, ray.o, ray.d, ray.nearT, fminf(ray.farT, 2000.0f), seed, th, ph, it);
+    return alt_march<kMesh>(S, ray.o, ray.d, ray.nearT, fminf(ray.farT, 2000.0f), seed, th, ph, it);
   }
   const float nt = ray.nearT, ft = fminf(ray.farT, 2000.0f);
   const float dt = fminf(0.5f, (ft - nt) / 100);
   float t = nt;
-  const float fprev = field_eval<false>(S, ray.o, seed, c, nullptr);
+  const float fprev = field_eval<false, kMesh>(S, ray.o, seed, c, nullptr);
   const bool sp = fprev > 0.0f;
   while (t < ft) {
-    const float fc = field_eval<false>(S, madd(ray.d, t, ray.o), seed, c, nullptr);
+    const float fc = field_eval<false, kMesh>(S, madd(ray.d, t, ray.o), seed, c, nullptr);
     if ((fc > 0.0f) != sp && fabsf(fprev - fc) > 1e-6f) return true;
     if (!(t + dt > t)) return false;
     t += dt;
@@ -1624,7 +1634,9 @@ __device__ __forceinline__ float rr_survival(const GuideShade& gs, F3 T) {
 // One iteration of the bounce loop of PathTracer::trace (trace.cpp:138-210) after
 // the march: handleVolume + sampleCondition for medium hits, else
 // Scene::intersect + handleSurface + medium switch, or the skydrome.
-template <bool kMulti>
+// kTri false: no scene of the launch holds a triangle mesh (as a primitive or as
+// its mean), so no BVH code is compiled.
+template <bool kMulti, bool kTri>
 __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = *a.nA;
@@ -1680,16 +1692,16 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
     MeanAt mp{};
     F3 gr;
     if (S.medium_kind == MUPSI_MEDIUM_GP) {
-      mp.mu = mean_eval<true>(S, Pp, &mp.grad);
-      field_eval<true>(S, Pp, iseed, cur, &gr, &mp);
+      mp.mu = mean_eval<true, kTri>(S, Pp, &mp.grad);
+      field_eval<true, kTri>(S, Pp, iseed, cur, &gr, &mp);
     } else {
-      gr = alt_grad(S, Pp, iseed);
+      gr = alt_grad<kTri>(S, Pp, iseed);
     }
     const F3 nrm = normalized(gr);
     const F3 wo = neg(ray.d);
     const F3 alb = mk(0.8f, 0.8f, 0.8f);
     LightSample ls;
-    if (choose_light(S, Pp, rng, ls)) {
+    if (choose_light<kTri>(S, Pp, rng, ls)) {
       Ray sr = make_ray(madd(nrm, S.eps, Pp), ls.d);
       const float bias = ls.dist - S.eps * fabsf(dot(ls.d, nrm));
       sr.farT = bias * 0.99f;
@@ -1723,7 +1735,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
         H.ac[pid] = ac;
         shadow = true;
       } else {
-        const float trans = march_serial(S, sr, iseed, cur) ? 0.0f : 1.0f;
+        const float trans = march_serial<kTri>(S, sr, iseed, cur) ? 0.0f : 1.0f;
         L = mk(fmaf(trans * (Tt.x * (f.x * ls.w.x)), ac, L.x),
                fmaf(trans * (Tt.y * (f.y * ls.w.y)), ac, L.y),
                fmaf(trans * (Tt.z * (f.z * ls.w.z)), ac, L.z));
@@ -1782,7 +1794,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
   if (alive && exited) {
     F3 Pp, Ng;
     float uu = 0.f, vv = 0.f;
-    const int hp = scene_intersect(S, ray, Pp, Ng, uu, vv);
+    const int hp = scene_intersect<kTri>(S, ray, Pp, Ng, uu, vv);
     if (hp >= 0) {
       // handleSurface (trace.cpp:32-75)
       const DevPrim& pr = S.prims[hp];
@@ -1792,11 +1804,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
       if (pr.bsdf == MUPSI_BSDF_LAMBERTIAN)
         alb = pr.albedo.rgb ? tex_at(pr.albedo, uu, vv, nullptr) : mk(0.8f, 0.8f, 0.8f);
       LightSample ls;
-      if (choose_light(S, Pp, rng, ls)) {
+      if (choose_light<kTri>(S, Pp, rng, ls)) {
         Ray sr = make_ray(madd(nrm, S.eps, Pp), ls.d);
         const float bias = ls.dist - S.eps * fabsf(dot(ls.d, nrm));
         sr.farT = bias * 0.99f;
-        if (!scene_occluded(S, sr)) {
+        if (!scene_occluded<kTri>(S, sr)) {
           const F3 w = bsdf_weight(pr.bsdf, alb, wo, nrm, ls.d);
           const float ac = fabsf(dot(ls.d, nrm));
           const F3 Tt = Te();
@@ -2499,6 +2511,39 @@ struct GuideBuffers {
   float* tr_w = nullptr;     // tr_capacity
 };
 
+using EventPairs = std::vector<std::pair<hipEvent_t, hipEvent_t>>;
+
+static void destroy_events(EventPairs& evs) {
+  for (auto& e : evs) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  evs.clear();
+}
+static double elapsed_ms(const EventPairs& evs) {
+  double ms = 0;
+  for (const auto& e : evs) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e.first, e.second);
+    ms += t;
+  }
+  return ms;
+}
+
+// A captured plain-tracer wavefront of n paths over the pixel grid: the graph
+// reads its first pass from kb_dev, so one capture serves every later launch
+// of the same size.
+struct WavefrontGraph {
+  hipGraphExec_t exec = nullptr;
+  int64_t n = -1;
+  int* kb_dev = nullptr;
+  EventPairs evs, gevs;  // empty: a capture records no events
+  DeviceBuffers mem;
+  ~WavefrontGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 struct mupsi_scene {
   mupsi_ctx* ctx = nullptr;
   DevScene ds{};
@@ -2513,7 +2558,19 @@ struct mupsi_scene {
   std::unique_ptr<GuideBuffers> gb;
   mupsi_guide* guide = nullptr;  // attached guide, or null (plain tracer)
   mupsi_stats last{};
+  // plain renders replay a captured wavefront (mupsi_scene_set_graph); the
+  // capture holds pointers to pb and the scene constants, so it is dropped
+  // whenever either changes
+  int use_graph = 0;
+  int64_t graph_builds = 0;
+  std::unique_ptr<WavefrontGraph> graph;
 };
+
+// The scene holds a triangle mesh, as a primitive or as the mean: the shading
+// kernel then needs the BVH code (its sphere-only instance has none).
+static bool scene_has_tri(const mupsi_scene* sc) {
+  return !sc->meshes.empty() || sc->ds.mean_type == MUPSI_MEAN_MESH;
+}
 
 // Distances from the centres of a grid of cubes to a mean mesh (local space): for
 // any point of a cube the distance is within the half diagonal of the centre's
@@ -2711,6 +2768,7 @@ extern "C" mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t me
       return st;
     }
   }
+  scene->graph.reset();  // the capture holds the old scene constants
   DevScene& D = scene->ds;
   D.gp = H.ds.gp;
   D.mean_type = H.ds.mean_type;
@@ -2734,6 +2792,7 @@ extern "C" mupsi_status mupsi_scene_set_settings(mupsi_scene* scene, int32_t max
     return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_settings: unknown gp_mode");
   HIP_TRY(hipSetDevice(scene->ctx->device));
   HIP_TRY(hipStreamSynchronize(scene->ctx->stream));
+  scene->graph.reset();  // the capture holds the old settings
   for (DevScene* d : {&scene->ds, &scene->host.ds}) {
     d->max_bounce = max_bounce;
     d->gp_mode = gp_mode == MUPSI_GP_RENEWAL_PLUS ? 1 : 0;
@@ -2792,6 +2851,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
 
 static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
   if (sc->pb && sc->pb->capacity >= n) return MUPSI_OK;
+  sc->graph.reset();  // it holds pointers into the old buffers
   return alloc_path_buffers(sc->pb, n, sc->ds.n_lights > 0 && sc->ds.has_gp, false);
 }
 
@@ -2822,6 +2882,21 @@ static mupsi_status ensure_guide_buffers(mupsi_scene* sc, int64_t n, int64_t rec
 
 void detach_guide(mupsi_scene* s, const mupsi_guide* g) {
   if (s && s->guide == g) s->guide = nullptr;
+}
+
+extern "C" mupsi_status mupsi_scene_set_graph(mupsi_scene* sc, int32_t use_graph) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_graph: null scene");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  sc->use_graph = use_graph ? 1 : 0;
+  if (!sc->use_graph) sc->graph.reset();
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_scene_graph_builds(mupsi_scene* sc, int64_t* builds) {
+  if (!sc || !builds) return fail(MUPSI_ERR_INVALID, "mupsi_scene_graph_builds: null argument");
+  *builds = sc->graph_builds;
+  return MUPSI_OK;
 }
 
 extern "C" mupsi_status mupsi_scene_set_guide(mupsi_scene* sc, mupsi_guide* g) {
@@ -2900,9 +2975,15 @@ struct GuideRun {
   double ms;  // guide prep + inference time (HIP events)
 };
 
-static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
-                                  const volatile int32_t* abort_flag, double* march_ms,
-                                  int* launches, GuideRun* gr = nullptr) {
+// Enqueues the wavefront over n paths described by src (init, then the bounce
+// loop's marches and shades) on the context stream, leaving per-path radiance
+// in P.L*.  With timed, HIP events bracket every march launch (evs) and guide
+// inference (gevs); the caller reads them once the stream has drained (a
+// graph capture records none).  abort_flag is polled between bounce
+// iterations (null: never, as under graph capture).
+static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
+                                      const volatile int32_t* abort_flag, EventPairs& evs,
+                                      EventPairs& gevs, GuideRun* gr, bool timed = true) {
   mupsi_ctx* ctx = sc->ctx;
   hipStream_t st = ctx->stream;
   PathBuffers& pb = *sc->pb;
@@ -2912,7 +2993,6 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
   InitArgs ia{pb.P, src, ni, pb.qA[0], nullptr};
   hipLaunchKernelGGL(k_init<false>, dim3(grid_for(n)), dim3(256), 0, st, S, ia);
   HIP_TRY(hipGetLastError());
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> evs, gevs;
   mupsi_status status = MUPSI_OK;
   GuideBuffers* gb = gr ? sc->gb.get() : nullptr;
   const int K = gr ? gr->g->cfg.max_records : 0;
@@ -2947,13 +3027,15 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
       ma.steps = nullptr;
       ma.work = work;
       ma.stats = pb.stats;
-      hipEvent_t e0, e1;
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
-      evs.emplace_back(e0, e1);
-      HIP_TRY(hipEventRecord(e0, st));
+      if (timed) {
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        evs.emplace_back(e0, e1);
+        HIP_TRY(hipEventRecord(e0, st));
+      }
       launch_march(ctx, S, ma, st);
-      HIP_TRY(hipEventRecord(e1, st));
+      if (timed) HIP_TRY(hipEventRecord(evs.back().second, st));
       HIP_TRY(hipGetLastError());
       if (gr) {
         hipEvent_t g0, g1;
@@ -2980,7 +3062,8 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
     if (defer) hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nS, 1, 0, -1);
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
                  defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(n)), dim3(256), 0, st, S, sa);
+    if (scene_has_tri(sc)) hipLaunchKernelGGL((k_shade<false, true>), dim3(grid_for(n)), dim3(256), 0, st, S, sa);
+    else hipLaunchKernelGGL((k_shade<false, false>), dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
     if (defer) {
       // transmittance of this iteration's medium light samples (gpmedium.cpp:105-129)
@@ -3004,36 +3087,70 @@ static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n
       ma.steps = nullptr;
       ma.work = work;
       ma.stats = pb.stats;
-      hipEvent_t e0, e1;
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
-      evs.emplace_back(e0, e1);
-      HIP_TRY(hipEventRecord(e0, st));
+      if (timed) {
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        evs.emplace_back(e0, e1);
+        HIP_TRY(hipEventRecord(e0, st));
+      }
       launch_march(ctx, S, ma, st);
-      HIP_TRY(hipEventRecord(e1, st));
+      if (timed) HIP_TRY(hipEventRecord(evs.back().second, st));
       NeeArgs na{pb.P, H, pb.qS, nS, gs};
       hipLaunchKernelGGL(k_nee_finish, dim3(grid_for(n)), dim3(256), 0, st, na);
       HIP_TRY(hipGetLastError());
     }
   }
+  return status;
+}
+
+// Run the wavefront over n paths described by src; leaves per-path radiance in P.L*.
+// With sc->use_graph, a plain render (no guide, no abort flag, the pixel grid)
+// replays a graph captured from the same sequence of launches.
+static mupsi_status run_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
+                                  const volatile int32_t* abort_flag, double* march_ms,
+                                  int* launches, GuideRun* gr = nullptr) {
+  hipStream_t st = sc->ctx->stream;
+  if (sc->use_graph && !gr && !abort_flag && !src.xs && !src.plan) {
+    if (!sc->graph || sc->graph->n != n) {
+      HIP_TRY(hipStreamSynchronize(st));
+      sc->graph.reset();
+      auto g = std::make_unique<WavefrontGraph>();
+      HIP_TRY(g->mem.alloc(1, &g->kb_dev));
+      PathSrc gsrc = src;
+      gsrc.k_begin_dev = g->kb_dev;
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      const mupsi_status cs =
+          enqueue_wavefront(sc, gsrc, n, nullptr, g->evs, g->gevs, nullptr, /*timed=*/false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (cs != MUPSI_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return cs;
+      }
+      HIP_TRY(ce);
+      const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie);
+      g->n = n;
+      sc->graph = std::move(g);
+      ++sc->graph_builds;
+    }
+    WavefrontGraph& g = *sc->graph;
+    HIP_TRY(hipMemcpyAsync(g.kb_dev, &src.k_begin, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipGraphLaunch(g.exec, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // the replayed launches are not timed one by one (march_ms stays as it is)
+    return MUPSI_OK;
+  }
+  EventPairs evs, gevs;
+  mupsi_status status = enqueue_wavefront(sc, src, n, abort_flag, evs, gevs, gr);
   HIP_TRY(hipStreamSynchronize(st));
-  double ms = 0;
-  for (auto& e : evs) {
-    float t = 0;
-    (void)hipEventElapsedTime(&t, e.first, e.second);
-    ms += t;
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
-  }
-  if (march_ms) *march_ms += ms;
+  if (march_ms) *march_ms += elapsed_ms(evs);
   if (launches) *launches += (int)evs.size();
-  for (auto& e : gevs) {
-    float t = 0;
-    (void)hipEventElapsedTime(&t, e.first, e.second);
-    if (gr) gr->ms += t;
-    (void)hipEventDestroy(e.first);
-    (void)hipEventDestroy(e.second);
-  }
+  if (gr) gr->ms += elapsed_ms(gevs);
+  destroy_events(evs);
+  destroy_events(gevs);
   return status;
 }
 
@@ -3078,7 +3195,7 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
                                            guide->cfg.train_passes - guide->stats.train_passes_done,
                                            (int64_t)(k_begin + k_count - kb), (int64_t)ppl})
                  : std::min(ppl, k_begin + k_count - kb);
-    PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
+    PathSrc src{sc->ds.w, (int)npix, kb, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr};
     GuideRun gr{guide, training ? 1 : 0, 0.0};
     st = run_wavefront(sc, src, (int64_t)kc * npix, abort_flag, &march_ms, &launches,
                        guide ? &gr : nullptr);
@@ -3208,7 +3325,7 @@ extern "C" mupsi_status mupsi_trace_paths(mupsi_scene* sc, const int32_t* x, con
   HIP_TRY(tmp.upload(k, n, &dk));
   PathBuffers& pb = *sc->pb;
   HIP_TRY(hipMemsetAsync(pb.stats, 0, 8 * sizeof(unsigned long long), sc->ctx->stream));
-  PathSrc src{sc->ds.w, sc->ds.w * sc->ds.h, 0, dx, dy, dk, nullptr, 0, nullptr};
+  PathSrc src{sc->ds.w, sc->ds.w * sc->ds.h, 0, nullptr, dx, dy, dk, nullptr, 0, nullptr};
   st = run_wavefront(sc, src, n, nullptr, nullptr, nullptr);
   if (st != MUPSI_OK) return st;
   float* drgb;
@@ -3237,6 +3354,7 @@ struct mupsi_batch {
   int max_bounce = 1;
   bool any_cached = false, any_plain = false;
   bool any_cached_mesh = false;  // window-marcher scenes with a mean mesh
+  bool any_tri = false;          // a scene holds a triangle mesh (primitive or mean)
   mupsi_batch_tuning tuning{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -3265,7 +3383,7 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
   PathBuffers& pb = *b->pb;
   const int64_t n = pb.capacity;
   mupsi_ctx* ctx = b->ctx;
-  PathSrc src{0, 1, 0, nullptr, nullptr, nullptr, b->d_plan, (int)b->scenes.size(), b->d_total};
+  PathSrc src{0, 1, 0, nullptr, nullptr, nullptr, nullptr, b->d_plan, (int)b->scenes.size(), b->d_total};
   hipLaunchKernelGGL(k_batch_counters, dim3(1), dim3(64), 0, st, pb.counters, b->d_total);
   InitArgs ia{pb.P, src, (int)n, pb.qA[0], b->d_scenes};
   hipLaunchKernelGGL(k_init<true>, dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, ia);
@@ -3329,7 +3447,8 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], pb.counters + 2 * nxt + 1,
                  gs, 0, ShadowSoA{}, nullptr, nullptr, b->d_scenes,
                  pb.qMm[nxt], nMm2, pb.qMs[nxt], nMs2};
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
+    if (b->any_tri) hipLaunchKernelGGL((k_shade<true, true>), dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
+    else hipLaunchKernelGGL((k_shade<true, false>), dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
   }
   for (size_t i = 0; i < b->scenes.size(); ++i) {
     const mupsi_scene* sc = b->scenes[i];
@@ -3455,6 +3574,7 @@ extern "C" mupsi_status mupsi_batch_create(mupsi_ctx* ctx, mupsi_scene* const* s
     ds.push_back(sc->ds);
     cc.push_back(cached_consts(sc->ds.gp));
     b->max_bounce = std::max(b->max_bounce, sc->ds.max_bounce);
+    if (scene_has_tri(sc)) b->any_tri = true;
     if (sc->ds.has_gp && !sc->ds.skip_medium) {
       if (!scene_uses_cached(sc->ds)) b->any_plain = true;
       else if (sc->ds.mean_type == MUPSI_MEAN_MESH) b->any_cached_mesh = true;
