@@ -1118,6 +1118,18 @@ __device__ __forceinline__ bool next_uncertain(const DevScene& S, const SkipInfo
   return classify_point<false>(S, k, q2, d, &fin) == 0;
 }
 
+// A bound on |f(p) - f(q)| / |p - q| for p, q in one cell (the same 27-cell
+// window): 1 for the sphere mean's distance, psi_scale x (27 ppc impulses) x
+// max |grad h| for the window sum, and for the conditioning term u h + g . grad h
+// |u| max|grad h| + |g| max|Hessian h| with max|Hessian h| <= 1.2132 |nk2|
+// (|nk2| e^{-x} (1 + 2x), x = r^2 / L^2); widened by 0.1% for rounding.
+__device__ __forceinline__ float skip_lipschitz(const DevGP& g, float psi_scale, int ppc,
+                                                const Cond& c) {
+  float l = 1.0f + fabsf(psi_scale) * (27.0f * (float)ppc) * g.grad_h_max;
+  if (c.active) l += fabsf(c.u) * g.grad_h_max + 1.2132f * fabsf(g.nk2) * norm(c.g);
+  return l * 1.001f;
+}
+
 // The window marcher: GPMedium::sampleDistance / transmittance
 // (gpmedium.cpp:46-129) with one group of GS lanes per ray (64 / GS rays per
 // wave).  The 27 x ppc impulses of the 3x3x3 cell window around the evaluation
@@ -1183,6 +1195,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   Cond c;
   c.active = false;
   int nsteps = 0;
+  float lip = 0.f;  // Lipschitz bound of f along the ray within one cell's window
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
@@ -1252,6 +1265,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
               c.g = mk(MARCH_ARG(cgx)[rid], MARCH_ARG(cgy)[rid], MARCH_ARG(cgz)[rid]);
             }
             sk = make_skip<kMesh>(scene(), c, o, d, ft);
+            lip = skip_lipschitz(G, cc.psi_scale, ppc, c);
             nsteps = 0;
             wvalid = false;  // new realization seed
             gi0 = -1;
@@ -1275,6 +1289,36 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
           if (!(t < ft)) {
             finish(false, q);
             break;
+          }
+          if (sk.kind == 1 && sp && !(t >= sk.ev0)) {
+            // Sphere mean, positive origin, t before the widened outer ball: every
+            // grid point up to ev0 lies outside that ball, so it is certain
+            // positive (the origin's sign) and the ray still approaches the
+            // centre (no early end).  They are stepped with the march's own f32
+            // additions, unclassified.  A ray that misses the ball (ev0 = inf)
+            // never changes sign: a miss, as the full march finds.
+            if (!(sk.ev0 < INFINITY)) {
+              finish(false, q);
+              break;
+            }
+            const float te = fminf(sk.ev0, ft);
+            int k = 0;
+            bool stuck = false;
+            while (t < te) {
+              ++k;
+              if (!(t + dt > t)) {
+                stuck = true;
+                break;
+              }
+              t += dt;
+            }
+            nsteps += k;
+            gi += k;
+            if (stuck) {
+              finish(false, q);
+              break;
+            }
+            continue;
           }
           const F3 p = madd(d, t, o);
           if (sk.kind == 1 && uncertain_t(sk, t)) {
@@ -1428,6 +1472,7 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
         } else if (!(t + dt > t)) {
           finish(false, q);
         } else {
+          float tl = t, fl = f;  // the last evaluated grid point
           t += dt;
           ++gi;
           if (pair) {  // the next grid point, as the next pass would take it
@@ -1437,6 +1482,32 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
             } else if (!(t + dt > t)) {
               finish(false, q2);
             } else {
+              tl = t;
+              fl = f2;
+              t += dt;
+            }
+          }
+          // Sphere means: the grid points after the last evaluated one that stay in
+          // its cell (the same 27-cell window, so f is Lipschitz there) and lie
+          // within |f| / lip of it keep f's sign, the origin's: no sign change,
+          // stepped with the march's own f32 additions.
+          if (phase == 2 && sk.kind == 1 && (fl > 0.0f) == sp) {
+            const float margin =
+                0.01f + 1e-3f * fabsf(fl) +
+                lip * 1e-5f * (fabsf(o.x) + fabsf(o.y) + fabsf(o.z) + tl);
+            const float reach = (fabsf(fl) - margin) / lip;
+            while (t < ft && t - tl < reach) {
+              const F3 p = madd(d, t, o);
+              if (cell_coord(p.x, G, cc.inv_kr, cc.kr_pow2) != wx ||
+                  cell_coord(p.y, G, cc.inv_kr, cc.kr_pow2) != wy ||
+                  cell_coord(p.z, G, cc.inv_kr, cc.kr_pow2) != wz)
+                break;
+              ++nsteps;
+              ++gi;
+              if (!(t + dt > t)) {
+                finish(false, p);
+                break;
+              }
               t += dt;
             }
           }
@@ -1637,10 +1708,7 @@ __device__ __forceinline__ float rr_survival(const GuideShade& gs, F3 T) {
 // kTri false: no scene of the launch holds a triangle mesh (as a primitive or as
 // its mean), so no BVH code is compiled.
 template <bool kMulti, bool kTri>
-__global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
-  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = *a.nA;
-  if (qi >= n) return;
+__device__ __forceinline__ void shade_path(const DevScene& S0, const ShadeArgs& a, int qi) {
   const int pid = a.qA[qi];
   const PathSoA& P = a.P;
   const DevScene& S = pick_scene<kMulti>(S0, a.scenes, P.sid, pid);
@@ -1912,6 +1980,12 @@ __global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
     wave_push(cont && medium && !S.skip_medium, pid, a.qM2, a.nM2);
   }
   if (a.defer_nee) wave_push(shadow, pid, a.qS, a.nS);
+}
+
+template <bool kMulti, bool kTri>
+__global__ __launch_bounds__(256) void k_shade(DevScene S0, ShadeArgs a) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi < *a.nA) shade_path<kMulti, kTri>(S0, a, qi);
 }
 
 // The deferred light samples after their march (k_march / k_march_window over
