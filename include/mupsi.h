@@ -223,6 +223,15 @@ mupsi_status mupsi_render_passes(mupsi_scene* scene, int32_t k_begin, int32_t k_
  * change.  mupsi_scene_graph_builds counts the captures made. */
 mupsi_status mupsi_scene_set_graph(mupsi_scene* scene, int32_t use_graph);
 mupsi_status mupsi_scene_graph_builds(mupsi_scene* scene, int64_t* builds);
+/* No reference counterpart (where the shading of a medium hit runs): with
+ * fused != 0 (the default) the GP-medium marcher of a render shades the hits of
+ * a sphere-mean GP medium itself -- PathTracer::handleVolume without emitters
+ * (trace.cpp:77-116) and GPMedium::sampleCondition (gpmedium.cpp:133-162) --
+ * and marches the path's next ray in the same launch, when the scene has no
+ * emitter and no guide is attached; other scenes are not affected.  The
+ * results are bit for bit those of fused = 0, where every hit goes back to the
+ * shading kernel of the bounce loop. */
+mupsi_status mupsi_scene_set_fused(mupsi_scene* scene, int32_t fused);
 mupsi_status mupsi_reset_accum(mupsi_scene* scene);
 /* rgb_sum: host, width*height*3 doubles (row-major, y = row); sample_count: passes accumulated. */
 mupsi_status mupsi_read_accum(mupsi_scene* scene, double* rgb_sum, int64_t* sample_count);

@@ -113,6 +113,12 @@ class Scene:
         check(self.lib.mupsi_scene_set_graph(self.handle, 1 if use_graph else 0),
               "mupsi_scene_set_graph")
 
+    def set_fused(self, fused: bool = True) -> None:
+        """Shade the medium hits of a sphere-mean GP medium inside the march launch
+        (mupsi_scene_set_fused; the default); the images are the same bits either way."""
+        check(self.lib.mupsi_scene_set_fused(self.handle, 1 if fused else 0),
+              "mupsi_scene_set_fused")
+
     def graph_builds(self) -> int:
         n = C.c_int64()
         check(self.lib.mupsi_scene_graph_builds(self.handle, C.byref(n)), "mupsi_scene_graph_builds")

@@ -127,6 +127,7 @@ SIGNATURES = {
     "mupsi_render_passes": (I32, [VP, I32, I32, I32, VP]),
     "mupsi_scene_set_graph": (I32, [VP, I32]),
     "mupsi_scene_graph_builds": (I32, [VP, C.POINTER(I64)]),
+    "mupsi_scene_set_fused": (I32, [VP, I32]),
     "mupsi_reset_accum": (I32, [VP]),
     "mupsi_read_accum": (I32, [VP, VP, C.POINTER(I64)]),
     "mupsi_write_accum": (I32, [VP, VP, I64]),

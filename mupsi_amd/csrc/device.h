@@ -64,12 +64,16 @@ __device__ __forceinline__ uint32_t make_seed5(uint32_t a, uint32_t b, uint32_t 
   return hash_combine(make_seed4(a, b, c, d), e);
 }
 // PCG-XSH-RR 64/32, multiplier 6364136223846793005, increment 1.
-__device__ __forceinline__ uint32_t pcg_next(uint64_t& s) {
-  const uint64_t old = s;
-  s = old * 6364136223846793005ULL + 1ULL;
+// The output permutation of the state a draw starts from.
+__device__ __forceinline__ uint32_t pcg_output(uint64_t old) {
   const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
   const uint32_t rot = (uint32_t)(old >> 59u);
   return __builtin_amdgcn_alignbit(xs, xs, rot);  // rotate right by rot
+}
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& s) {
+  const uint64_t old = s;
+  s = old * 6364136223846793005ULL + 1ULL;
+  return pcg_output(old);
 }
 // Random::normalizedUint: mantissa fill, [0, 1)
 __device__ __forceinline__ float u01(uint32_t i) {
@@ -211,7 +215,32 @@ __device__ __forceinline__ void cell_index(const DevGP& g, F3 p, int& cx, int& c
   cz = (int)(p.z / g.kr);
 }
 
+// One impulse's terms of psi's sums at p: (w h, w (p - s)_x k, w (p - s)_y k,
+// w (p - s)_z k) with h = exp(-|p - s|^2 / L^2), k = h (-2 / L^2), s the impulse
+// position and w = +-1 its weight (kernel.h:57-66).  Adding w t is adding or
+// subtracting t exactly as the reference's `w * h` terms do, since w * t = +-t
+// exactly.  psi_eval and the marcher's exact sums at a medium hit share it.
+__device__ __forceinline__ float4 psi_terms(const DevGP& g, F3 p, float sx, float sy, float sz,
+                                            float w) {
+  const float ddx = p.x - sx;
+  const float ddy = p.y - sy;
+  const float ddz = p.z - sz;
+  const float r2v = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
+  const float h = expf(-r2v / g.L2);
+  const float k = h * g.nk2;
+  return make_float4(w * h, w * (ddx * k), w * (ddy * k), w * (ddz * k));
+}
+
+// RawNoise's normalisation sigma * sum / sqrt(Var) of the accumulated sums.
+__device__ __forceinline__ void psi_finish(const DevGP& g, float sv, float sx, float sy, float sz,
+                                           float& v, F3& grad) {
+  v = g.sigma * sv / g.sqrt_var;
+  grad = mk(g.sigma * sx / g.sqrt_var, g.sigma * sy / g.sqrt_var, g.sigma * sz / g.sqrt_var);
+}
+
 // psi value and gradient at p (RawNoise, gpnoise.h:44); sum of |h| in *mass.
+// The sums run over the 27 neighbour cells in (dx, dy, dz) order, -1..1 each,
+// and over a cell's impulses in draw order: one float addition chain per sum.
 template <bool kGrad>
 __device__ __forceinline__ void psi_eval(const DevGP& g, F3 p, uint32_t seed, float& v,
                                          F3& grad, float* mass) {
@@ -233,26 +262,22 @@ __device__ __forceinline__ void psi_eval(const DevGP& g, F3 p, uint32_t seed, fl
           const uint32_t r1 = pcg_next(s);
           const uint32_t r2 = pcg_next(s);
           const uint32_t r3 = pcg_next(s);
-          const float ddx = p.x - fmaf(u01(r2), g.kr, bx);
-          const float ddy = p.y - fmaf(u01(r1), g.kr, by);
-          const float ddz = p.z - fmaf(u01(r0), g.kr, bz);
-          const float r2v = fmaf(ddx, ddx, fmaf(ddy, ddy, ddz * ddz));
-          const float h = expf(-r2v / g.L2);
-          const bool pos = (r3 >> 31) != 0;
-          sv = pos ? sv + h : sv - h;
+          const float4 t = psi_terms(g, p, fmaf(u01(r2), g.kr, bx), fmaf(u01(r1), g.kr, by),
+                                     fmaf(u01(r0), g.kr, bz), (r3 >> 31) ? 1.0f : -1.0f);
+          sv = sv + t.x;
           if (kGrad) {
-            const float k = h * g.nk2;
-            sx = pos ? sx + ddx * k : sx - ddx * k;
-            sy = pos ? sy + ddy * k : sy - ddy * k;
-            sz = pos ? sz + ddz * k : sz - ddz * k;
+            sx = sx + t.y;
+            sy = sy + t.z;
+            sz = sz + t.w;
           }
-          if (mass) m += h;
+          if (mass) m += fabsf(t.x);
         }
       }
     }
   }
-  v = g.sigma * sv / g.sqrt_var;
-  if (kGrad) grad = mk(g.sigma * sx / g.sqrt_var, g.sigma * sy / g.sqrt_var, g.sigma * sz / g.sqrt_var);
+  F3 gr;
+  psi_finish(g, sv, sx, sy, sz, v, gr);
+  if (kGrad) grad = gr;
   if (mass) *mass = g.sigma * m / g.sqrt_var;
 }
 
@@ -881,6 +906,30 @@ __device__ __forceinline__ float grid_medium_eval(const DevScene& S, F3 p, uint3
   return mu + noise;
 }
 
+// GPMedium::evalWithGradient's sum (gpmedium.cpp:19-41) of the mean (mu, grad mu),
+// psi (pv, grad psi) and the conditioning term of state c, at p.
+template <bool kGrad>
+__device__ __forceinline__ float field_combine(const DevGP& gp, F3 p, const Cond& c, float mu,
+                                               F3 mg, float pv, F3 pg, F3* grad) {
+  float f = mu + pv;
+  F3 g = mk(mg.x + pg.x, mg.y + pg.y, mg.z + pg.z);
+  if (c.active) {
+    const F3 r = sub(p, c.C);
+    const float h = kern_h(gp, c.C, p);
+    const F3 hg = mul(r, h * gp.nk2);
+    f = f + fmaf(c.u, h, dot(c.g, hg));
+    if (kGrad) {
+      const float k4 = gp.k4 * dot(r, c.g);
+      const F3 in = mk(fmaf(k4, r.x, gp.nk2 * c.g.x), fmaf(k4, r.y, gp.nk2 * c.g.y),
+                       fmaf(k4, r.z, gp.nk2 * c.g.z));
+      const F3 Hg = mul(in, h);
+      g = mk(g.x + fmaf(c.u, hg.x, Hg.x), g.y + fmaf(c.u, hg.y, Hg.y), g.z + fmaf(c.u, hg.z, Hg.z));
+    }
+  }
+  if (kGrad) *grad = g;
+  return f;
+}
+
 // GPMedium::evalWithGradient(p, seed, state) (gpmedium.cpp:19-41).  mean: the
 // mean's value and gradient at p when the caller has them already.
 template <bool kGrad, bool kMesh = true>
@@ -896,24 +945,7 @@ __device__ __forceinline__ float field_eval(const DevScene& S, F3 p, uint32_t se
   }
   float pv;
   psi_eval<kGrad>(S.gp, p, seed, pv, pg, nullptr);
-  float f = mu + pv;
-  F3 g = mk(mg.x + pg.x, mg.y + pg.y, mg.z + pg.z);
-  if (c.active) {
-    const DevGP& gp = S.gp;
-    const F3 r = sub(p, c.C);
-    const float h = kern_h(gp, c.C, p);
-    const F3 hg = mul(r, h * gp.nk2);
-    f = f + fmaf(c.u, h, dot(c.g, hg));
-    if (kGrad) {
-      const float k4 = gp.k4 * dot(r, c.g);
-      const F3 in = mk(fmaf(k4, r.x, gp.nk2 * c.g.x), fmaf(k4, r.y, gp.nk2 * c.g.y),
-                       fmaf(k4, r.z, gp.nk2 * c.g.z));
-      const F3 Hg = mul(in, h);
-      g = mk(g.x + fmaf(c.u, hg.x, Hg.x), g.y + fmaf(c.u, hg.y, Hg.y), g.z + fmaf(c.u, hg.z, Hg.z));
-    }
-  }
-  if (kGrad) *grad = g;
-  return f;
+  return field_combine<kGrad>(S.gp, p, c, mu, mg, pv, pg, grad);
 }
 
 // Bound on |conditioning term| for every p: |u| h + |g| |grad h| <= |u| + |g| max|grad h|.

@@ -772,6 +772,15 @@ struct MarchArgs {
   unsigned long long* stats;  // [0] field evals, [1] t-grid points (reference work), [2] rays
   const DevScene* scenes;     // batched launch: per-scene constants, indexed by sid[ray]
   const uint16_t* sid;
+  // fuse != 0: the marcher shades the rays' GP-medium hits itself and marches on
+  // (render wavefront of a sphere-mean GP medium without emitters or guide);
+  // the path state it updates, and path id -> (x, y, k) as in PathSrc
+  int fuse;
+  uint64_t* rng;
+  float *Tx, *Ty, *Tz;
+  int src_W, src_npix, src_k_begin;
+  const int* src_k_begin_dev;
+  const int32_t *src_xs, *src_ys, *src_ks;
 };
 static_assert(std::is_standard_layout<MarchArgs>::value, "MarchArgs fields are read by offset");
 
@@ -1118,6 +1127,49 @@ __device__ __forceinline__ bool next_uncertain(const DevScene& S, const SkipInfo
   return classify_point<false>(S, k, q2, d, &fin) == 0;
 }
 
+// ====================================================================== device: Renewal+
+
+// GPMedium::sampleCondition (gpmedium.cpp:133-162) as a 4x4 FP64 solve of
+//   [[h(C,C), grad h(C,C)^T], [grad h(C,C), H_h(C,C)]] [u; g] =
+//   [-mu(C)/sigma - psi(C); n - grad mu(C) - grad psi(C)]
+// with psi the next realization.  At r = 0 the matrix is diag(1, -2/L^2 I),
+// which gives the reference's closed form u = -mu/sigma - psi,
+// g = (-L^2/2)(n - grad mu - grad psi).
+// The FP64 system at C for the mean (mu, grad mu) and the next realization's
+// psi (pv, grad psi) at C.  The kernel is evaluated at r = C - C = 0, where the
+// off-diagonal entries h (-2/L^2) r_j and the r_j r_k parts of the Hessian are
+// exactly zero: Gaussian elimination (partial pivoting keeps every pivot on
+// the diagonal) leaves the right-hand side as it is and divides it by the
+// diagonal, which is what this computes, in FP64, from the same entries.
+__device__ __forceinline__ void renewal_solve_at(const DevGP& gp, F3 n, float mu, F3 mg, float pv,
+                                                 F3 pg, double& u, double g[3]) {
+  const double L2 = gp.L2;
+  const double h = exp(-0.0 / L2);               // h(C, C)
+  const double hjj = h * ((-2.0 / L2) + 0.0);    // H_h(C, C)_jj; r_j r_k = 0
+  const double b0 = -(double)mu / (double)gp.sigma - (double)pv;
+  u = b0 / h;
+  g[0] = ((double)n.x - (double)mg.x - (double)pg.x) / hjj;
+  g[1] = ((double)n.y - (double)mg.y - (double)pg.y) / hjj;
+  g[2] = ((double)n.z - (double)mg.z - (double)pg.z) / hjj;
+}
+
+__device__ __forceinline__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next,
+                                              double& u, double g[3],
+                                              const MeanAt* mean = nullptr) {
+  F3 pg;
+  float pv;
+  psi_eval<true>(S.gp, C, seed_next, pv, pg, nullptr);
+  F3 mg;
+  float mu;
+  if (mean) {  // the same mean_eval<true>(S, C) the caller made
+    mu = mean->mu;
+    mg = mean->grad;
+  } else {
+    mu = mean_eval<true>(S, C, &mg);
+  }
+  renewal_solve_at(S.gp, n, mu, mg, pv, pg, u, g);
+}
+
 // A bound on |f(p) - f(q)| / |p - q| for p, q in one cell (the same 27-cell
 // window): 1 for the sphere mean's distance, psi_scale x (27 ppc impulses) x
 // max |grad h| for the window sum, and for the conditioning term u h + g . grad h
@@ -1145,9 +1197,8 @@ __device__ __forceinline__ float skip_lipschitz(const DevGP& g, float psi_scale,
 //   kMulti   batched launch: every ray takes its scene's constants by sid[ray]
 // The MarchArgs stay the first parameter: MARCH_ARG reads them at offset 0.
 template <int PPC, bool kMesh, bool kMulti, int GS = kGroup>
-__global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
-                                                     CachedConsts cc0,
-                                                     const CachedConsts* ccs) {
+__device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene& S0,
+                                             const CachedConsts& cc0, const CachedConsts* ccs) {
   static_assert(GS == 8 || GS == 16 || GS == 32, "a ray's group is 8, 16 or 32 lanes");
   constexpr uint64_t kGroupMask = (1ull << GS) - 1ull;
   constexpr int NG = 64 / GS;  // rays (groups) per wave
@@ -1162,14 +1213,17 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   // batched launches) the extra LDS would cost resident blocks instead.
   constexpr bool kLdsTables = !kMesh && !kMulti && sizeof(win) <= 8192;
   __shared__ uint8_t new_off[kLdsTables ? 28 * 27 : 1];
-  __shared__ ulonglong2 jump[kLdsTables ? kMaxPpcCached : 1];
+  __shared__ ulonglong2 states[kLdsTables ? 4 * PPC : 1];
+  const int n = a.count_dev ? *a.count_dev : a.count_host;
+  // an empty queue (later bounce iterations where no path went back into the
+  // medium): the block ends before any setup (the whole block, one wave, returns)
+  if (n <= 0) return;
   if constexpr (kLdsTables) {
     for (int i = threadIdx.x; i < 28 * 27; i += 64) new_off[i] = kNewCells.off[i / 27][i % 27];
-    if (threadIdx.x < kMaxPpcCached)
-      jump[threadIdx.x] = make_ulonglong2(kJump.a[threadIdx.x], kJump.c[threadIdx.x]);
+    if (threadIdx.x < 4 * PPC)
+      states[threadIdx.x] = make_ulonglong2(kStates.a[threadIdx.x], kStates.c[threadIdx.x]);
     // (the first pass's barrier orders these writes before any read)
   }
-  const int n = a.count_dev ? *a.count_dev : a.count_host;
   const int lane = threadIdx.x;
   const int grp = lane / GS;
   const int gl = lane & (GS - 1);
@@ -1184,7 +1238,9 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     else return S0;
   };
   // ray state: identical in the GS lanes of a group
-  int phase = 0;  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left
+  // 0 needs a ray, 1 evaluate origin, 2 stepping, 3 no work left, 4/5 the ray ended
+  // (miss/hit) in this pass, 7 Renewal+ conditioning of a shaded hit
+  int phase = 0;
   int rid = 0;
   F3 o = mk(0, 0, 0), d = mk(0, 0, 0);
   float t = 0.f, ft = 0.f, dt = 0.f, nt = 0.f, fprev = 0.f;
@@ -1199,6 +1255,14 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
+  // Medium hits shaded in the marcher (MarchArgs::fuse): phase 6 never persists
+  // (a hit, phase 5, is shaded at the end of its pass), phase 7 conditions the
+  // next ray at C = c.C with normal c.g (the window is regenerated there with the
+  // next realization in that pass), and next_ray marks a phase-0 group whose next
+  // ray is already in its registers.
+  constexpr bool kFuse = !kMesh && !kMulti;
+  const bool fuse = kFuse && MARCH_ARG(fuse) != 0;
+  bool next_ray = false;
   // mesh means: Mesh::inside at grid points gi0 .. gi0 + GS - 1 of the current ray
   int gi = 0, gi0 = -1;
   uint32_t imask = 0;
@@ -1231,8 +1295,11 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
     // next ray in the next pass (a second fetch round cost more than the pass it
     // saves such a group).
     {
-      const bool need = phase == 0;
+      const bool need = phase == 0 && !next_ray;
       const uint64_t m = __ballot(need && gl == 0);
+      // a path whose hit the marcher shaded marches its next ray (set up in
+      // registers) without going back to the queue
+      bool take = phase == 0 && next_ray;
       if (m) {
         const int leader = __ffsll((unsigned long long)m) - 1;
         int base = 0;
@@ -1254,7 +1321,6 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
             nt = MARCH_ARG(nt)[rid];
             const float* const ftp = MARCH_ARG(ft);
             ft = fminf(ftp ? ftp[rid] : 3.402823466e+38f, 2000.0f);
-            dt = fminf(0.5f, (ft - nt) / 100);
             seed = MARCH_ARG(seed)[rid];
             const uint32_t* const flags = MARCH_ARG(flags);
             const uint8_t* const act = MARCH_ARG(active);
@@ -1264,17 +1330,22 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
               c.u = MARCH_ARG(cu)[rid];
               c.g = mk(MARCH_ARG(cgx)[rid], MARCH_ARG(cgy)[rid], MARCH_ARG(cgz)[rid]);
             }
-            sk = make_skip<kMesh>(scene(), c, o, d, ft);
-            lip = skip_lipschitz(G, cc.psi_scale, ppc, c);
-            nsteps = 0;
-            wvalid = false;  // new realization seed
-            gi0 = -1;
-            if (gl == 0) ++rays;
-            phase = 1;
+            take = true;
           } else {
             phase = 3;
           }
         }
+      }
+      if (take) {
+        next_ray = false;
+        dt = fminf(0.5f, (ft - nt) / 100);
+        sk = make_skip<kMesh>(scene(), c, o, d, ft);
+        lip = skip_lipschitz(G, cc.psi_scale, ppc, c);
+        nsteps = 0;
+        wvalid = false;  // new realization seed
+        gi0 = -1;
+        if (gl == 0) ++rays;
+        phase = 1;
       }
       if (phase == 1) {
         q = o;
@@ -1353,11 +1424,13 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
       }
     }
     if (__ballot(phase != 3) == 0) break;
+    const bool cond_pass = kFuse && phase == 7;
+    if (cond_pass) q = c.C;
 
     // Window: cells entering the 27-cell neighbourhood of the evaluation point.
     int newcells = -1, nnew = 0;
     int cx = 0, cy = 0, cz = 0;
-    if (do_eval) {
+    if (do_eval || cond_pass) {
       cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
       cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
       cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
@@ -1377,9 +1450,9 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
         }
       }
     }
-    // Regeneration of the entered cells of every group of the wave as one pool of
-    // items over the 64 lanes (item j of group g: impulse i of entered cell k),
-    // written to the same window slots a group's own lanes would write.
+    // Regeneration of the entered cells of every group of the wave, as one pool
+    // over the 64 lanes, written to the same window slots a group's own lanes
+    // would write.
     const int items = nnew * ppc;
     if (gl == 0 && newcells >= 0) {
       regen_cell[grp] = make_int4(cx, cy, cz, (int)seed);
@@ -1387,29 +1460,91 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
                                  __float_as_int(G.kr));
     }
     __syncthreads();  // the previous evaluation's reads of the window are complete
-    int start[NG + 1];  // first pooled item of each group (wave-uniform)
+    int start[NG + 1];  // first pooled item (batched) or entered cell of each group (wave-uniform)
     start[0] = 0;
+    if constexpr (kMulti) {
+      // item j of group g: impulse i of entered cell k, with the group's own ppc
 #pragma unroll
-    for (int g = 0; g < NG; ++g) start[g + 1] = start[g] + __builtin_amdgcn_readlane(items, g * GS);
-    for (int j = lane; j < start[NG]; j += 64) {
-      int g = 0;
+      for (int g = 0; g < NG; ++g) start[g + 1] = start[g] + __builtin_amdgcn_readlane(items, g * GS);
+      for (int j = lane; j < start[NG]; j += 64) {
+        int g = 0;
 #pragma unroll
-      for (int h = 1; h < NG; ++h) g += j >= start[h] ? 1 : 0;
-      const int4 rc = regen_cell[g];
-      const int2 ra = regen_aux[g];
-      const int gppc = kMulti ? ra.x >> 11 : PPC;  // a compile-time divisor when it can be
-      const int it = j - start[g];
-      const int k = it / gppc;
-      const int i = it - k * gppc;
-      const int no = kLdsTables ? new_off[(ra.x & 31) * 27 + k] : kNewCells.off[ra.x & 31][k];
-      const int ox = (no & 3) - 1, oy = ((no >> 2) & 3) - 1, oz = (no >> 4) - 1;  // in {-1,0,1}
-      int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
-      rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
-      ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
-      rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
-      const ulonglong2 jm = kLdsTables ? jump[i] : make_ulonglong2(kJump.a[i], kJump.c[i]);
-      win[g][(rx * 9 + ry * 3 + rz) * gppc + i] = make_impulse(
-          __int_as_float(ra.y), rc.x + ox, rc.y + oy, rc.z + oz, (uint32_t)rc.w, jm.x, jm.y);
+        for (int h = 1; h < NG; ++h) g += j >= start[h] ? 1 : 0;
+        const int4 rc = regen_cell[g];
+        const int2 ra = regen_aux[g];
+        const int gppc = ra.x >> 11;
+        const int it = j - start[g];
+        const int k = it / gppc;
+        const int i = it - k * gppc;
+        const int no = kNewCells.off[ra.x & 31][k];
+        const int ox = (no & 3) - 1, oy = ((no >> 2) & 3) - 1, oz = (no >> 4) - 1;  // in {-1,0,1}
+        int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
+        rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
+        ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
+        rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
+        win[g][(rx * 9 + ry * 3 + rz) * gppc + i] = make_impulse(
+            __int_as_float(ra.y), rc.x + ox, rc.y + oy, rc.z + oz, (uint32_t)rc.w, kJump.a[i],
+            kJump.c[i]);
+      }
+    } else {
+      // Two steps, 64 entered cells at a time: one lane per cell decodes it once
+      // (coordinates, first PCG state make_seed + 1, base position n kr, window
+      // slot); then one lane per impulse i takes the cell's values by a lane
+      // permute and its four draws from states 4i .. 4i + 3 of the cell's stream
+      // (make_impulse's draws, computed without the chain of steps).
+#pragma unroll
+      for (int g = 0; g < NG; ++g) start[g + 1] = start[g] + __builtin_amdgcn_readlane(nnew, g * GS);
+      const int ncell = start[NG];
+      float4* const wflat = &win[0][0];
+      for (int c0 = 0; c0 < ncell; c0 += 64) {
+        const int cl = c0 + lane;
+        uint32_t s0 = 0u;
+        float bx = 0.0f, by = 0.0f, bz = 0.0f;
+        int slot = 0;
+        if (cl < ncell) {
+          int g = 0;
+#pragma unroll
+          for (int h = 1; h < NG; ++h) g += cl >= start[h] ? 1 : 0;
+          const int4 rc = regen_cell[g];
+          const int2 ra = regen_aux[g];
+          const int k = cl - start[g];
+          const int no = kLdsTables ? new_off[(ra.x & 31) * 27 + k] : kNewCells.off[ra.x & 31][k];
+          const int ox = (no & 3) - 1, oy = ((no >> 2) & 3) - 1, oz = (no >> 4) - 1;  // in {-1,0,1}
+          int rx = ((ra.x >> 5) & 3) + ox, ry = ((ra.x >> 7) & 3) + oy, rz = ((ra.x >> 9) & 3) + oz;
+          rx = rx < 0 ? rx + 3 : (rx > 2 ? rx - 3 : rx);
+          ry = ry < 0 ? ry + 3 : (ry > 2 ? ry - 3 : ry);
+          rz = rz < 0 ? rz + 3 : (rz > 2 ? rz - 3 : rz);
+          const int nx = rc.x + ox, ny = rc.y + oy, nz = rc.z + oz;
+          s0 = make_seed4((uint32_t)nx, (uint32_t)ny, (uint32_t)nz, (uint32_t)rc.w) + 1u;
+          bx = (float)nx * G.kr;
+          by = (float)ny * G.kr;
+          bz = (float)nz * G.kr;
+          slot = g * (27 * PPC) + (rx * 9 + ry * 3 + rz) * PPC;
+        }
+        const int nitem = min(64, ncell - c0) * PPC;
+        for (int m = 0; m < nitem; m += 64) {  // every lane takes part in the permutes
+          const int j = m + lane;
+          const int c = j / PPC;
+          const int src = min(c, 63) * 4;
+          const uint32_t cs0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)s0);
+          const float ex = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(bx)));
+          const float ey = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(by)));
+          const float ez = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(bz)));
+          const int sl = __builtin_amdgcn_ds_bpermute(src, slot);
+          if (j < nitem) {
+            const int i = j - c * PPC;
+            uint32_t r[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              const int k = 4 * i + d;
+              const ulonglong2 st = kLdsTables ? states[k] : make_ulonglong2(kStates.a[k], kStates.c[k]);
+              r[d] = pcg_draw_from(cs0, st.x, st.y);
+            }
+            wflat[sl + i] = make_float4(fmaf(u01(r[2]), G.kr, ex), fmaf(u01(r[1]), G.kr, ey),
+                                        fmaf(u01(r[0]), G.kr, ez), (r[3] >> 31) ? 1.0f : -1.0f);
+          }
+        }
+      }
     }
     if (newcells >= 0) {
       wx = cx;
@@ -1514,7 +1649,138 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
         }
       }
     }
-    if (phase >= 4) {  // retire the ray: outputs at the final t, p = o + d t
+    if constexpr (kFuse) {
+      // Medium hits shaded here, as the shading kernel's handleVolume does for a
+      // scene without emitters (trace.cpp:77-116; Renewal+, gpmedium.cpp:133-162):
+      // the hit's normal (psi at the hit point, this realization) and the
+      // conditioning (psi at C = the hit point, the next realization) take psi
+      // and its gradient from the group's window, summed in psi_eval's order, so
+      // every value is the shading kernel's.  Each lane first replaces its own
+      // slots (the ones it summed) by their terms; after the barrier every lane
+      // of the group adds the 27 x PPC terms in that order.
+      const bool hit_now = fuse && phase == 5;
+      if (__ballot(hit_now || cond_pass) != 0) {
+        const F3 at = hit_now ? madd(d, t, o) : c.C;  // the hit point, or C
+        if (hit_now || cond_pass) {
+          for (int sl = gl; sl < 27 * PPC; sl += GS) {
+            const float4 im = W[sl];
+            W[sl] = psi_terms(G, at, im.x, im.y, im.z, im.w);
+          }
+        }
+        __syncthreads();
+        if (hit_now || cond_pass) {
+          float sv = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f;
+          const int r0x = mod3(wx - 1), r0y = mod3(wy - 1), r0z = mod3(wz - 1);
+#pragma unroll 1
+          for (int ax = 0; ax < 3; ++ax) {  // neighbour dx = ax - 1, window residue rx
+            const int rx = r0x + ax < 3 ? r0x + ax : r0x + ax - 3;
+#pragma unroll 1
+            for (int ay = 0; ay < 3; ++ay) {
+              const int ry = r0y + ay < 3 ? r0y + ay : r0y + ay - 3;
+#pragma unroll 1
+              for (int az = 0; az < 3; ++az) {
+                const int rz = r0z + az < 3 ? r0z + az : r0z + az - 3;
+                const float4* cw = W + (rx * 9 + ry * 3 + rz) * PPC;
+#pragma unroll
+                for (int i = 0; i < PPC; ++i) {
+                  const float4 tm = cw[i];
+                  sv = sv + tm.x;
+                  sx = sx + tm.y;
+                  sy = sy + tm.z;
+                  sz = sz + tm.w;
+                }
+              }
+            }
+          }
+          float pv;
+          F3 pg, mg;
+          psi_finish(G, sv, sx, sy, sz, pv, pg);
+          const float mu = mean_eval<true, false>(scene(), at, &mg);
+          wvalid = false;  // the window holds terms now
+          if (hit_now) {
+            F3 gr;
+            field_combine<true>(G, at, c, mu, mg, pv, pg, &gr);
+            const F3 nrm = normalized(gr);
+            const uint32_t fl = MARCH_ARG(flags)[rid];
+            Pcg rng{MARCH_ARG(rng)[rid]};
+            F3 T = mk(MARCH_ARG(Tx)[rid], MARCH_ARG(Ty)[rid], MARCH_ARG(Tz)[rid]);
+            F3 wi, wt;
+            bsdf_sample(MUPSI_BSDF_LAMBERTIAN, mk(0.8f, 0.8f, 0.8f), neg(d), nrm, rng, wi, wt);
+            T = cmul(T, wt);
+            const float surv = max3(T);
+            const int bounce = (int)(fl & kBounceMask) + 1;
+            if (gl == 0) refsteps += (uint32_t)nsteps;
+            if (rng.next1D() > surv || bounce >= scene().max_bounce) {
+              // the path ends here (roulette, or its last bounce); its radiance
+              // stays as it is and the shading kernel passes it over
+              if (gl == 0) const_cast<uint8_t*>(MARCH_ARG(hit))[rid] = 2;
+              phase = 0;
+            } else {
+              T = divs(T, surv);
+              const bool single = scene().gp_mode != MUPSI_GP_RENEWAL_PLUS;
+              const PathSrc src{MARCH_ARG(src_W),   MARCH_ARG(src_npix), MARCH_ARG(src_k_begin),
+                                MARCH_ARG(src_k_begin_dev), MARCH_ARG(src_xs), MARCH_ARG(src_ys),
+                                MARCH_ARG(src_ks), nullptr, 0, nullptr};
+              int x, y, k;
+              path_xyk(src, rid, x, y, k);
+              const uint32_t ns =
+                  single ? scene().single_seed_i
+                         : constant_sampler_i(make_seed5((uint32_t)x, (uint32_t)y, (uint32_t)k, 42u,
+                                                         (uint32_t)bounce));
+              const Ray nr = make_ray(madd(nrm, scene().eps, at), wi);
+              if (gl == 0) {  // the path state the shading kernel would leave
+                MARCH_ARG(rng)[rid] = rng.s;
+                MARCH_ARG(Tx)[rid] = T.x;
+                MARCH_ARG(Ty)[rid] = T.y;
+                MARCH_ARG(Tz)[rid] = T.z;
+                const_cast<float*>(MARCH_ARG(ox))[rid] = nr.o.x;
+                const_cast<float*>(MARCH_ARG(oy))[rid] = nr.o.y;
+                const_cast<float*>(MARCH_ARG(oz))[rid] = nr.o.z;
+                const_cast<float*>(MARCH_ARG(dx))[rid] = nr.d.x;
+                const_cast<float*>(MARCH_ARG(dy))[rid] = nr.d.y;
+                const_cast<float*>(MARCH_ARG(dz))[rid] = nr.d.z;
+                const_cast<float*>(MARCH_ARG(nt))[rid] = scene().near_eps;
+                const_cast<uint32_t*>(MARCH_ARG(flags))[rid] =
+                    (uint32_t)bounce | kFlagMedium | (single ? 0u : kFlagCond) | (fl & kRecMask);
+                const_cast<uint32_t*>(MARCH_ARG(seed))[rid] = ns;
+              }
+              o = nr.o;
+              d = nr.d;
+              nt = scene().near_eps;
+              ft = 2000.0f;  // min(FLT_MAX, 2000): render rays have no far bound
+              seed = ns;
+              c.active = false;
+              if (single) {
+                next_ray = true;
+                phase = 0;
+              } else {  // condition at C = the hit point with normal nrm first
+                c.C = at;
+                c.g = nrm;
+                phase = 7;
+              }
+            }
+          } else {
+            double u, g3[3];
+            renewal_solve_at(G, c.g, mu, mg, pv, pg, u, g3);
+            c.active = true;
+            c.u = (float)u;
+            c.g = mk((float)g3[0], (float)g3[1], (float)g3[2]);
+            if (gl == 0) {
+              const_cast<float*>(MARCH_ARG(cx))[rid] = c.C.x;
+              const_cast<float*>(MARCH_ARG(cy))[rid] = c.C.y;
+              const_cast<float*>(MARCH_ARG(cz))[rid] = c.C.z;
+              const_cast<float*>(MARCH_ARG(cu))[rid] = c.u;
+              const_cast<float*>(MARCH_ARG(cgx))[rid] = c.g.x;
+              const_cast<float*>(MARCH_ARG(cgy))[rid] = c.g.y;
+              const_cast<float*>(MARCH_ARG(cgz))[rid] = c.g.z;
+            }
+            next_ray = true;
+            phase = 0;
+          }
+        }
+      }
+    }
+    if (phase == 4 || phase == 5) {  // retire the ray: outputs at the final t, p = o + d t
       if (gl == 0) {
         const bool hit = phase == 5;
         MARCH_ARG(hit)[rid] = hit ? 1 : 0;
@@ -1544,62 +1810,21 @@ __global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0,
   }
 }
 
-// ====================================================================== device: Renewal+
-
-// GPMedium::sampleCondition (gpmedium.cpp:133-162) as a 4x4 FP64 solve of
-//   [[h(C,C), grad h(C,C)^T], [grad h(C,C), H_h(C,C)]] [u; g] =
-//   [-mu(C)/sigma - psi(C); n - grad mu(C) - grad psi(C)]
-// with psi the next realization.  At r = 0 the matrix is diag(1, -2/L^2 I),
-// which gives the reference's closed form u = -mu/sigma - psi,
-// g = (-L^2/2)(n - grad mu - grad psi).
-__device__ __forceinline__ void renewal_solve(const DevScene& S, F3 C, F3 n, uint32_t seed_next,
-                                              double& u, double g[3],
-                                              const MeanAt* mean = nullptr) {
-  F3 pg;
-  float pv;
-  psi_eval<true>(S.gp, C, seed_next, pv, pg, nullptr);
-  F3 mg;
-  float mu;
-  if (mean) {  // the same mean_eval<true>(S, C) the caller made
-    mu = mean->mu;
-    mg = mean->grad;
-  } else {
-    mu = mean_eval<true>(S, C, &mg);
-  }
-  const double L2 = S.gp.L2;
-  const double r[3] = {0.0, 0.0, 0.0};  // C - C
-  const double h = exp(-(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) / L2);
-  double A[4][5];
-  A[0][0] = h;
-  for (int j = 0; j < 3; ++j) {
-    const double gh = h * (-2.0 / L2) * r[j];
-    A[0][1 + j] = gh;
-    A[1 + j][0] = gh;
-    for (int k = 0; k < 3; ++k)
-      A[1 + j][1 + k] = h * ((j == k ? -2.0 / L2 : 0.0) + 4.0 / (L2 * L2) * r[j] * r[k]);
-  }
-  A[0][4] = -(double)mu / (double)S.gp.sigma - (double)pv;
-  A[1][4] = (double)n.x - (double)mg.x - (double)pg.x;
-  A[2][4] = (double)n.y - (double)mg.y - (double)pg.y;
-  A[3][4] = (double)n.z - (double)mg.z - (double)pg.z;
-  for (int col = 0; col < 4; ++col) {  // Gaussian elimination, partial pivoting
-    int piv = col;
-    for (int r2 = col + 1; r2 < 4; ++r2)
-      if (fabs(A[r2][col]) > fabs(A[piv][col])) piv = r2;
-    if (piv != col)
-      for (int k = 0; k < 5; ++k) {
-        const double tmp = A[col][k];
-        A[col][k] = A[piv][k];
-        A[piv][k] = tmp;
-      }
-    for (int r2 = 0; r2 < 4; ++r2) {
-      if (r2 == col) continue;
-      const double f = A[r2][col] / A[col][col];
-      for (int k = col; k < 5; ++k) A[r2][k] -= f * A[col][k];
-    }
-  }
-  u = A[0][4] / A[0][0];
-  for (int j = 0; j < 3; ++j) g[j] = A[1 + j][4] / A[1 + j][1 + j];
+// The window marcher's kernels.  MarchArgs stays the first parameter: MARCH_ARG
+// reads it at offset 0 of the kernel arguments.
+// Mean meshes and batched launches.
+template <int PPC, bool kMesh, bool kMulti, int GS = kGroup>
+__global__ __launch_bounds__(64) void k_march_window(MarchArgs a, DevScene S0, CachedConsts cc0,
+                                                     const CachedConsts* ccs) {
+  march_window<PPC, kMesh, kMulti, GS>(a, S0, cc0, ccs);
+}
+// Sphere means of a per-scene launch, which also shade their medium hits
+// (MarchArgs::fuse): 5 waves per SIMD, as the marching loop alone needs; the
+// registers of the rarely run shading code are spilled there instead.
+template <int PPC, int GS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_march_sphere(
+    MarchArgs a, DevScene S0, CachedConsts cc0, const CachedConsts* ccs) {
+  march_window<PPC, false, false, GS>(a, S0, cc0, ccs);
 }
 
 // ====================================================================== kernels: wavefront
@@ -1751,7 +1976,11 @@ __device__ __forceinline__ void shade_path(const DevScene& S0, const ShadeArgs& 
     open = false;
   };
 
-  if (medium && !S.skip_medium && P.mhit[pid]) {
+  // march result: 0 no hit, 1 hit, 2 the marcher shaded the path's medium hits
+  // and the path ended there (roulette or its last bounce)
+  const uint8_t mh = (medium && !S.skip_medium) ? P.mhit[pid] : 0;
+  if (mh == 2) alive = false;
+  if (mh == 1) {
     // handleVolume (trace.cpp:77-116)
     close_record(L);
     const uint32_t iseed = P.mseed[pid];
@@ -2500,7 +2729,7 @@ extern "C" mupsi_status mupsi_open(int32_t device, mupsi_ctx** out) {
   int per_cu = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<false>, 256, 0));
   ctx->march_blocks = std::max(1, per_cu) * ctx->cus;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_window<3, false, false, sphere_group(3)>, 64, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_sphere<3, sphere_group(3)>, 64, 0));
   ctx->march_cached_blocks = std::max(1, per_cu) * ctx->cus;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mesh_closest, 256, 0));
   // at most four per CU: five fit the LDS but ran the 4M-query closest-point
@@ -2638,6 +2867,9 @@ struct mupsi_scene {
   int use_graph = 0;
   int64_t graph_builds = 0;
   std::unique_ptr<WavefrontGraph> graph;
+  // the marcher shades GP-medium hits itself where the scene allows it
+  // (mupsi_scene_set_fused; fuse_march)
+  int fused = 1;
 };
 
 // The scene holds a triangle mesh, as a primitive or as the mean: the shading
@@ -2967,6 +3199,15 @@ extern "C" mupsi_status mupsi_scene_set_graph(mupsi_scene* sc, int32_t use_graph
   return MUPSI_OK;
 }
 
+extern "C" mupsi_status mupsi_scene_set_fused(mupsi_scene* sc, int32_t fused) {
+  if (!sc) return fail(MUPSI_ERR_INVALID, "mupsi_scene_set_fused: null scene");
+  HIP_TRY(hipSetDevice(sc->ctx->device));
+  HIP_TRY(hipStreamSynchronize(sc->ctx->stream));
+  sc->fused = fused ? 1 : 0;
+  sc->graph.reset();  // the capture holds the march arguments
+  return MUPSI_OK;
+}
+
 extern "C" mupsi_status mupsi_scene_graph_builds(mupsi_scene* sc, int64_t* builds) {
   if (!sc || !builds) return fail(MUPSI_ERR_INVALID, "mupsi_scene_graph_builds: null argument");
   *builds = sc->graph_builds;
@@ -3023,8 +3264,8 @@ static void launch_march(mupsi_ctx* ctx, const DevScene& S, const MarchArgs& ma,
       hipLaunchKernelGGL((k_march_window<P, true, false, mesh_group(P)>), grid, block, 0, st, \
                          ma, S, cc, nullptr);                                               \
     else                                                                                    \
-      hipLaunchKernelGGL((k_march_window<P, false, false, sphere_group(P)>), grid, block, 0,   \
-                         st, ma, S, cc, nullptr);                                           \
+      hipLaunchKernelGGL((k_march_sphere<P, sphere_group(P)>), grid, block, 0, st, ma, S, cc, \
+                         nullptr);                                                          \
     break;
     switch (S.gp.ppc) {
       MUPSI_CACHED(1)
@@ -3055,6 +3296,17 @@ struct GuideRun {
 // inference (gevs); the caller reads them once the stream has drained (a
 // graph capture records none).  abort_flag is polled between bounce
 // iterations (null: never, as under graph capture).
+// The window marcher shades the GP-medium hits of a render itself (handleVolume
+// and Renewal+ inside the march launch, MarchArgs::fuse) when the hit needs
+// nothing else: a sphere-mean GP medium within the window marcher's points per
+// cell, no emitter (no light sample, hence no transmittance march) and no guide.
+static bool fuse_march(const mupsi_scene* sc, bool guided) {
+  const DevScene& S = sc->ds;
+  return sc->fused && !guided && S.medium_kind == MUPSI_MEDIUM_GP && S.has_gp &&
+         !S.skip_medium && S.mean_type == MUPSI_MEAN_SPHERE && S.gp.ppc <= kMaxPpcCached &&
+         (S.n_lights == 0 || S.n_prims == 0);
+}
+
 static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64_t n,
                                       const volatile int32_t* abort_flag, EventPairs& evs,
                                       EventPairs& gevs, GuideRun* gr, bool timed = true) {
@@ -3101,6 +3353,20 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
       ma.steps = nullptr;
       ma.work = work;
       ma.stats = pb.stats;
+      if (fuse_march(sc, gr != nullptr)) {
+        ma.fuse = 1;
+        ma.rng = pb.P.rng;
+        ma.Tx = pb.P.Tx;
+        ma.Ty = pb.P.Ty;
+        ma.Tz = pb.P.Tz;
+        ma.src_W = src.W;
+        ma.src_npix = src.npix;
+        ma.src_k_begin = src.k_begin;
+        ma.src_k_begin_dev = src.k_begin_dev;
+        ma.src_xs = src.xs;
+        ma.src_ys = src.ys;
+        ma.src_ks = src.ks;
+      }
       if (timed) {
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));

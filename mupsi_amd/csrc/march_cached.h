@@ -55,6 +55,32 @@ constexpr LcgJump make_jumps() {
 }
 __constant__ constexpr LcgJump kJump = make_jumps();
 
+// State k of a cell's draw sequence from its first state s0 (k LCG steps):
+// s_k = A^k s0 + (A^{k-1} + ... + 1), as (A^k, that sum) for k < 4 * kMaxPpcCached.
+// With s0 a 32-bit value each state costs one 32 x 64-bit product, and the four
+// draws of an impulse do not wait on one another.
+struct LcgStates {
+  uint64_t a[4 * kMaxPpcCached];
+  uint64_t c[4 * kMaxPpcCached];
+};
+constexpr LcgStates make_states() {
+  LcgStates j{};
+  uint64_t a = 1, c = 0;
+  for (int k = 0; k < 4 * kMaxPpcCached; ++k) {
+    j.a[k] = a;
+    j.c[k] = c;
+    c = c * 6364136223846793005ULL + 1ULL;
+    a = a * 6364136223846793005ULL;
+  }
+  return j;
+}
+__constant__ constexpr LcgStates kStates = make_states();
+
+// The draw made from state A s0 + c of a stream whose first state s0 is 32-bit.
+__device__ __forceinline__ uint32_t pcg_draw_from(uint32_t s0, uint64_t a, uint64_t c) {
+  return pcg_output((uint64_t)s0 * a + c);
+}
+
 // New cells of the window when its centre moves by delta (each |delta_a| <= 1):
 // {d in {-1,0,1}^3 : some axis a has delta_a != 0 and d_a == delta_a}.
 // Entry [delta index][k] is the neighbour index 9(dx+1) + 3(dy+1) + (dz+1);
