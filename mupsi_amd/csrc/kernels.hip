@@ -1251,7 +1251,9 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
   Cond c;
   c.active = false;
   int nsteps = 0;
-  float lip = 0.f;  // Lipschitz bound of f along the ray within one cell's window
+  // Lipschitz bound of f along the ray within one cell's window, and its
+  // reciprocal made slightly smaller (reach below stays an underestimate)
+  float lip = 0.f, inv_lip = 0.f;
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
@@ -1341,6 +1343,7 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
         dt = fminf(0.5f, (ft - nt) / 100);
         sk = make_skip<kMesh>(scene(), c, o, d, ft);
         lip = skip_lipschitz(G, cc.psi_scale, ppc, c);
+        inv_lip = (1.0f / lip) * 0.999999f;
         nsteps = 0;
         wvalid = false;  // new realization seed
         gi0 = -1;
@@ -1431,13 +1434,12 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
     int newcells = -1, nnew = 0;
     int cx = 0, cy = 0, cz = 0;
     if (do_eval || cond_pass) {
-      cx = cell_coord(q.x, G, cc.inv_kr, cc.kr_pow2);
-      cy = cell_coord(q.y, G, cc.inv_kr, cc.kr_pow2);
-      cz = cell_coord(q.z, G, cc.inv_kr, cc.kr_pow2);
-      if (pair)
-        pair = cell_coord(q2.x, G, cc.inv_kr, cc.kr_pow2) == cx &&
-               cell_coord(q2.y, G, cc.inv_kr, cc.kr_pow2) == cy &&
-               cell_coord(q2.z, G, cc.inv_kr, cc.kr_pow2) == cz;
+      cell_coords(q, G, cc.inv_kr, cc.kr_pow2, cx, cy, cz);
+      if (pair) {
+        int px2, py2, pz2;
+        cell_coords(q2, G, cc.inv_kr, cc.kr_pow2, px2, py2, pz2);
+        pair = (px2 == cx) & (py2 == cy) & (pz2 == cz);
+      }
       if (!wvalid) {
         newcells = 27;
         nnew = 27;
@@ -1630,13 +1632,12 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
             const float margin =
                 0.01f + 1e-3f * fabsf(fl) +
                 lip * 1e-5f * (fabsf(o.x) + fabsf(o.y) + fabsf(o.z) + tl);
-            const float reach = (fabsf(fl) - margin) / lip;
+            const float reach = (fabsf(fl) - margin) * inv_lip;
             while (t < ft && t - tl < reach) {
               const F3 p = madd(d, t, o);
-              if (cell_coord(p.x, G, cc.inv_kr, cc.kr_pow2) != wx ||
-                  cell_coord(p.y, G, cc.inv_kr, cc.kr_pow2) != wy ||
-                  cell_coord(p.z, G, cc.inv_kr, cc.kr_pow2) != wz)
-                break;
+              int px2, py2, pz2;
+              cell_coords(p, G, cc.inv_kr, cc.kr_pow2, px2, py2, pz2);
+              if ((px2 != wx) | (py2 != wy) | (pz2 != wz)) break;
               ++nsteps;
               ++gi;
               if (!(t + dt > t)) {

@@ -142,6 +142,23 @@ __device__ __forceinline__ int cell_coord(float q, const DevGP& g, float inv_kr,
   return (int)(q / g.kr);
 }
 
+// cell_coord of the three coordinates of q, the power-of-two test taken once.
+__device__ __forceinline__ void cell_coords(F3 q, const DevGP& g, float inv_kr, bool pow2, int& cx,
+                                            int& cy, int& cz) {
+  const float ax = q.x * inv_kr, ay = q.y * inv_kr, az = q.z * inv_kr;
+  cx = (int)ax;
+  cy = (int)ay;
+  cz = (int)az;
+  if (!pow2) {
+    const auto near_int = [](float a) {
+      return !(fabsf(a - rintf(a)) > 4.0f * 1.1920929e-7f * fmaxf(1.0f, fabsf(a)));
+    };
+    if (near_int(ax)) cx = (int)(q.x / g.kr);
+    if (near_int(ay)) cy = (int)(q.y / g.kr);
+    if (near_int(az)) cz = (int)(q.z / g.kr);
+  }
+}
+
 // Impulse of cell (nx, ny, nz) of realization seed as (position, weight); kr is
 // the cell size, (ja, jc) the jump to the impulse's first draw (kJump.a[i],
 // kJump.c[i] for impulse i).
