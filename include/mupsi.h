@@ -225,7 +225,8 @@ mupsi_status mupsi_scene_set_graph(mupsi_scene* scene, int32_t use_graph);
 mupsi_status mupsi_scene_graph_builds(mupsi_scene* scene, int64_t* builds);
 /* No reference counterpart (where the shading of a medium hit runs): with
  * fused != 0 (the default) the GP-medium marcher of a render shades the hits of
- * a sphere-mean GP medium itself -- PathTracer::handleVolume without emitters
+ * a sphere-mean GP medium (at most 8 points per cell) itself --
+ * PathTracer::handleVolume without emitters
  * (trace.cpp:77-116) and GPMedium::sampleCondition (gpmedium.cpp:133-162) --
  * and marches the path's next ray in the same launch, when the scene has no
  * emitter and no guide is attached; other scenes are not affected.  The

@@ -2,7 +2,8 @@
 default) give the accumulation, march statistics and replayed path radiance
 that the shading kernel gives when every hit goes back to it (fused off): bit
 for bit, for renewal_plus and single_realization, group sizes 8 and 16, a
-non-power-of-two cell size, a low bounce cap, with and without the graph."""
+non-power-of-two cell size, a low bounce cap, with and without the graph.  A
+mesh mean is not fused: the switch leaves it as it is."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _scene(name, **kw):
-    sc = scenes.preset_scene(name, width=40, height=40)
+    sub = kw.pop("mesh_subdivisions", 7)
+    sc = scenes.preset_scene(name, width=40, height=40, mesh_subdivisions=sub)
     sc["kernel"].update(kw.pop("kernel", {}))
     sc.update(kw)
     return sc
@@ -24,6 +26,7 @@ CASES = [
     ("sphere_renewal_256", {"kernel": {"points_per_cell": 1, "length_scale": 1.3}}),   # 8 lanes per ray
     ("sphere_renewal_256", {"kernel": {"points_per_cell": 5, "length_scale": 0.37}}),  # 16 lanes, kr not 2^k
     ("sphere_renewal_256", {"max_bounce": 3}),
+    ("icosphere_renewal_512", {"mesh_subdivisions": 3}),                                # not fused
 ]
 
 
