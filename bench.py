@@ -36,19 +36,37 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per k_march_window<3> launch on this workload: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`, FETCH_SIZE scaled by
-# 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
-# profiles/k_march_window_sphere_renewal_256x256x64.md holds the table.
-MARCH_TRAFFIC_BYTES = 92_911_236
+# HBM bytes per k_march_sphere<3, 16> launch on this workload (the step's 9 launches averaged):
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`,
+# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
+# profiles/k_march_sphere_sphere_renewal_256x256x64.md holds the table.
+MARCH_TRAFFIC_BYTES = 135_100_000
 PRESET = "sphere_renewal_256"
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "
             "mean sphere r70 @ (0,0,-500) in a null r130 sphere, seeded 64x32 env map")
 
 
+ORACLE_FLAGS = "g++ -O3 -std=c++20 -fopenmp -ffp-contract=off -mavx2 -mfma"
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene: dict, target_s: float = 15.0) -> dict:
-    """Time the oracle's faithful restatement on a bounded, deterministic pixel sample."""
+    """Time the oracle's faithful restatement on a bounded, deterministic pixel sample.
+
+    The oracle is built with ORACLE_FLAGS (oracle/Makefile): x86-64-v3 instead of
+    the reference's -march=native, because the library is built in one place and
+    timed on another host, and -ffp-contract=off so that its float operation
+    order is the documented one."""
     from oracle import oracle as O
     osc = O.OracleScene(scene)
     threads = O.lib().orc_max_threads()
@@ -64,6 +82,7 @@ def cpu_baseline(scene: dict, target_s: float = 15.0) -> dict:
         "value": paths / secs / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
         "sample": f"pass k=0 of every {stride}th pixel ({paths} paths of the {W}x{H} image), "
                   f"{secs:.1f} s wall on {threads} OpenMP threads",
+        "cpu_model": cpu_model(), "build": ORACLE_FLAGS,
     }
 
 
@@ -285,10 +304,10 @@ def traversal_bench(ctx, sc: dict, n: int = 1 << 22) -> dict:
     return out
 
 
-def configs_bench(ctx, mesh_passes: int = 4, batch_div: int = 16) -> dict:
+def configs_bench(ctx, mesh_passes: int = 64, batch_div: int = 16) -> dict:
     """Throughput on the other workloads: sphere_single_128 (single_realization,
-    128x128x16, every pass), icosphere_renewal_512 (mesh mean, 512x512, the first
-    mesh_passes of 64 passes: per-path cost does not depend on the pass) and
+    128x128x16, every pass), icosphere_renewal_512 (mesh mean, 512x512, all 64
+    passes by default) and
     generated_batch (64 generated scenes, the first spp/batch_div passes of each,
     rendered scene by scene and as one autotuned batch; the two must give
     identical images)."""
@@ -496,7 +515,7 @@ def main() -> int:
                    else "hipGraph replay"},
         "roofline": {
             "bound": "valu",
-            "kernel": f"k_march_window<{ppc}, false, false, 16>",
+            "kernel": f"k_march_sphere<{ppc}, 16>",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

@@ -1257,10 +1257,13 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
-  // Medium hits shaded in the marcher (MarchArgs::fuse): phase 6 never persists
-  // (a hit, phase 5, is shaded at the end of its pass), phase 7 conditions the
-  // next ray at C = c.C with normal c.g (the window is regenerated there with the
-  // next realization in that pass), and next_ray marks a phase-0 group whose next
+  // Medium hits shaded in the marcher (MarchArgs::fuse): a hit (phase 5) is
+  // shaded at the end of its pass when the group's window is that of the hit
+  // point's cell (the hit was evaluated); a hit that certain-sign classification
+  // decided at a point that was not evaluated first takes a pass that moves the
+  // window there (phase 6, same realization).  Phase 7 conditions the next ray
+  // at C = c.C with normal c.g (the window is regenerated there with the next
+  // realization in that pass), and next_ray marks a phase-0 group whose next
   // ray is already in its registers.
   constexpr bool kFuse = !kMesh && !kMulti;
   const bool fuse = kFuse && MARCH_ARG(fuse) != 0;
@@ -1428,12 +1431,14 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
     }
     if (__ballot(phase != 3) == 0) break;
     const bool cond_pass = kFuse && phase == 7;
+    const bool norm_pass = kFuse && phase == 6;
     if (cond_pass) q = c.C;
+    if (norm_pass) q = madd(d, t, o);  // the hit point
 
     // Window: cells entering the 27-cell neighbourhood of the evaluation point.
     int newcells = -1, nnew = 0;
     int cx = 0, cy = 0, cz = 0;
-    if (do_eval || cond_pass) {
+    if (do_eval || cond_pass || norm_pass) {
       cell_coords(q, G, cc.inv_kr, cc.kr_pow2, cx, cy, cz);
       if (pair) {
         int px2, py2, pz2;
@@ -1659,7 +1664,13 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
       // every value is the shading kernel's.  Each lane first replaces its own
       // slots (the ones it summed) by their terms; after the barrier every lane
       // of the group adds the 27 x PPC terms in that order.
-      const bool hit_now = fuse && phase == 5;
+      bool hit_now = norm_pass;
+      if (fuse && phase == 5) {
+        int hx, hy, hz;
+        cell_coords(madd(d, t, o), G, cc.inv_kr, cc.kr_pow2, hx, hy, hz);
+        if (wvalid && hx == wx && hy == wy && hz == wz) hit_now = true;
+        else phase = 6;  // the window moves to the hit point's cell first
+      }
       if (__ballot(hit_now || cond_pass) != 0) {
         const F3 at = hit_now ? madd(d, t, o) : c.C;  // the hit point, or C
         if (hit_now || cond_pass) {

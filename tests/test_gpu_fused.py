@@ -2,8 +2,9 @@
 default) give the accumulation, march statistics and replayed path radiance
 that the shading kernel gives when every hit goes back to it (fused off): bit
 for bit, for renewal_plus and single_realization, group sizes 8 and 16, a
-non-power-of-two cell size, a low bounce cap, with and without the graph.  A
-mesh mean is not fused: the switch leaves it as it is."""
+non-power-of-two cell size, a low bounce cap, hits that certain-sign
+classification decides at points that were not evaluated, with and without
+the graph.  A mesh mean is not fused: the switch leaves it as it is."""
 import numpy as np
 import pytest
 
@@ -26,6 +27,9 @@ CASES = [
     ("sphere_renewal_256", {"kernel": {"points_per_cell": 1, "length_scale": 1.3}}),   # 8 lanes per ray
     ("sphere_renewal_256", {"kernel": {"points_per_cell": 5, "length_scale": 0.37}}),  # 16 lanes, kr not 2^k
     ("sphere_renewal_256", {"max_bounce": 3}),
+    # |psi| <= 0.1 < dt / 2: certain-sign classification decides hits at points
+    # that were not evaluated (the window moves to the hit point's cell first)
+    ("sphere_renewal_256", {"kernel": {"sigma": 0.1, "points_per_cell": 1, "length_scale": 0.2}}),
     ("icosphere_renewal_512", {"mesh_subdivisions": 3}),                                # not fused
 ]
 
