@@ -134,12 +134,13 @@ struct DevMesh {
   int n_nodes, n_faces;      // n_nodes: nodes of the reference's tree
   DevXf xf;
   float lbmin[3], lbmax[3];  // local-space bounds of the whole mesh
-  // distance to the mesh from the centres of a grid of cubes (local space), or
-  // null: dist(p, mesh) lies within dgrid[(iz * dn[1] + iy) * dn[0] + ix] +- dhalf
-  // (the half diagonal plus a rounding margin) for every p of that cube
+  // distance to the mesh from the centres c of a grid of cubes (local space), or
+  // null: dist(p, mesh) lies within dlb[(iz * dn[1] + iy) * dn[0] + ix] +- |p - c|
+  // (widened by a relative 1e-5 and dslack for rounding) for every p of that
+  // cube, the distance being 1-Lipschitz; c = dorg + (i + 1/2) dcell per axis
   const float* dlb;
   int dn[3];
-  float dorg[3], dinv, dhalf;
+  float dorg[3], dinv, dcell, dslack;
 };
 struct DevPrim {
   int type, bsdf;
@@ -775,8 +776,12 @@ __device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, 
         fz < (float)m.dn[2]) {
       const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
       const float dc = m.dlb[((size_t)iz * m.dn[1] + iy) * m.dn[0] + ix];
-      lo = fmaxf(lo, dc - m.dhalf);
-      hi = dc + m.dhalf;
+      // the cube's centre as the grid was built from it
+      const F3 c = mk(m.dorg[0] + (ix + 0.5f) * m.dcell, m.dorg[1] + (iy + 0.5f) * m.dcell,
+                      m.dorg[2] + (iz + 0.5f) * m.dcell);
+      const float r = sqrtf(sq3(sub(pl, c))) * (1.0f + 1e-5f) + m.dslack;
+      lo = fmaxf(lo, dc - r);
+      hi = dc + r;
     }
   }
 }

@@ -2912,7 +2912,9 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
   float mx = 0.0f;
   for (int a = 0; a < 3; ++a) mx = std::max(mx, m->lbmax[a] - m->lbmin[a]);
   if (m->n_nodes == 0 || !(mx > 0.0f) || !std::isfinite(mx)) return MUPSI_OK;
-  constexpr int kGrid = 64;
+  // 128 cubes along the longest axis: an evaluation needs the exact distance only
+  // where the bound straddles the surface, a band about |p - c| wide
+  constexpr int kGrid = 128;
   const float pad = 0.25f * mx;
   const float cell = (mx + 2.0f * pad) / kGrid;
   for (int a = 0; a < 3; ++a) {
@@ -2920,8 +2922,9 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
     m->dn[a] = std::max(1, std::min(kGrid, (int)std::ceil((m->lbmax[a] - m->lbmin[a] + 2.0f * pad) / cell)));
   }
   m->dinv = 1.0f / cell;
-  // half diagonal of a cube, widened for the rounding of the centre distance
-  m->dhalf = cell * 0.8660254037844386f * (1.0f + 1e-5f) + 1e-6f * (1.0f + mx);
+  m->dcell = cell;
+  // rounding margin of the centre distance and of |p - c|
+  m->dslack = 1e-6f * (1.0f + mx);
   const int n = m->dn[0] * m->dn[1] * m->dn[2];
   float* grid = nullptr;
   HIP_TRY(mem.alloc(n, &grid));
