@@ -414,9 +414,19 @@ def main() -> int:
     import torch
     from mupsi_amd import Context, Scene, scenes
 
+    def copy_probe(mb: int) -> None:
+        src = torch.empty(mb << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"bench: copy probe of {mb} MiB done", file=sys.stderr, flush=True)
+        del src, dst
+
     if args.only:
         ctx = Context(0)
         print(f"bench: {ctx.info()} section {args.only}", file=sys.stderr, flush=True)
+        if args.copy_probe_mb:
+            copy_probe(args.copy_probe_mb)
         if args.only == "guiding":
             out = guiding_bench(ctx)
         elif args.only == "configs":
@@ -435,12 +445,7 @@ def main() -> int:
     print(f"bench: {ctx.info()} scene {W}x{H} spp {spp}", file=sys.stderr, flush=True)
 
     if args.copy_probe_mb:
-        src = torch.empty(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"bench: copy probe of {args.copy_probe_mb} MiB done", file=sys.stderr, flush=True)
-        del src, dst
+        copy_probe(args.copy_probe_mb)
 
     for i in range(args.warmup):
         s.reset()
