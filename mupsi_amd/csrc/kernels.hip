@@ -543,19 +543,21 @@ __device__ __forceinline__ SkipInfo make_skip(const DevScene& S, const Cond& c, 
     if (a > 0.0) {
       const double rw = (double)ro * (1.0 + 1e-5) + slack;
       const double disc = b * b - a * (cc - rw * rw);
+      // (the widenings w dwarf the rounding of a product with 1/a)
+      const double ia = 1.0 / a;
       if (disc >= 0.0) {
         const double sq = sqrt(disc);
-        const double w = 1e-5 * (1.0 + fabs(b) + sq) / a + 1e-4;
-        k.ev0 = (float)((-b - sq) / a - w);
-        k.ev1 = (float)((-b + sq) / a + w);
+        const double w = 1e-5 * (1.0 + fabs(b) + sq) * ia + 1e-4;
+        k.ev0 = (float)((-b - sq) * ia - w);
+        k.ev1 = (float)((-b + sq) * ia + w);
       }
       if (ri > 0.0f) {
         const double rn = (double)ri * (1.0 - 1e-5) - slack;
         const double disci = b * b - a * (cc - rn * rn);
         if (rn > 0.0 && disci > 0.0) {
           const double sq = sqrt(disci);
-          const double w = 1e-5 * (1.0 + fabs(b) + sq) / a + 1e-4;
-          const double i0 = (-b - sq) / a + w, i1 = (-b + sq) / a - w;
+          const double w = 1e-5 * (1.0 + fabs(b) + sq) * ia + 1e-4;
+          const double i0 = (-b - sq) * ia + w, i1 = (-b + sq) * ia - w;
           if (i0 < i1) {
             k.in0 = (float)i0;
             k.in1 = (float)i1;
@@ -1170,6 +1172,25 @@ __device__ __forceinline__ void renewal_solve(const DevScene& S, F3 C, F3 n, uin
   renewal_solve_at(S.gp, n, mu, mg, pv, pg, u, g);
 }
 
+// A time before which every grid point o + d t lies in cell (wx, wy, wz) of size
+// kr, whatever the rounding.  Cell index w along an axis (trunc toward zero)
+// covers [w kr, (w+1) kr) for w > 0, ((w-1) kr, w kr] for w < 0 and (-kr, kr) for
+// w = 0; the ray leaves through the face it moves toward.  The exit time is less
+// a margin for the rounding of the face, of the exit time (an approximate
+// reciprocal) and of the points' coordinates (relative 1e-5, far above each).
+__device__ __forceinline__ float cell_exit_t(F3 o, F3 d, int wx, int wy, int wz, float kr) {
+  const auto axis = [kr](float oa, float da, int w) {
+    if (!(da != 0.0f)) return INFINITY;
+    const int face = da > 0.0f ? (w >= 0 ? w + 1 : w) : (w > 0 ? w : w - 1);
+    const float b = (float)face * kr;
+    const float id = __builtin_amdgcn_rcpf(da);
+    const float tb = (b - oa) * id;
+    const float m = 1e-5f * ((fabsf(b) + fabsf(oa) + kr) * fabsf(id) + fabsf(tb));
+    return tb - m;
+  };
+  return fminf(axis(o.x, d.x, wx), fminf(axis(o.y, d.y, wy), axis(o.z, d.z, wz)));
+}
+
 // A bound on |f(p) - f(q)| / |p - q| for p, q in one cell (the same 27-cell
 // window): 1 for the sphere mean's distance, psi_scale x (27 ppc impulses) x
 // max |grad h| for the window sum, and for the conditioning term u h + g . grad h
@@ -1638,7 +1659,19 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
                 0.01f + 1e-3f * fabsf(fl) +
                 lip * 1e-5f * (fabsf(o.x) + fabsf(o.y) + fabsf(o.z) + tl);
             const float reach = (fabsf(fl) - margin) * inv_lip;
-            while (t < ft && t - tl < reach) {
+            // points before the cell's exit (less a rounding margin) and within reach
+            const float tfast = fminf(fminf(ft, tl + reach), cell_exit_t(o, d, wx, wy, wz, G.kr));
+            while (t < tfast) {
+              ++nsteps;
+              const float tn = t + dt;
+              if (!(tn > t)) {
+                finish(false, o);
+                break;
+              }
+              t = tn;
+            }
+            // the rest, each checked against the cell as it is computed
+            while (phase == 2 && t < ft && t - tl < reach) {
               const F3 p = madd(d, t, o);
               int px2, py2, pz2;
               cell_coords(p, G, cc.inv_kr, cc.kr_pow2, px2, py2, pz2);
@@ -1673,6 +1706,15 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
       }
       if (__ballot(hit_now || cond_pass) != 0) {
         const F3 at = hit_now ? madd(d, t, o) : c.C;  // the hit point, or C
+        // the path state the hit's shading reads, loaded before the sums below
+        uint32_t fl = 0u;
+        Pcg rng{0ull};
+        F3 T = mk(0.0f, 0.0f, 0.0f);
+        if (hit_now) {
+          fl = MARCH_ARG(flags)[rid];
+          rng.s = MARCH_ARG(rng)[rid];
+          T = mk(MARCH_ARG(Tx)[rid], MARCH_ARG(Ty)[rid], MARCH_ARG(Tz)[rid]);
+        }
         if (hit_now || cond_pass) {
           for (int sl = gl; sl < 27 * PPC; sl += GS) {
             const float4 im = W[sl];
@@ -1713,9 +1755,6 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
             F3 gr;
             field_combine<true>(G, at, c, mu, mg, pv, pg, &gr);
             const F3 nrm = normalized(gr);
-            const uint32_t fl = MARCH_ARG(flags)[rid];
-            Pcg rng{MARCH_ARG(rng)[rid]};
-            F3 T = mk(MARCH_ARG(Tx)[rid], MARCH_ARG(Ty)[rid], MARCH_ARG(Tz)[rid]);
             F3 wi, wt;
             bsdf_sample(MUPSI_BSDF_LAMBERTIAN, mk(0.8f, 0.8f, 0.8f), neg(d), nrm, rng, wi, wt);
             T = cmul(T, wt);
