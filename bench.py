@@ -278,11 +278,16 @@ def traversal_bench(ctx, sc: dict, n: int = 1 << 22) -> dict:
     n0, t0 = batch.mesh_query_cost(s, -1, 0, pt)
     out["closest_unbounded_nodes_per_query"] = float(n0.sum().item()) / n
     out["closest_unbounded_tris_per_query"] = float(t0.sum().item()) / n
-    # inside: the reference's loop of casts (kind 1) beside the single traversal the
-    # kernel runs (kind 3), whose counts give the algorithmic bytes
+    # inside: the reference's loop of casts (kind 1) and the stackless walk over the
+    # tree (kind 4) beside the pass over the point's leaf bin the kernel runs
+    # (kind 3: a "node" is a bin entry, 32 bytes like a node), whose counts give
+    # the algorithmic bytes
     n1, t1 = batch.mesh_query_cost(s, -1, 1, pt)
     out["inside_reference_casts_nodes_per_query"] = float(n1.sum().item()) / n
     out["inside_reference_casts_tris_per_query"] = float(t1.sum().item()) / n
+    n4, t4 = batch.mesh_query_cost(s, -1, 4, pt)
+    out["inside_walk_nodes_per_query"] = float(n4.sum().item()) / n
+    out["inside_walk_tris_per_query"] = float(t4.sum().item()) / n
     for kind, name, fn in ((2, "closest", batch.mesh_closest), (3, "inside", batch.mesh_inside)):
         nodes, tris = batch.mesh_query_cost(s, -1, kind, pt)
         nb = int(nodes.sum().item())

@@ -339,8 +339,11 @@ mupsi_status mupsi_mesh_inside(mupsi_scene* scene, int32_t prim, const float* px
  * tested), Mesh::inside -> BVH::intersect (kind 1; mesh.cpp:341-362,
  * bvh.cpp:197-250, every re-cast included), or distToPoint started from the
  * distance-grid bound as mupsi_mesh_closest runs it on a mean mesh (kind 2),
- * or Mesh::inside as mupsi_mesh_inside runs it: one traversal collecting the
- * +x ray's hits, the reference's casts where that cannot decide (kind 3).  The
+ * or Mesh::inside as mupsi_mesh_inside runs it: one pass collecting the +x
+ * ray's hits over the leaves of the point's bin where the mesh carries leaf
+ * bins (mean meshes; a node count is a bin entry read), the reference's casts
+ * where that cannot decide (kind 3), or the same with the stackless walk over
+ * the tree in place of the bins (kind 4).  The
  * reference keeps no such counters; they give the traversal's algorithmic
  * bytes (32 per node, 52 per triangle: index, face and three vertices). */
 mupsi_status mupsi_mesh_query_cost(mupsi_scene* scene, int32_t prim, int32_t kind,
@@ -370,6 +373,21 @@ mupsi_status mupsi_bvh_build(const float* vertices, int32_t n_vertices, const in
 mupsi_status mupsi_bvh_layout(const float* vertices, int32_t n_vertices, const int32_t* faces,
                               int32_t n_faces, int32_t* n_records, int32_t* depth,
                               int32_t* stackless);
+
+/* The leaf bins a mean mesh carries for Mesh::inside (no reference counterpart:
+ * it reports how the library finds the leaves the +x parity ray of
+ * src/primitives/mesh.cpp:341-362 can reach): the local-space y-z box cut into
+ * ny x nz square bins of side 1/org_inv[2] (a power of two) from (org_inv[0],
+ * org_inv[1]); every leaf listed in each bin its y-z bounds reach, 8 words per
+ * entry {xmax, ymin, ymax, zmin, zmax, xmin, leaf reference bits, 0}, sorted by
+ * xmax, largest first; bin iz * ny + iy holds entries offsets[b] ..
+ * offsets[b + 1] - 1.  ny = nz = 0: no bins (a single leaf, or a tree deeper
+ * than the stackless walk allows).  entries_out / offsets_out are written when
+ * non-null and at least n_entries / ny * nz + 1 long. */
+mupsi_status mupsi_bvh_xbins(const float* vertices, int32_t n_vertices, const int32_t* faces,
+                             int32_t n_faces, int32_t* ny, int32_t* nz, float* org_inv,
+                             int64_t* n_entries, float* entries_out, int64_t entries_cap,
+                             uint32_t* offsets_out, int64_t offsets_cap);
 
 /* ---- batched scenes ----
  * Several uploaded scenes of one context traced in shared wavefront launches

@@ -149,6 +149,8 @@ SIGNATURES = {
     "mupsi_last_kernel_ms": (I32, [VP, C.POINTER(C.c_float)]),
     "mupsi_bvh_layout": (I32, [VP, I32, VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
     "mupsi_bvh_build": (I32, [VP, I32, VP, I32, VP, C.POINTER(I32), VP, C.POINTER(C.c_uint32)]),
+    "mupsi_bvh_xbins": (I32, [VP, I32, VP, I32, C.POINTER(I32), C.POINTER(I32), VP,
+                              C.POINTER(C.c_int64), VP, C.c_int64, VP, C.c_int64]),
     "mupsi_batch_create": (I32, [VP, VP, I32, C.POINTER(VP)]),
     "mupsi_batch_free": (None, [VP]),
     "mupsi_batch_set_tuning": (I32, [VP, C.POINTER(BatchTuning)]),

@@ -210,7 +210,9 @@ def mesh_inside(scene, prim: int, p):
 def mesh_query_cost(scene, prim: int, kind: int, p):
     """Per point: BVH nodes tested and triangles tested by distToPoint (kind 0),
     inside as the reference's loop of casts (kind 1), distToPoint from the
-    distance-grid bound (kind 2) or inside as the library runs it (kind 3);
+    distance-grid bound (kind 2), inside as the library runs it (kind 3: over
+    the point's leaf bin where the mesh has leaf bins, a node count being a bin
+    entry read) or inside as the stackless walk over the tree (kind 4);
     int64 tensors (n,)."""
     torch = _torch()
     n = p.shape[0]

@@ -141,6 +141,15 @@ struct DevMesh {
   const float* dlb;
   int dn[3];
   float dorg[3], dinv, dcell, dslack;
+  // the leaves binned by their y-z bounds (mean meshes whose tree allows the
+  // stackless walk), or null: bin (iy, iz) = ((y - xb_org[0]) * xb_inv,
+  // (z - xb_org[1]) * xb_inv) truncated, entries xb_off[b] .. xb_off[b + 1] - 1
+  // of xb_ent (two float4 each: xmax, ymin, ymax, zmin | zmax, xmin, leaf
+  // reference bits, 0), sorted by xmax, largest first
+  const float4* xb_ent;
+  const uint32_t* xb_off;
+  int xb_n[2];
+  float xb_org[2], xb_inv;
 };
 struct DevPrim {
   int type, bsdf;
@@ -676,12 +685,24 @@ __device__ float mesh_dist(const DevMesh& m, F3 p, uint32_t& ti, F3& out, C&& cn
 // tests the boxes the reference's casts would test first (both children of
 // every hit internal node).  Otherwise the reference's loop of casts runs as
 // written.
-// kOneTraversal false: the reference's loop of casts only (its traversal cost).
-template <bool kOneTraversal = true, bool kTop = false, class C = NoCount>
+//
+// Where the mesh carries leaf bins (xb_off), the leaves the walk would reach are
+// found without walking: a leaf's triangles are tested by the walk iff the leaf
+// passes its own slab test (its ancestors' boxes contain its box, so their
+// tests pass whenever its test does: the slab test's x range only widens and
+// its y-z conditions only loosen), and a leaf that passes holds pl.y and pl.z
+// within its y-z bounds, so it is listed in pl's bin.  The bin's entries are
+// slab-tested as the walk tests the leaf, the same triangles are tested, and
+// the hits, the conditions (a)-(c) and the parity are the walk's; entries are
+// sorted by xmax, so the scan stops at the first leaf whose tmax = xmax - pl.x
+// falls below nearT (every later one fails the slab test there as well).
+// kWalk: 0 the reference's loop of casts only (its traversal cost), 1 the
+// stackless walk, 2 the leaf bins where the mesh has them, else the walk.
+template <int kWalk = 2, bool kTop = false, class C = NoCount>
 __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
   const F3 pl = xf_to_local(m.xf, p);
   if (m.n_nodes == 0) return false;
-  if constexpr (kOneTraversal) {
+  if constexpr (kWalk > 0) {
     if (m.stackless) {
       const Ray r = make_ray(pl, mk(1, 0, 0));
       constexpr int kHits = 8;
@@ -708,8 +729,22 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
           }
         }
       };
-      cnt.node();
-      if (slab_tmin(m.lbmin, m.lbmax, r) <= r.farT) {
+      cnt.node();  // the root's box, or a bin's offsets
+      if (kWalk == 2 && m.xb_off) {
+        const float fy = (pl.y - m.xb_org[0]) * m.xb_inv, fz = (pl.z - m.xb_org[1]) * m.xb_inv;
+        // outside every bin, pl lies outside every leaf's y-z bounds: no hit
+        if (fy >= 0.0f && fz >= 0.0f && fy < (float)m.xb_n[0] && fz < (float)m.xb_n[1]) {
+          const uint32_t b = (uint32_t)fz * (uint32_t)m.xb_n[0] + (uint32_t)fy;
+          const uint32_t e1 = m.xb_off[b + 1];
+          for (uint32_t e = m.xb_off[b]; e < e1 && ok; ++e) {
+            cnt.node();
+            const float4 A = m.xb_ent[2 * e], B = m.xb_ent[2 * e + 1];
+            if (!(A.x - pl.x >= r.nearT)) break;
+            const float bmin[3] = {B.y, A.y, A.w}, bmax[3] = {A.x, A.z, B.x};
+            if (slab_tmin(bmin, bmax, r) <= r.farT) leaf(__float_as_uint(B.z), B.y, A.x);
+          }
+        }
+      } else if (slab_tmin(m.lbmin, m.lbmax, r) <= r.farT) {
         if (is_leaf(m.root_ref)) {
           leaf(m.root_ref, m.lbmin[0], m.lbmax[0]);
         } else {

@@ -180,4 +180,79 @@ bool build_records(const BvhResult& b, const std::vector<uint8_t>& order, BvhRec
   return true;
 }
 
+
+bool build_xbins(const BvhRecords& L, const float lbmin[3], const float lbmax[3], XBins& out,
+                 size_t max_entries) {
+  out = XBins{};
+  if (L.recs.empty()) return false;
+  struct Leaf {
+    float bmin[3], bmax[3];
+    uint32_t ref;
+  };
+  std::vector<Leaf> leaves;
+  leaves.reserve(L.recs.size() + 1);
+  for (const BvhRec& R : L.recs) {
+    if (R.l & kLeafFlag) leaves.push_back({{R.lmin[0], R.lmin[1], R.lmin[2]}, {R.lmax[0], R.lmax[1], R.lmax[2]}, R.l});
+    if (R.r & kLeafFlag) leaves.push_back({{R.rmin[0], R.rmin[1], R.rmin[2]}, {R.rmax[0], R.rmax[1], R.rmax[2]}, R.r});
+  }
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(lbmin[a]) || !std::isfinite(lbmax[a]) || !(lbmin[a] <= lbmax[a])) return false;
+  std::vector<float> ext;
+  ext.reserve(leaves.size());
+  for (const Leaf& f : leaves) {
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(f.bmin[a]) || !std::isfinite(f.bmax[a]) || !(f.bmin[a] <= f.bmax[a]))
+        return false;
+    ext.push_back(std::max(f.bmax[1] - f.bmin[1], f.bmax[2] - f.bmin[2]));
+  }
+  std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
+  const float med = ext[ext.size() / 2];
+  const float span = std::max(lbmax[1] - lbmin[1], lbmax[2] - lbmin[2]);
+  int e = (int)std::floor(std::log2(std::max(med, span * 1e-4f)));
+  if (!(span > 0.0f)) e = 0;
+  e = std::max(-60, std::min(60, e));
+  const float y0 = lbmin[1], z0 = lbmin[2];
+  for (; e <= 64; ++e) {
+    const float inv = std::ldexp(1.0f, -e);
+    // the device's bin coordinate: (y - y0) * inv, one rounding (inv a power of two)
+    const auto bin = [&](float v, float v0) { return (int64_t)std::floor((v - v0) * inv); };
+    const int64_t ny = bin(lbmax[1], y0) + 1, nz = bin(lbmax[2], z0) + 1;
+    if (ny < 1 || nz < 1 || (uint64_t)(ny * nz) > max_entries) continue;
+    size_t total = 0;
+    for (const Leaf& f : leaves)
+      total += (size_t)(bin(f.bmax[1], y0) - bin(f.bmin[1], y0) + 1) *
+               (size_t)(bin(f.bmax[2], z0) - bin(f.bmin[2], z0) + 1);
+    if (total > max_entries) continue;
+    std::vector<uint32_t> count((size_t)(ny * nz) + 1, 0);
+    for (const Leaf& f : leaves)
+      for (int64_t iz = bin(f.bmin[2], z0); iz <= bin(f.bmax[2], z0); ++iz)
+        for (int64_t iy = bin(f.bmin[1], y0); iy <= bin(f.bmax[1], y0); ++iy)
+          ++count[(size_t)(iz * ny + iy) + 1];
+    for (size_t b = 1; b < count.size(); ++b) count[b] += count[b - 1];
+    out.offsets = count;
+    out.entries.assign(total, XBinEntry{});
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for (const Leaf& f : leaves) {
+      const XBinEntry E{f.bmax[0], f.bmin[1], f.bmax[1], f.bmin[2], f.bmax[2], f.bmin[0], f.ref, 0u};
+      for (int64_t iz = bin(f.bmin[2], z0); iz <= bin(f.bmax[2], z0); ++iz)
+        for (int64_t iy = bin(f.bmin[1], y0); iy <= bin(f.bmax[1], y0); ++iy)
+          out.entries[fill[(size_t)(iz * ny + iy)]++] = E;
+    }
+    for (size_t b = 0; b + 1 < out.offsets.size(); ++b)
+      std::sort(out.entries.begin() + out.offsets[b], out.entries.begin() + out.offsets[b + 1],
+                [](const XBinEntry& a, const XBinEntry& c) {
+                  return a.xmax > c.xmax || (a.xmax == c.xmax && a.ref < c.ref);
+                });
+    out.ny = (int)ny;
+    out.nz = (int)nz;
+    out.y0 = y0;
+    out.z0 = z0;
+    out.inv = inv;
+    out.log2_side = e;
+    return true;
+  }
+  out = XBins{};
+  return false;
+}
+
 }  // namespace mupsi

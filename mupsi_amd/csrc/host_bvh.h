@@ -71,4 +71,35 @@ struct BvhRecords {
 // once, or a leaf slot does not fit the reference encoding.
 bool build_records(const BvhResult& b, const std::vector<uint8_t>& order, BvhRecords& out);
 
+// Leaves binned by their y-z bounds, for Mesh::inside's +x parity ray: the
+// local-space y-z box of the mesh is cut into square bins of side 2^e (a power
+// of two, so that a bin coordinate is (y - y0) * 2^-e with one rounding, in the
+// subtraction, and is monotone in y); every leaf is listed in every bin its y-z
+// bounds reach.  A point whose y, z lie inside a leaf's bounds then finds that
+// leaf in its own bin, the host and the device computing bin coordinates with
+// the same float operations.  A bin's entries are sorted by the leaf's max x,
+// largest first, so that a +x ray from x stops at the first leaf ending before x.
+struct XBinEntry {  // 32 bytes, read as two float4
+  float xmax, ymin, ymax, zmin;
+  float zmax, xmin;
+  uint32_t ref;  // the leaf's reference (kLeafFlag | count << 28 | first slot)
+  uint32_t pad;
+};
+static_assert(sizeof(XBinEntry) == 32, "XBinEntry must stay 32 bytes");
+
+struct XBins {
+  std::vector<XBinEntry> entries;
+  std::vector<uint32_t> offsets;  // ny * nz + 1: bin (iy, iz) holds [off[b], off[b + 1]), b = iz * ny + iy
+  int ny = 0, nz = 0;
+  float y0 = 0.f, z0 = 0.f, inv = 0.f;  // inv = 2^-e
+  int log2_side = 0;
+};
+
+// Bins of the leaves of L (a tree of internal nodes; false for a single leaf,
+// non-finite bounds, or more than max_entries entries at every side tried).
+// The side starts at the power of two at or below the median leaf extent in y
+// and z and doubles while the bins or entries exceed max_entries.
+bool build_xbins(const BvhRecords& L, const float lbmin[3], const float lbmax[3], XBins& out,
+                 size_t max_entries = 8u << 20);
+
 }  // namespace mupsi

@@ -2687,7 +2687,8 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
 }
 // Traversal cost per point of Mesh::distToPoint (kind 0), Mesh::inside (kind 1), or
 // Mesh::distToPoint searched from the distance grid's bound (kind 2), or inside
-// as mupsi_mesh_inside runs it (kind 3).
+// as mupsi_mesh_inside runs it (kind 3), or inside as the stackless walk over the
+// tree (kind 4; kind 3 where the mesh has no leaf bins).
 __global__ __launch_bounds__(256) void k_mesh_cost(DevMesh m, int kind, const float* px,
                                                    const float* py, const float* pz, int64_t n,
                                                    uint32_t* nodes, uint32_t* tris) {
@@ -2700,9 +2701,11 @@ __global__ __launch_bounds__(256) void k_mesh_cost(DevMesh m, int kind, const fl
       if (kind == 0) (void)mesh_dist(m, p, ti, o, c);
       else (void)mesh_dist_grid(m, p, ti, o, c);
     } else if (kind == 1) {
-      (void)mesh_inside<false>(m, p, c);  // the reference's casts
+      (void)mesh_inside<0>(m, p, c);  // the reference's casts
+    } else if (kind == 4) {
+      (void)mesh_inside<1>(m, p, c);  // the stackless walk without the leaf bins
     } else {
-      (void)mesh_inside<true>(m, p, c);   // as the library runs it
+      (void)mesh_inside<2>(m, p, c);  // as the library runs it
     }
     nodes[i] = c.nodes;
     tris[i] = c.tris;
@@ -2981,7 +2984,10 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
 // walk: every node reachable once, indices in range, and a depth at which the
 // reference's near/far traversal (at most one waiting sibling per level) can
 // never fill its 64-entry stack, so no cast of the reference drops a subtree.
-static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh* out) {
+// with_bins: also the leaf bins of inside() (mean meshes, whose inside() the
+// marcher queries at every grid point; trees that allow the stackless walk)
+static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh* out,
+                                bool with_bins = false) {
   DevMesh m{};
   BvhRecords L;
   if (!build_records(hm.bvh, hm.order, L))
@@ -3015,6 +3021,18 @@ static mupsi_status upload_mesh(DeviceBuffers& mem, const HostMesh& hm, DevMesh*
     m.lbmin[a] = hm.bvh.nodes.empty() ? 0.f : hm.bvh.nodes[hm.bvh.root].bmin[a];
     m.lbmax[a] = hm.bvh.nodes.empty() ? 0.f : hm.bvh.nodes[hm.bvh.root].bmax[a];
   }
+  XBins xb;
+  if (with_bins && m.stackless && build_xbins(L, m.lbmin, m.lbmax, xb)) {
+    const XBinEntry* ent;
+    HIP_TRY(mem.upload(xb.entries.data(), xb.entries.size(), &ent));
+    HIP_TRY(mem.upload(xb.offsets.data(), xb.offsets.size(), &m.xb_off));
+    m.xb_ent = reinterpret_cast<const float4*>(ent);
+    m.xb_n[0] = xb.ny;
+    m.xb_n[1] = xb.nz;
+    m.xb_org[0] = xb.y0;
+    m.xb_org[1] = xb.z0;
+    m.xb_inv = xb.inv;
+  }
   *out = m;
   return MUPSI_OK;
 }
@@ -3041,7 +3059,7 @@ extern "C" mupsi_status mupsi_scene_upload(mupsi_ctx* ctx, const mupsi_scene_des
   }
   sc->mean_mem = std::make_unique<DeviceBuffers>();
   if (H.has_mean_mesh) {
-    st = upload_mesh(*sc->mean_mem, H.mean_mesh, &sc->mean_mesh);
+    st = upload_mesh(*sc->mean_mem, H.mean_mesh, &sc->mean_mesh, true);
     if (st != MUPSI_OK) return st;
     st = build_dist_grid(*sc->mean_mem, &sc->mean_mesh);
     if (st != MUPSI_OK) return st;
@@ -3119,7 +3137,7 @@ extern "C" mupsi_status mupsi_scene_set_gp_medium(mupsi_scene* scene, int32_t me
   auto mem = std::make_unique<DeviceBuffers>();
   DevMesh mm{};
   if (H.has_mean_mesh) {
-    st = upload_mesh(*mem, H.mean_mesh, &mm);
+    st = upload_mesh(*mem, H.mean_mesh, &mm, true);
     if (st == MUPSI_OK) st = build_dist_grid(*mem, &mm);
     if (st != MUPSI_OK) {
       H.ds = old_ds;
@@ -4429,7 +4447,7 @@ extern "C" mupsi_status mupsi_mesh_query_cost(mupsi_scene* sc, int32_t prim, int
                                               int64_t n, uint32_t* nodes, uint32_t* tris) {
   const DevMesh* m = pick_mesh(sc, prim);
   if (!m) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: no such mesh");
-  if (kind < 0 || kind > 3) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: kind must be 0 to 3");
+  if (kind < 0 || kind > 4) return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: kind must be 0 to 4");
   if (n <= 0) return MUPSI_OK;
   if (!px || !py || !pz || !nodes || !tris)
     return fail(MUPSI_ERR_INVALID, "mupsi_mesh_query_cost: null buffer");
@@ -4455,6 +4473,41 @@ extern "C" mupsi_status mupsi_bvh_layout(const float* vertices, int32_t n_vertic
   if (n_records) *n_records = (int32_t)L.recs.size();
   if (depth) *depth = L.depth;
   if (stackless) *stackless = L.depth <= dev::kStack - 4 ? 1 : 0;
+  return MUPSI_OK;
+}
+
+extern "C" mupsi_status mupsi_bvh_xbins(const float* vertices, int32_t n_vertices,
+                                        const int32_t* faces, int32_t n_faces, int32_t* ny,
+                                        int32_t* nz, float* org_inv, int64_t* n_entries,
+                                        float* entries_out, int64_t entries_cap,
+                                        uint32_t* offsets_out, int64_t offsets_cap) {
+  if (!vertices || !faces || n_faces < 0 || n_vertices < 0 || !ny || !nz || !n_entries)
+    return fail(MUPSI_ERR_INVALID, "mupsi_bvh_xbins: bad argument");
+  for (int64_t i = 0; i < 3LL * n_faces; ++i)
+    if (faces[i] < 0 || faces[i] >= n_vertices)
+      return fail(MUPSI_ERR_INVALID, "mupsi_bvh_xbins: face index out of range");
+  const BvhResult b = build_bvh(vertices, faces, (uint32_t)n_faces);
+  BvhRecords L;
+  if (!build_records(b, bvh_order(b), L))
+    return fail(MUPSI_ERR_INVALID, "mupsi_bvh_xbins: the BVH does not fit the record layout");
+  *ny = *nz = 0;
+  *n_entries = 0;
+  if (L.depth > dev::kStack - 4 || b.nodes.empty()) return MUPSI_OK;
+  const BvhNode& root = b.nodes[b.root];
+  XBins xb;
+  if (!build_xbins(L, root.bmin, root.bmax, xb)) return MUPSI_OK;
+  *ny = xb.ny;
+  *nz = xb.nz;
+  *n_entries = (int64_t)xb.entries.size();
+  if (org_inv) {
+    org_inv[0] = xb.y0;
+    org_inv[1] = xb.z0;
+    org_inv[2] = xb.inv;
+  }
+  if (entries_out && entries_cap >= (int64_t)xb.entries.size())
+    std::memcpy(entries_out, xb.entries.data(), xb.entries.size() * sizeof(XBinEntry));
+  if (offsets_out && offsets_cap >= (int64_t)xb.offsets.size())
+    std::memcpy(offsets_out, xb.offsets.data(), xb.offsets.size() * sizeof(uint32_t));
   return MUPSI_OK;
 }
 

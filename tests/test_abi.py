@@ -127,3 +127,71 @@ def test_bvh_device_layout(which):
     assert (which == "chain") == (sl == 0)
     if which == "chain":
         assert depth > 60
+
+
+def _xbins(m):
+    lib = _abi.load()
+    v = np.ascontiguousarray(m["vertices"], np.float32)
+    f = np.ascontiguousarray(m["faces"], np.int32)
+    ny, nz, ne = C.c_int32(), C.c_int32(), C.c_int64()
+    oi = np.zeros(3, np.float32)
+    args = (v.ctypes.data_as(C.c_void_p), len(v), f.ctypes.data_as(C.c_void_p), len(f),
+            C.byref(ny), C.byref(nz), oi.ctypes.data_as(C.c_void_p), C.byref(ne))
+    assert lib.mupsi_bvh_xbins(*args, None, 0, None, 0) == 0
+    ent = np.zeros((max(ne.value, 1), 8), np.float32)
+    off = np.zeros(ny.value * nz.value + 1, np.uint32)
+    assert lib.mupsi_bvh_xbins(*args, ent.ctypes.data_as(C.c_void_p), ne.value,
+                               off.ctypes.data_as(C.c_void_p), len(off)) == 0
+    return ny.value, nz.value, oi, ent[: ne.value], off
+
+
+@pytest.mark.parametrize("sub", [2, 5])
+def test_inside_leaf_bins(sub):
+    """The leaf bins of Mesh::inside: the bin of every point (y, z) computed in
+    float32 as the kernels compute it, (y - y0) * 2^-e truncated, lists every
+    leaf whose y-z bounds hold the point, the points including each leaf's
+    corners and values one ulp either side of them; a bin's leaves are sorted by
+    max x, largest first; every entry is a leaf of the reference tree with its
+    bounds; a tree deeper than the stackless walk allows gets no bins."""
+    m = scenes.icosphere(sub, displacement=0.05)
+    nodes, _, root = O.bvh_build(m["vertices"], m["faces"])
+    ny, nz, oi, ent, off = _xbins(m)
+    assert ny > 0 and nz > 0 and off[-1] == len(ent)
+    y0, z0, inv = (np.float32(x) for x in oi)
+    assert inv > 0 and np.frexp(inv)[0] == 0.5          # a power of two
+    u = nodes.view(np.uint32)
+    leaf = (u[:, 7] & 0x80000000) != 0
+    lb = nodes[leaf, :6]
+    lref = (0x80000000 | ((u[leaf, 7] & 0x7FFFFFFF) << 28) | u[leaf, 6]).astype(np.uint32)
+    key = {(int(r), tuple(b)) for r, b in zip(lref, lb.tolist())}
+    eref = ent[:, 6].view(np.uint32)
+    ebox = np.stack([ent[:, 5], ent[:, 1], ent[:, 3], ent[:, 0], ent[:, 2], ent[:, 4]], 1)
+    assert {(int(r), tuple(b)) for r, b in zip(eref, ebox.tolist())} == key
+    for b in range(len(off) - 1):
+        xs = ent[off[b]:off[b + 1], 0]
+        assert np.all(xs[:-1] >= xs[1:])
+    rng = np.random.default_rng(sub)
+    pick = rng.choice(len(lb), min(len(lb), 400), replace=False)
+    ys, zs = [], []
+    for i in pick:
+        for yv in (lb[i, 1], lb[i, 4]):
+            for zv in (lb[i, 2], lb[i, 5]):
+                for dy in (-np.inf, 0, np.inf):
+                    for dz in (-np.inf, 0, np.inf):
+                        ys.append(np.nextafter(yv, np.float32(dy)) if dy else yv)
+                        zs.append(np.nextafter(zv, np.float32(dz)) if dz else zv)
+    lo, hi = nodes[root, :3], nodes[root, 3:6]
+    ys = np.concatenate([np.array(ys, np.float32), rng.uniform(lo[1], hi[1], 4000).astype(np.float32)])
+    zs = np.concatenate([np.array(zs, np.float32), rng.uniform(lo[2], hi[2], 4000).astype(np.float32)])
+    fy = (ys - y0) * inv
+    fz = (zs - z0) * inv
+    assert fy.dtype == np.float32 and fz.dtype == np.float32
+    for y, z, a, c in zip(ys, zs, fy, fz):
+        holds = (lb[:, 1] <= y) & (y <= lb[:, 4]) & (lb[:, 2] <= z) & (z <= lb[:, 5])
+        if not (a >= 0 and c >= 0 and a < ny and c < nz):
+            assert not holds.any()
+            continue
+        b = int(c) * ny + int(a)
+        listed = set(eref[off[b]:off[b + 1]].tolist())
+        assert set(lref[holds].tolist()) <= listed
+    assert _xbins(scenes.chain_mesh())[:2] == (0, 0)

@@ -161,3 +161,48 @@ def test_deep_tree_queries_bit_exact(ctx, dev):
     h = tri >= 0
     assert h.sum() > n // 4
     assert np.array_equal(t[h].view(np.uint32), ot[h].view(np.uint32))
+
+
+def test_inside_leaf_bins_at_leaf_bounds(ctx, dev):
+    """inside() over the leaf bins of a mean mesh at the points where a bin
+    lookup could go wrong: y and z on a leaf's bounds and one ulp either side
+    (identity transform, so the local point is the query point), x spread
+    across the mesh.  Bits against the oracle's casts; the triangles tested equal
+    the stackless walk's (kind 4) wherever neither falls back to the casts, and
+    no more bin entries are read than the walk tests boxes."""
+    import torch
+    from mupsi_amd import Scene, batch
+    m = scenes.icosphere(5, displacement=0.05)
+    sc = {"width": 8, "height": 8, "camera": dict(scenes.CAMERA), "max_bounce": 2,
+          "gp_mode": "renewal_plus",
+          "kernel": {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},
+          "gp_medium": {"mean_type": "mesh", "mesh": m},
+          "primitives": [{"type": "mesh", "mesh": m, "bsdf": "lambertian"}]}
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    nodes, _, _ = O.bvh_build(m["vertices"], m["faces"])
+    lb = nodes[(nodes.view(np.uint32)[:, 7] & 0x80000000) != 0, :6]
+    rng = np.random.default_rng(31)
+    i = rng.integers(0, len(lb), 6000)
+    y = np.where(rng.random(6000) < 0.5, lb[i, 1], lb[i, 4]).astype(np.float32)
+    z = np.where(rng.random(6000) < 0.5, lb[i, 2], lb[i, 5]).astype(np.float32)
+    step = rng.integers(-1, 2, (2, 6000))
+    y = np.where(step[0] < 0, np.nextafter(y, np.float32(-np.inf)),
+                 np.where(step[0] > 0, np.nextafter(y, np.float32(np.inf)), y))
+    z = np.where(step[1] < 0, np.nextafter(z, np.float32(-np.inf)),
+                 np.where(step[1] > 0, np.nextafter(z, np.float32(np.inf)), z))
+    x = rng.uniform(-1.2, 1.2, 6000).astype(np.float32)
+    p = np.ascontiguousarray(np.stack([x, y, z], 1).astype(np.float32))
+    pt = torch.from_numpy(p).to(dev)
+    ins = batch.mesh_inside(s, -1, pt).cpu().numpy()
+    assert np.array_equal(ins, osc.mesh_inside(-1, p))
+    assert np.array_equal(batch.mesh_inside(s, 0, pt).cpu().numpy(), ins)   # no bins on a primitive
+    n3, t3 = (a.cpu().numpy().astype(np.int64) for a in batch.mesh_query_cost(s, -1, 3, pt))
+    n4, t4 = (a.cpu().numpy().astype(np.int64) for a in batch.mesh_query_cost(s, -1, 4, pt))
+    n1, _ = (a.cpu().numpy().astype(np.int64) for a in batch.mesh_query_cost(s, -1, 1, pt))
+    single = (n4 < n1) | (t4 == 0)
+    print(f"bins: {n3.mean():.1f} entries + {t3.mean():.2f} tris per query; "
+          f"walk {n4.mean():.1f} boxes + {t4.mean():.2f} tris; {single.mean():.4f} single pass")
+    assert single.mean() > 0.9
+    assert np.array_equal(t3[single], t4[single])
+    assert n3.sum() < n4.sum()
