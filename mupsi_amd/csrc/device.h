@@ -141,6 +141,13 @@ struct DevMesh {
   const float* dlb;
   int dn[3];
   float dorg[3], dinv, dcell, dslack;
+  // a coarser grid of the same kind spanning a larger box around the first, or
+  // null: it bounds the distance at points outside the first grid (points the
+  // marcher evaluates where the noise and the conditioning can reach the mean
+  // surface's value), where the exact search would otherwise start unbounded
+  const float* olb;
+  int on[3];
+  float oorg[3], oinv, ocell, oslack;
   // the leaves binned by their y-z bounds (mean meshes whose tree allows the
   // stackless walk), or null: bin (iy, iz) = ((y - xb_org[0]) * xb_inv,
   // (z - xb_org[1]) * xb_inv) truncated, entries xb_off[b] .. xb_off[b + 1] - 1
@@ -795,8 +802,26 @@ __device__ bool mesh_inside(const DevMesh& m, F3 p, C&& cnt = C()) {
   }
   return (hits % 2) == 1;
 }
+// A distance grid's bounds at pl: the exact distance dc from the centre c of
+// pl's cube, widened by |pl - c| (distance is 1-Lipschitz) and for rounding.
+// False where pl lies outside the grid.
+__device__ __forceinline__ bool dist_grid_at(const float* g, const int* n, const float* org, float inv,
+                                             float cell, float slack, F3 pl, float& lo, float& hi) {
+  const float fx = (pl.x - org[0]) * inv, fy = (pl.y - org[1]) * inv, fz = (pl.z - org[2]) * inv;
+  if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)n[0] && fy < (float)n[1] &&
+        fz < (float)n[2]))
+    return false;
+  const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  const float dc = g[((size_t)iz * n[1] + iy) * n[0] + ix];
+  // the cube's centre as the grid was built from it
+  const F3 c = mk(org[0] + (ix + 0.5f) * cell, org[1] + (iy + 0.5f) * cell, org[2] + (iz + 0.5f) * cell);
+  const float r = sqrtf(sq3(sub(pl, c))) * (1.0f + 1e-5f) + slack;
+  lo = fmaxf(lo, dc - r);
+  hi = dc + r;
+  return true;
+}
 // Bounds of the local-space distance from pl to the mesh: lo from the mesh's
-// bounding box and the distance grid, hi from the grid (infinite outside it).
+// bounding box and the distance grids, hi from the grids (infinite outside both).
 __device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, float& lo,
                                                        float& hi) {
   const float dx = fmaxf(fmaxf(m.lbmin[0] - pl.x, 0.0f), pl.x - m.lbmax[0]);
@@ -804,21 +829,8 @@ __device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, 
   const float dz = fmaxf(fmaxf(m.lbmin[2] - pl.z, 0.0f), pl.z - m.lbmax[2]);
   lo = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
   hi = INFINITY;
-  if (m.dlb) {
-    const float fx = (pl.x - m.dorg[0]) * m.dinv, fy = (pl.y - m.dorg[1]) * m.dinv,
-                fz = (pl.z - m.dorg[2]) * m.dinv;
-    if (fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)m.dn[0] && fy < (float)m.dn[1] &&
-        fz < (float)m.dn[2]) {
-      const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-      const float dc = m.dlb[((size_t)iz * m.dn[1] + iy) * m.dn[0] + ix];
-      // the cube's centre as the grid was built from it
-      const F3 c = mk(m.dorg[0] + (ix + 0.5f) * m.dcell, m.dorg[1] + (iy + 0.5f) * m.dcell,
-                      m.dorg[2] + (iz + 0.5f) * m.dcell);
-      const float r = sqrtf(sq3(sub(pl, c))) * (1.0f + 1e-5f) + m.dslack;
-      lo = fmaxf(lo, dc - r);
-      hi = dc + r;
-    }
-  }
+  if (m.dlb && dist_grid_at(m.dlb, m.dn, m.dorg, m.dinv, m.dcell, m.dslack, pl, lo, hi)) return;
+  if (m.olb) (void)dist_grid_at(m.olb, m.on, m.oorg, m.oinv, m.ocell, m.oslack, pl, lo, hi);
 }
 // Whether the +x parity ray of Mesh::inside can reach the mesh's bounding box
 // from pl (otherwise inside() is exactly false).

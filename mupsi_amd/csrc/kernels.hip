@@ -2935,48 +2935,70 @@ static bool scene_has_tri(const mupsi_scene* sc) {
 // Distances from the centres of a grid of cubes to a mean mesh (local space): for
 // any point of a cube the distance is within the half diagonal of the centre's
 // (distance is 1-Lipschitz).
-__global__ __launch_bounds__(256) void k_dist_grid(DevMesh m0, float cell, float* out) {
+// Distance from the centre of every cube of a grid (origin org, n cubes per axis
+// of side cell) to the mesh, in local space.
+__global__ __launch_bounds__(256) void k_dist_grid(DevMesh m0, float ox, float oy, float oz, int nx,
+                                                   int ny, int nz, float cell, float* out) {
   __shared__ BvhRec top[kTopRecs];
   const DevMesh m = stage_top(m0, top);
-  const int n = m.dn[0] * m.dn[1] * m.dn[2];
+  const int n = nx * ny * nz;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int ix = i % m.dn[0], iy = (i / m.dn[0]) % m.dn[1], iz = i / (m.dn[0] * m.dn[1]);
-  const F3 c = mk(m.dorg[0] + (ix + 0.5f) * cell, m.dorg[1] + (iy + 0.5f) * cell,
-                  m.dorg[2] + (iz + 0.5f) * cell);
+  const int ix = i % nx, iy = (i / nx) % ny, iz = i / (nx * ny);
+  const F3 c = mk(ox + (ix + 0.5f) * cell, oy + (iy + 0.5f) * cell, oz + (iz + 0.5f) * cell);
   uint32_t ti = 0;
   F3 o;
   const float d = bvh_closest<true>(m, c, ti, o);
   out[i] = d;
 }
 
+// One distance grid of k cubes along the longest axis over the mesh's local box
+// padded by pad_rel times that axis's length on every side.
+static mupsi_status make_dist_grid(DeviceBuffers& mem, const DevMesh& m, int k, float pad_rel,
+                                   const float** g, int* n, float* org, float* inv, float* cellp,
+                                   float* slack) {
+  float mx = 0.0f;
+  for (int a = 0; a < 3; ++a) mx = std::max(mx, m.lbmax[a] - m.lbmin[a]);
+  const float pad = pad_rel * mx;
+  const float cell = (mx + 2.0f * pad) / k;
+  for (int a = 0; a < 3; ++a) {
+    org[a] = m.lbmin[a] - pad;
+    n[a] = std::max(1, std::min(k, (int)std::ceil((m.lbmax[a] - m.lbmin[a] + 2.0f * pad) / cell)));
+  }
+  *inv = 1.0f / cell;
+  *cellp = cell;
+  // rounding margin of the centre distance and of |p - c|: a few ulps of the
+  // largest coordinate in the grid
+  *slack = 1e-6f * (1.0f + mx + 2.0f * pad);
+  const int cnt = n[0] * n[1] * n[2];
+  float* grid = nullptr;
+  HIP_TRY(mem.alloc(cnt, &grid));
+  DevMesh mm = m;
+  mm.dlb = nullptr;
+  mm.olb = nullptr;
+  hipLaunchKernelGGL(k_dist_grid, dim3(grid_for(cnt)), dim3(256), 0, 0, mm, org[0], org[1], org[2],
+                     n[0], n[1], n[2], cell, grid);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  *g = grid;
+  return MUPSI_OK;
+}
+
 static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
   float mx = 0.0f;
   for (int a = 0; a < 3; ++a) mx = std::max(mx, m->lbmax[a] - m->lbmin[a]);
   if (m->n_nodes == 0 || !(mx > 0.0f) || !std::isfinite(mx)) return MUPSI_OK;
-  // 128 cubes along the longest axis: an evaluation needs the exact distance only
-  // where the bound straddles the surface, a band about |p - c| wide
-  constexpr int kGrid = 128;
-  const float pad = 0.25f * mx;
-  const float cell = (mx + 2.0f * pad) / kGrid;
-  for (int a = 0; a < 3; ++a) {
-    m->dorg[a] = m->lbmin[a] - pad;
-    m->dn[a] = std::max(1, std::min(kGrid, (int)std::ceil((m->lbmax[a] - m->lbmin[a] + 2.0f * pad) / cell)));
-  }
-  m->dinv = 1.0f / cell;
-  m->dcell = cell;
-  // rounding margin of the centre distance and of |p - c|
-  m->dslack = 1e-6f * (1.0f + mx);
-  const int n = m->dn[0] * m->dn[1] * m->dn[2];
-  float* grid = nullptr;
-  HIP_TRY(mem.alloc(n, &grid));
-  DevMesh mm = *m;
-  mm.dlb = nullptr;
-  hipLaunchKernelGGL(k_dist_grid, dim3(grid_for(n)), dim3(256), 0, 0, mm, cell, grid);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipDeviceSynchronize());
-  m->dlb = grid;
-  return MUPSI_OK;
+  // 128 cubes along the longest axis over the box padded by a quarter: an
+  // evaluation needs the exact distance only where the bound straddles the
+  // surface, a band about |p - c| wide
+  mupsi_status st = make_dist_grid(mem, *m, 128, 0.25f, &m->dlb, m->dn, m->dorg, &m->dinv,
+                                   &m->dcell, &m->dslack);
+  if (st != MUPSI_OK) return st;
+  // 96 cubes over the box padded by 1.5 times its length: points farther out
+  // are evaluated where the field's noise and conditioning terms can reach the
+  // distance there, and each such point would take an unbounded exact search
+  return make_dist_grid(mem, *m, 96, 1.5f, &m->olb, m->on, m->oorg, &m->oinv, &m->ocell,
+                        &m->oslack);
 }
 
 // Skip links of a left-first pre-order walk (the next node once a node's subtree
