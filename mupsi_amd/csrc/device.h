@@ -854,6 +854,27 @@ __device__ __forceinline__ F3 mesh_world_normal(const DevMesh& m, F3 n) {
 }
 
 // ------------------------------------------------------------------ mean + field
+// The mesh mean (meanfunction.cpp:22-39): the sign from Mesh::inside, the
+// distance from Mesh::distToPoint (searched from the distance grids' bound), the
+// gradient along p - closest point (the face normal on the surface).  kTop: the
+// closest-point traversal reads the top records from the kernel's LDS copy.
+template <bool kGrad, bool kTop = false>
+__device__ __forceinline__ float mesh_mean_eval(const DevMesh& M, F3 p, F3* grad) {
+  const float sg = mesh_inside(M, p) ? -1.0f : 1.0f;
+  uint32_t ti = 0;
+  F3 o;
+  const float d = mesh_dist_grid<kTop>(M, p, ti, o);
+  if (kGrad) {
+    if (d < 1e-6f) {
+      const float* fnp = M.fnormals + 3 * ti;
+      *grad = mesh_world_normal(M, mk(fnp[0], fnp[1], fnp[2]));
+    } else {
+      *grad = normalized(mul(sub(p, o), sg));
+    }
+  }
+  return sg * d;
+}
+
 // mu and grad mu (meanfunction.cpp:13-39).  For a mesh mean the reference calls
 // inside() and distToPoint() once for the value and again for the gradient;
 // both calls return the same bits, so one pair serves both here.
@@ -865,19 +886,7 @@ __device__ __forceinline__ float mean_eval(const DevScene& S, F3 p, F3* grad) {
     if (kGrad) *grad = normalized(r);
     return norm(r) - S.mean_r;
   }
-  const float sg = mesh_inside(S.mean_mesh, p) ? -1.0f : 1.0f;
-  uint32_t ti = 0;
-  F3 o;
-  const float d = mesh_dist_grid(S.mean_mesh, p, ti, o);
-  if (kGrad) {
-    if (d < 1e-6f) {
-      const float* fnp = S.mean_mesh.fnormals + 3 * ti;
-      *grad = mesh_world_normal(S.mean_mesh, mk(fnp[0], fnp[1], fnp[2]));
-    } else {
-      *grad = normalized(mul(sub(p, o), sg));
-    }
-  }
-  return sg * d;
+  return mesh_mean_eval<kGrad>(S.mean_mesh, p, grad);
 }
 
 // The mean and its gradient at one point, for callers that need them twice.

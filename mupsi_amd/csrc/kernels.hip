@@ -1964,6 +1964,9 @@ struct ShadeArgs {
   int* nMm2;
   int* qMs2;               // and those of mesh-mean scenes with ppc <= 8 to queue Ms
   int* nMs2;
+  // single-scene launches of a mesh mean: the mean and its gradient at each
+  // medium hit (mu, grad), computed by k_hit_mean after the march; else null
+  const float4* hmean;
 };
 
 // The guided tracer keeps T as the plain tracer's throughput (albedo weights and
@@ -2040,7 +2043,13 @@ __device__ __forceinline__ void shade_path(const DevScene& S0, const ShadeArgs& 
     MeanAt mp{};
     F3 gr;
     if (S.medium_kind == MUPSI_MEDIUM_GP) {
-      mp.mu = mean_eval<true, kTri>(S, Pp, &mp.grad);
+      if (!kMulti && a.hmean) {  // mean_eval<true>(S, Pp), computed by k_hit_mean
+        const float4 h = a.hmean[pid];
+        mp.mu = h.x;
+        mp.grad = mk(h.y, h.z, h.w);
+      } else {
+        mp.mu = mean_eval<true, kTri>(S, Pp, &mp.grad);
+      }
       field_eval<true, kTri>(S, Pp, iseed, cur, &gr, &mp);
     } else {
       gr = alt_grad<kTri>(S, Pp, iseed);
@@ -2685,6 +2694,27 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
                                                      const float* pz, int64_t n, uint8_t* in) {
   MESH_QUERY_LOOP(i, n) in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
 }
+// The mesh mean and its gradient at every medium hit of a bounce iteration (the
+// paths of the march's queue with march result 1), for the shading kernel: the
+// closest-point search at a hit point is a chain of dependent node loads, which
+// inside the shading kernel ran on its few hit lanes at that kernel's occupancy
+// and without the LDS copy of the tree's top.  Same function, same bits.
+__global__ __launch_bounds__(256) void k_hit_mean(DevScene S, const int* qM, const int* nM,
+                                                  const uint8_t* mhit, const float* px,
+                                                  const float* py, const float* pz, float4* out) {
+  __shared__ BvhRec top[kTopRecs];
+  const int n = *nM;
+  if (n <= 0) return;  // the whole grid: no hit to serve
+  const DevMesh m = stage_top(S.mean_mesh, top);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int pid = qM[i];
+    if (mhit[pid] != 1) continue;
+    F3 g;
+    const float mu = mesh_mean_eval<true, true>(m, mk(px[pid], py[pid], pz[pid]), &g);
+    out[pid] = make_float4(mu, g.x, g.y, g.z);
+  }
+}
+
 // Traversal cost per point of Mesh::distToPoint (kind 0), Mesh::inside (kind 1), or
 // Mesh::distToPoint searched from the distance grid's bound (kind 2), or inside
 // as mupsi_mesh_inside runs it (kind 3), or inside as the stackless walk over the
@@ -2847,6 +2877,7 @@ struct PathBuffers {
   int* qS = nullptr;
   int* qMm[2]{};  // batched launches: medium paths for the lane-per-ray marcher
   int* qMs[2]{};  // batched launches: window-marcher paths of mesh-mean scenes
+  float4* hmean = nullptr;  // k_hit_mean's output, one per path
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -3244,6 +3275,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   }
   HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
+  if (!mesh_queue) HIP_TRY(m.alloc(n, &pb->hmean));
   pb->capacity = n;
   out = std::move(pb);
   return MUPSI_OK;
@@ -3497,8 +3529,19 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
     const bool defer = pb.qS != nullptr;
     int* nS = pb.counters + 7;
     if (defer) hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nS, 1, 0, -1);
+    // a mesh mean's hits: their mean and gradient in a kernel of their own
+    const bool hit_mean = it > 0 && S.medium_kind == MUPSI_MEDIUM_GP && S.has_gp &&
+                          !S.skip_medium && S.mean_type == MUPSI_MEAN_MESH &&
+                          S.mean_mesh.n_nodes > 0 && pb.hmean;
+    if (hit_mean) {
+      hipLaunchKernelGGL(k_hit_mean, dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
+                         dim3(256), 0, st, S, pb.qM[cur], nM, pb.P.mhit, pb.P.mpx, pb.P.mpy,
+                         pb.P.mpz, pb.hmean);
+      HIP_TRY(hipGetLastError());
+    }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
-                 defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr, nullptr, nullptr};
+                 defer ? 1 : 0, pb.SH, pb.qS, nS, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 hit_mean ? pb.hmean : nullptr};
     if (scene_has_tri(sc)) hipLaunchKernelGGL((k_shade<false, true>), dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     else hipLaunchKernelGGL((k_shade<false, false>), dim3(grid_for(n)), dim3(256), 0, st, S, sa);
     HIP_TRY(hipGetLastError());
