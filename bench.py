@@ -7,7 +7,7 @@ Workload (one step): the sphere-mean GPIS scene in renewal_plus mode at
 every pixel, i.e. 4,194,304 paths, rendered into the FP64 accumulator.
 
 Reported next to it:
-  roofline      the GP-medium march kernel (k_march_window<ppc>): algorithmic FP32
+  roofline      the GP-medium march kernel (k_march_sphere<ppc, 16>): algorithmic FP32
                 flops per launch (field evaluations x (27*ppc*16 + 9)) over its
                 HIP-event time, against the MI355X FP32 vector peak; its HBM traffic
                 per launch is the rocprofv3 FETCH_SIZE + WRITE_SIZE figure measured for
@@ -485,8 +485,12 @@ def main() -> int:
     paths = W * H * spp * args.steps
     ms_per_step = 1e3 * secs / args.steps
     ppc = scene["kernel"]["points_per_cell"]
+    # a field evaluation: psi's value (16 flops per impulse) and the sphere mean (9);
+    # the value-and-gradient sums of the medium hits the kernel shades (about one
+    # per 140 evaluations on this workload) are not counted
     flops_per_eval = 27 * ppc * 16 + 9
-    achieved = evals_t * flops_per_eval / (march_ms * 1e-3) / 1e12 if march_ms > 0 else 0.0
+    flops_t = evals_t * flops_per_eval
+    achieved = flops_t / (march_ms * 1e-3) / 1e12 if march_ms > 0 else 0.0
     img = s.image()
     print(f"bench: {args.steps} steps in {secs:.3f} s; march kernel {march_ms:.1f} ms over "
           f"{launches} launches; {evals} field evals (reference grid points {steps_ref}); "
@@ -531,7 +535,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": MARCH_TRAFFIC_BYTES,
-            "flops_per_launch": evals_t * flops_per_eval / max(1, launches),
+            "flops_per_launch": flops_t / max(1, launches),
+            "field_evals_per_step": evals_t / min(args.steps, 3),
             "avg_launch_ms": march_ms / max(1, launches),
         },
         "cpu_baseline": cpu,

@@ -1964,8 +1964,8 @@ struct ShadeArgs {
   int* nMm2;
   int* qMs2;               // and those of mesh-mean scenes with ppc <= 8 to queue Ms
   int* nMs2;
-  // single-scene launches of a mesh mean: the mean and its gradient at each
-  // medium hit (mu, grad), computed by k_hit_mean after the march; else null
+  // the mean and its gradient (mu, grad) at each medium hit of a mesh-mean
+  // scene, computed by k_hit_mean after the march; else null
   const float4* hmean;
 };
 
@@ -2043,7 +2043,8 @@ __device__ __forceinline__ void shade_path(const DevScene& S0, const ShadeArgs& 
     MeanAt mp{};
     F3 gr;
     if (S.medium_kind == MUPSI_MEDIUM_GP) {
-      if (!kMulti && a.hmean) {  // mean_eval<true>(S, Pp), computed by k_hit_mean
+      if (a.hmean && S.mean_type == MUPSI_MEAN_MESH && S.mean_mesh.n_nodes > 0) {
+        // mean_eval<true>(S, Pp), computed by k_hit_mean
         const float4 h = a.hmean[pid];
         mp.mu = h.x;
         mp.grad = mk(h.y, h.z, h.w);
@@ -2695,22 +2696,39 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
   MESH_QUERY_LOOP(i, n) in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
 }
 // The mesh mean and its gradient at every medium hit of a bounce iteration (the
-// paths of the march's queue with march result 1), for the shading kernel: the
+// paths of a march queue with march result 1), for the shading kernel: the
 // closest-point search at a hit point is a chain of dependent node loads, which
 // inside the shading kernel ran on its few hit lanes at that kernel's occupancy
 // and without the LDS copy of the tree's top.  Same function, same bits.
-__global__ __launch_bounds__(256) void k_hit_mean(DevScene S, const int* qM, const int* nM,
-                                                  const uint8_t* mhit, const float* px,
-                                                  const float* py, const float* pz, float4* out) {
-  __shared__ BvhRec top[kTopRecs];
+// kMulti: a batched launch, each path's scene from its id (no LDS copy: the
+// paths' meshes differ); paths of scenes without a GP medium with a mesh mean
+// are passed over, as the shading kernel reads no mean for them.
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_hit_mean(DevScene S0, const DevScene* scenes,
+                                                  const uint16_t* sid, const int* qM,
+                                                  const int* nM, const uint8_t* mhit,
+                                                  const float* px, const float* py,
+                                                  const float* pz, float4* out) {
+  __shared__ BvhRec top[kMulti ? 1 : kTopRecs];
   const int n = *nM;
   if (n <= 0) return;  // the whole grid: no hit to serve
-  const DevMesh m = stage_top(S.mean_mesh, top);
+  DevMesh m;
+  if constexpr (!kMulti) m = stage_top(S0.mean_mesh, top);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int pid = qM[i];
     if (mhit[pid] != 1) continue;
     F3 g;
-    const float mu = mesh_mean_eval<true, true>(m, mk(px[pid], py[pid], pz[pid]), &g);
+    float mu;
+    const F3 p = mk(px[pid], py[pid], pz[pid]);
+    if constexpr (kMulti) {
+      const DevScene& S = scenes[sid[pid]];
+      if (!(S.medium_kind == MUPSI_MEDIUM_GP && S.mean_type == MUPSI_MEAN_MESH &&
+            S.mean_mesh.n_nodes > 0))
+        continue;
+      mu = mesh_mean_eval<true, false>(S.mean_mesh, p, &g);
+    } else {
+      mu = mesh_mean_eval<true, true>(m, p, &g);
+    }
     out[pid] = make_float4(mu, g.x, g.y, g.z);
   }
 }
@@ -3275,7 +3293,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   }
   HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
-  if (!mesh_queue) HIP_TRY(m.alloc(n, &pb->hmean));
+  HIP_TRY(m.alloc(n, &pb->hmean));
   pb->capacity = n;
   out = std::move(pb);
   return MUPSI_OK;
@@ -3534,9 +3552,10 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
                           !S.skip_medium && S.mean_type == MUPSI_MEAN_MESH &&
                           S.mean_mesh.n_nodes > 0 && pb.hmean;
     if (hit_mean) {
-      hipLaunchKernelGGL(k_hit_mean, dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
-                         dim3(256), 0, st, S, pb.qM[cur], nM, pb.P.mhit, pb.P.mpx, pb.P.mpy,
-                         pb.P.mpz, pb.hmean);
+      hipLaunchKernelGGL(k_hit_mean<false>,
+                         dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
+                         dim3(256), 0, st, S, nullptr, nullptr, pb.qM[cur], nM, pb.P.mhit,
+                         pb.P.mpx, pb.P.mpy, pb.P.mpz, pb.hmean);
       HIP_TRY(hipGetLastError());
     }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
@@ -3834,6 +3853,7 @@ struct mupsi_batch {
   int max_bounce = 1;
   bool any_cached = false, any_plain = false;
   bool any_cached_mesh = false;  // window-marcher scenes with a mean mesh
+  bool any_mesh_mean = false;    // a GP medium with a mean mesh (k_hit_mean serves its hits)
   bool any_tri = false;          // a scene holds a triangle mesh (primitive or mean)
   mupsi_batch_tuning tuning{};
   hipGraph_t graph = nullptr;
@@ -3924,9 +3944,23 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nMm2, 1, 0, -1);
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, nMs2, 1, 0, -1);
     GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
+    // the mesh-mean hits of this iteration's marches (window marcher of mesh means,
+    // lane-per-ray marcher): their mean and gradient in a kernel of their own
+    const bool hit_mean = it > 0 && b->any_mesh_mean && pb.hmean;
+    if (hit_mean) {
+      const dim3 hgrid(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks)));
+      if (b->any_cached_mesh)
+        hipLaunchKernelGGL(k_hit_mean<true>, hgrid, dim3(256), 0, st, DevScene{}, b->d_scenes,
+                           pb.P.sid, pb.qMs[cur], pb.counters + 12 + cur, pb.P.mhit, pb.P.mpx,
+                           pb.P.mpy, pb.P.mpz, pb.hmean);
+      if (b->any_plain)
+        hipLaunchKernelGGL(k_hit_mean<true>, hgrid, dim3(256), 0, st, DevScene{}, b->d_scenes,
+                           pb.P.sid, pb.qMm[cur], pb.counters + 8 + cur, pb.P.mhit, pb.P.mpx,
+                           pb.P.mpy, pb.P.mpz, pb.hmean);
+    }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], pb.counters + 2 * nxt + 1,
                  gs, 0, ShadowSoA{}, nullptr, nullptr, b->d_scenes,
-                 pb.qMm[nxt], nMm2, pb.qMs[nxt], nMs2};
+                 pb.qMm[nxt], nMm2, pb.qMs[nxt], nMs2, hit_mean ? pb.hmean : nullptr};
     if (b->any_tri) hipLaunchKernelGGL((k_shade<true, true>), dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
     else hipLaunchKernelGGL((k_shade<true, false>), dim3(grid_for(n)), dim3(256), 0, st, DevScene{}, sa);
   }
@@ -4059,6 +4093,8 @@ extern "C" mupsi_status mupsi_batch_create(mupsi_ctx* ctx, mupsi_scene* const* s
       if (!scene_uses_cached(sc->ds)) b->any_plain = true;
       else if (sc->ds.mean_type == MUPSI_MEAN_MESH) b->any_cached_mesh = true;
       else b->any_cached = true;
+      if (sc->ds.medium_kind == MUPSI_MEDIUM_GP && sc->ds.mean_type == MUPSI_MEAN_MESH)
+        b->any_mesh_mean = true;
     }
   }
   const DevScene* dds;
