@@ -495,14 +495,14 @@ __global__ void k_encode(EncodeBox b, const float* p, const float* nrm, const fl
 }
 
 // Host: one gradient evaluation (k_mlp_train) over `batch` samples into grad_part.
-int train_blocks(int batch) { return std::max(1, std::min(batch / 64, kMaxTrainBlocks)); }
+int train_block_count(int batch) { return std::max(1, std::min(batch / 64, kMaxTrainBlocks)); }
 
 void launch_train(const mupsi_guide* g, const float* feats, const float* dirs, const float* w,
                   const int* n_dev, int n_host, int batch, int random, uint32_t seed,
                   hipStream_t st) {
   TrainArgs ta{feats, dirs, w, n_dev, n_host, batch, random, seed, g->grad_part, g->loss_part,
                g->cfg.alpha};
-  hipLaunchKernelGGL(k_mlp_train, dim3(train_blocks(batch)), dim3(256), 0, st, g->net(), ta);
+  hipLaunchKernelGGL(k_mlp_train, dim3(train_block_count(batch)), dim3(256), 0, st, g->net(), ta);
 }
 
 void launch_adam(mupsi_guide* g, int batch, hipStream_t st) {
@@ -511,7 +511,7 @@ void launch_adam(mupsi_guide* g, int batch, hipStream_t st) {
   const float bc2 = (float)(1.0 - std::pow(0.999, (double)g->step));
   hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->theta, g->adam_m,
                      g->adam_v, reinterpret_cast<uint16_t*>(g->wf), reinterpret_cast<uint16_t*>(g->wt),
-                     g->grad_part, train_blocks(batch), 1.0f / (float)batch, g->cfg.learning_rate,
+                     g->grad_part, train_block_count(batch), 1.0f / (float)batch, g->cfg.learning_rate,
                      bc1, bc2);
 }
 
@@ -541,6 +541,8 @@ std::vector<float> init_params(uint64_t seed) {
 
 namespace mupsi {
 namespace guide {
+
+int train_blocks(int batch) { return train_block_count(batch); }
 
 void launch_infer(const mupsi_guide* g, const float* feats, const int* count_dev, int count_host,
                   const int* q_pid, float* raw_out, float* mix_out, hipStream_t st) {
@@ -677,13 +679,13 @@ extern "C" mupsi_status mupsi_guide_gradient(mupsi_guide* g, const float* feats,
   hipStream_t st = g->ctx->stream;
   launch_train(g, feats, dirs, weights, nullptr, (int)n, batch, 0, 0u, st);
   hipLaunchKernelGGL(k_grad_reduce, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->grad_part,
-                     train_blocks(batch), 1.0f / (float)n, g->grad_out);
+                     train_block_count(batch), 1.0f / (float)n, g->grad_out);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   if (grad_host)
     HIP_TRY(hipMemcpy(grad_host, g->grad_out, kNumParams * sizeof(float), hipMemcpyDeviceToHost));
   if (loss_host) {
-    std::vector<float> lp(train_blocks(batch));
+    std::vector<float> lp(train_block_count(batch));
     HIP_TRY(hipMemcpy(lp.data(), g->loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost));
     double s = 0;
     for (float v : lp) s += v;
@@ -706,7 +708,7 @@ extern "C" mupsi_status mupsi_guide_train_step(mupsi_guide* g, const float* feat
   const float bc2 = (float)(1.0 - std::pow(0.999, (double)g->step));
   hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)kNumParams * kRedLanes)), dim3(256), 0, st, g->theta, g->adam_m,
                      g->adam_v, reinterpret_cast<uint16_t*>(g->wf), reinterpret_cast<uint16_t*>(g->wt),
-                     g->grad_part, train_blocks(batch), 1.0f / (float)n, g->cfg.learning_rate, bc1, bc2);
+                     g->grad_part, train_block_count(batch), 1.0f / (float)n, g->cfg.learning_rate, bc1, bc2);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   g->stats.train_steps += 1;

@@ -48,5 +48,9 @@ void launch_infer(const mupsi_guide* g, const float* feats, const int* count_dev
 mupsi_status train_random(mupsi_guide* g, const float* feats, const float* dirs, const float* w,
                           int n, int steps, int batch, hipStream_t st);
 
+// Blocks of a training step on `batch` samples: one per 64 samples, at most the
+// number of per-block partials the guide keeps (each block writes one loss partial).
+int train_blocks(int batch);
+
 }  // namespace guide
 }  // namespace mupsi

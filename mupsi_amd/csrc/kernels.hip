@@ -3745,9 +3745,9 @@ extern "C" mupsi_status mupsi_render_passes(mupsi_scene* sc, int32_t k_begin, in
       guide->stats.train_samples += count;
       guide->stats.train_passes_done += kc;
       if (gb.tr_size > 0 && steps > 0) {
-        float lp[256];
-        const int nb = std::max(1, std::min(batch / 64, 256));
-        HIP_TRY(hipMemcpy(lp, guide->loss_part, nb * sizeof(float), hipMemcpyDeviceToHost));
+        const int nb = guide::train_blocks(batch);
+        std::vector<float> lp((size_t)nb);
+        HIP_TRY(hipMemcpy(lp.data(), guide->loss_part, nb * sizeof(float), hipMemcpyDeviceToHost));
         double ls = 0;
         for (int i = 0; i < nb; ++i) ls += lp[i];
         guide->stats.last_loss = ls / batch;
