@@ -208,8 +208,9 @@ bool build_xbins(const BvhRecords& L, const float lbmin[3], const float lbmax[3]
   std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
   const float med = ext[ext.size() / 2];
   const float span = std::max(lbmax[1] - lbmin[1], lbmax[2] - lbmin[2]);
-  int e = (int)std::floor(std::log2(std::max(med, span * 1e-4f)));
-  if (!(span > 0.0f)) e = 0;
+  if (!std::isfinite(span)) return false;  // bin coordinates would overflow
+  const float base = std::max(med, span * 1e-4f);
+  int e = base > 0.0f ? (int)std::floor(std::log2(base)) : 0;
   e = std::max(-60, std::min(60, e));
   const float y0 = lbmin[1], z0 = lbmin[2];
   for (; e <= 64; ++e) {

@@ -195,3 +195,35 @@ def test_inside_leaf_bins(sub):
         listed = set(eref[off[b]:off[b + 1]].tolist())
         assert set(lref[holds].tolist()) <= listed
     assert _xbins(scenes.chain_mesh())[:2] == (0, 0)
+
+
+def test_inside_leaf_bins_flat_mesh():
+    """A mesh flat in y (a 12 x 12 grid of quads in the x-z plane): one row of
+    bins, every leaf listed in the bins its z bounds reach, and a point on the
+    plane finds every leaf holding it in its bin."""
+    n = 12
+    xs, zs = np.meshgrid(np.linspace(-1, 1, n + 1), np.linspace(-1, 1, n + 1), indexing="ij")
+    v = np.stack([xs.ravel(), np.zeros(xs.size), zs.ravel()], 1).astype(np.float32)
+    f = []
+    for i in range(n):
+        for j in range(n):
+            a, b, c, d = i * (n + 1) + j, (i + 1) * (n + 1) + j, (i + 1) * (n + 1) + j + 1, i * (n + 1) + j + 1
+            f += [[a, b, c], [a, c, d]]
+    m = {"vertices": v, "faces": np.array(f, np.int32)}
+    ny, nz, oi, ent, off = _xbins(m)
+    assert ny == 1 and nz > 1 and off[-1] == len(ent) > 0
+    y0, z0, inv = (np.float32(x) for x in oi)
+    nodes, _, _ = O.bvh_build(m["vertices"], m["faces"])
+    u = nodes.view(np.uint32)
+    leaf = (u[:, 7] & 0x80000000) != 0
+    lb = nodes[leaf, :6]
+    lref = (0x80000000 | ((u[leaf, 7] & 0x7FFFFFFF) << 28) | u[leaf, 6]).astype(np.uint32)
+    eref = ent[:, 6].view(np.uint32)
+    rng = np.random.default_rng(5)
+    for z in np.concatenate([lb[:, 2], lb[:, 5], rng.uniform(-1, 1, 200)]).astype(np.float32):
+        y = np.float32(0.0)
+        c = (z - z0) * inv
+        assert 0 <= c < nz
+        b = int(c) * ny + int((y - y0) * inv)
+        holds = (lb[:, 1] <= y) & (y <= lb[:, 4]) & (lb[:, 2] <= z) & (z <= lb[:, 5])
+        assert set(lref[holds].tolist()) <= set(eref[off[b]:off[b + 1]].tolist())
