@@ -148,6 +148,12 @@ struct DevMesh {
   const float* olb;
   int on[3];
   float oorg[3], oinv, ocell, oslack;
+  // a third, coarsest grid of the same kind around the second, or null: the far
+  // points that conditioning grown over many Renewal+ updates can still bring
+  // near zero
+  const float* flb;
+  int fn[3];
+  float forg[3], finv, fcell, fslack;
   // the leaves binned by their y-z bounds (mean meshes whose tree allows the
   // stackless walk), or null: bin (iy, iz) = ((y - xb_org[0]) * xb_inv,
   // (z - xb_org[1]) * xb_inv) truncated, entries xb_off[b] .. xb_off[b + 1] - 1
@@ -830,7 +836,8 @@ __device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, 
   lo = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
   hi = INFINITY;
   if (m.dlb && dist_grid_at(m.dlb, m.dn, m.dorg, m.dinv, m.dcell, m.dslack, pl, lo, hi)) return;
-  if (m.olb) (void)dist_grid_at(m.olb, m.on, m.oorg, m.oinv, m.ocell, m.oslack, pl, lo, hi);
+  if (m.olb && dist_grid_at(m.olb, m.on, m.oorg, m.oinv, m.ocell, m.oslack, pl, lo, hi)) return;
+  if (m.flb) (void)dist_grid_at(m.flb, m.fn, m.forg, m.finv, m.fcell, m.fslack, pl, lo, hi);
 }
 // Whether the +x parity ray of Mesh::inside can reach the mesh's bounding box
 // from pl (otherwise inside() is exactly false).

@@ -3025,6 +3025,7 @@ static mupsi_status make_dist_grid(DeviceBuffers& mem, const DevMesh& m, int k, 
   DevMesh mm = m;
   mm.dlb = nullptr;
   mm.olb = nullptr;
+  mm.flb = nullptr;
   hipLaunchKernelGGL(k_dist_grid, dim3(grid_for(cnt)), dim3(256), 0, 0, mm, org[0], org[1], org[2],
                      n[0], n[1], n[2], cell, grid);
   HIP_TRY(hipGetLastError());
@@ -3046,8 +3047,14 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
   // 96 cubes over the box padded by 1.5 times its length: points farther out
   // are evaluated where the field's noise and conditioning terms can reach the
   // distance there, and each such point would take an unbounded exact search
-  return make_dist_grid(mem, *m, 96, 1.5f, &m->olb, m->on, m->oorg, &m->oinv, &m->ocell,
-                        &m->oslack);
+  st = make_dist_grid(mem, *m, 96, 1.5f, &m->olb, m->on, m->oorg, &m->oinv, &m->ocell,
+                      &m->oslack);
+  if (st != MUPSI_OK) return st;
+  // 64 cubes over the box padded by 8 times its length: Renewal+ multiplies the
+  // conditioning by about 1/sigma at every medium hit, so after many bounces of
+  // a low-sigma medium it reaches the mean's value far from the mesh
+  return make_dist_grid(mem, *m, 64, 8.0f, &m->flb, m->fn, m->forg, &m->finv, &m->fcell,
+                        &m->fslack);
 }
 
 // Skip links of a left-first pre-order walk (the next node once a node's subtree
