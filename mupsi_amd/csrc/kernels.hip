@@ -1277,6 +1277,10 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
   float lip = 0.f, inv_lip = 0.f;
   int wx = 0, wy = 0, wz = 0;
   bool wvalid = false;
+  // mesh means whose cells span several grid steps: |psi| <= wbound at every
+  // point of the window's cell (from the window's impulses; INFINITY: unknown),
+  // and |conditioning term| <= condb everywhere on the ray
+  float wbound = INFINITY, condb = 0.0f;
   uint32_t evals = 0, refsteps = 0, rays = 0;  // per lane; a launch stays far below 2^32
   // Medium hits shaded in the marcher (MarchArgs::fuse): a hit (phase 5) is
   // shaded at the end of its pass when the group's window is that of the hit
@@ -1368,8 +1372,10 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
         sk = make_skip<kMesh>(scene(), c, o, d, ft);
         lip = skip_lipschitz(G, cc.psi_scale, ppc, c);
         inv_lip = (1.0f / lip) * 0.999999f;
+        if constexpr (kMesh) condb = cond_bound(G, c);
         nsteps = 0;
         wvalid = false;  // new realization seed
+        wbound = INFINITY;
         gi0 = -1;
         if (gl == 0) ++rays;
         phase = 1;
@@ -1581,6 +1587,40 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
       wvalid = true;
     }
     __syncthreads();
+    if constexpr (kMesh) {
+      // Mesh means with cells of more than two grid steps: a bound of |psi| over
+      // every point of the regenerated window's cell.  psi there sums exactly
+      // these 27 x ppc impulses (weights +-1), each term at most exp(-dist^2 / L^2)
+      // for the impulse's distance to the cell's box (widened for the rounding of
+      // the cell test, the exp2 and the sum).
+      const bool wb_now = newcells >= 0 && G.kr > 2.0f * dt;
+      if (__ballot(wb_now) != 0) {
+        float part = 0.0f;
+        if (wb_now) {
+          float blo[3], bhi[3];
+          const int wc[3] = {wx, wy, wz};
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax) {
+            const int nc = wc[ax];
+            const float lo_c = (float)(nc > 0 ? nc : nc - 1) * G.kr;
+            const float hi_c = (float)(nc >= 0 ? nc + 1 : nc) * G.kr;
+            const float wd = 1e-5f * (fabsf(lo_c) + fabsf(hi_c) + G.kr);
+            blo[ax] = lo_c - wd;
+            bhi[ax] = hi_c + wd;
+          }
+          for (int sl = gl; sl < 27 * ppc; sl += GS) {
+            const float4 im = W[sl];
+            const float ex = fmaxf(fmaxf(blo[0] - im.x, im.x - bhi[0]), 0.0f);
+            const float ey = fmaxf(fmaxf(blo[1] - im.y, im.y - bhi[1]), 0.0f);
+            const float ez = fmaxf(fmaxf(blo[2] - im.z, im.z - bhi[2]), 0.0f);
+            const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+            part += __builtin_amdgcn_exp2f(d2 * 0.9999f * cc.psi_k);
+          }
+        }
+        const float tot = group_sum<GS>(part);
+        if (wb_now) wbound = tot * fabsf(cc.psi_scale) * 1.001f + 1e-6f;
+      }
+    }
 
     // Window sums at q (and q2): each lane takes the same slots in the same order
     // for either point, so a point's sum does not depend on whether it was paired.
@@ -1683,6 +1723,59 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
                 break;
               }
               t += dt;
+            }
+          }
+          if constexpr (kMesh) {
+            // Mesh means: the grid points after the last evaluated one up to the
+            // window cell's exit (less a rounding margin) are certain positive,
+            // the origin's sign, when that segment lies outside the +x parity slab
+            // (inside() is exactly false there, so the mean is the distance), the
+            // ray moves away from the mesh's box (the distance to a box is convex
+            // along a line, so it only grows) and the distance at the segment's
+            // start exceeds the cell's psi bound plus the conditioning bound with
+            // cached_field's margin: stepped with the march's own f32 additions.
+            const DevMesh& M = scene().mean_mesh;
+            if (phase == 2 && sk.kind == 2 && sp && wbound < INFINITY && M.xf.s > 0.0f) {
+              const float te = fminf(ft, cell_exit_t(o, d, wx, wy, wz, G.kr));
+              if (t < te) {
+                const F3 pa = xf_to_local(M.xf, madd(d, t, o));
+                const F3 pb = xf_to_local(M.xf, madd(d, te, o));
+                const float ea = 1e-4f * (1.0f + fmaxf(fabsf(pa.x) + fabsf(pa.y) + fabsf(pa.z),
+                                                       fabsf(pb.x) + fabsf(pb.y) + fabsf(pb.z)));
+                const bool out_slab =
+                    (fminf(pa.x, pb.x) > M.lbmax[0] + ea) || (fminf(pa.y, pb.y) > M.lbmax[1] + ea) ||
+                    (fmaxf(pa.y, pb.y) < M.lbmin[1] - ea) || (fminf(pa.z, pb.z) > M.lbmax[2] + ea) ||
+                    (fmaxf(pa.z, pb.z) < M.lbmin[2] - ea);
+                const float pv[3] = {pa.x, pa.y, pa.z}, dv[3] = {d.x, d.y, d.z};
+                float e2 = 0.0f, grow = 0.0f;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                  const float over = pv[ax] - M.lbmax[ax], under = M.lbmin[ax] - pv[ax];
+                  if (over > 0.0f) {
+                    e2 = fmaf(over, over, e2);
+                    grow = fmaf(over, dv[ax], grow);
+                  } else if (under > 0.0f) {
+                    e2 = fmaf(under, under, e2);
+                    grow = fmaf(-under, dv[ax], grow);
+                  }
+                }
+                const float low = sqrtf(e2) * M.xf.s * (1.0f - 1e-5f) - ea * M.xf.s;
+                const float eb = wbound + condb;
+                if (out_slab && grow >= 0.0f && low > eb + 1e-3f + 1e-5f * (eb + low)) {
+                  int k = 0;
+                  while (t < te) {
+                    ++k;
+                    const float tn = t + dt;
+                    if (!(tn > t)) {
+                      finish(false, o);
+                      break;
+                    }
+                    t = tn;
+                  }
+                  nsteps += k;
+                  gi += k;
+                }
+              }
             }
           }
         }
