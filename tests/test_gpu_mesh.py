@@ -206,3 +206,38 @@ def test_inside_leaf_bins_at_leaf_bounds(ctx, dev):
     assert single.mean() > 0.9
     assert np.array_equal(t3[single], t4[single])
     assert n3.sum() < n4.sum()
+
+
+@pytest.mark.parametrize("scale", [2.0, 0.35])
+def test_closest_across_distance_grids(ctx, dev, scale):
+    """Closest-point queries of a mean mesh at points in each of its distance
+    grids (the fine grid around the box, the outer grid to 1.5 box lengths, the
+    far grid to 8) and beyond all three: the search starts from the grid's upper
+    bound there, so a bound below the true distance would change the result.
+    Distance bits, triangle ids and closest points against the oracle's unbounded
+    search."""
+    import torch
+    from mupsi_amd import Scene, batch
+    m = scenes.icosphere(4, displacement=0.05)
+    c = np.array([0.3, -0.2, -5.0])
+    xf = {"scale": scale, "position": list(c)}
+    sc = {"width": 8, "height": 8, "camera": dict(scenes.CAMERA), "max_bounce": 2,
+          "gp_mode": "renewal_plus",
+          "kernel": {"sigma": 0.3, "length_scale": 0.5, "aniso": [1, 1, 1], "points_per_cell": 4},
+          "gp_medium": {"mean_type": "mesh", "mesh": m, "transform": xf}}
+    s = Scene(ctx, sc)
+    osc = O.OracleScene(sc)
+    rng = np.random.default_rng(41)
+    ext = 2.1 * scale  # the box's length (world)
+    pts = []
+    for lo, hi in ((0.9, 1.3), (1.4, 2.2), (2.5, 4.5), (6.0, 16.0), (20.0, 40.0)):
+        dirs = rng.normal(size=(3000, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        pts.append(c + dirs * (rng.uniform(lo, hi, (3000, 1)) * ext / 2))
+    p = np.concatenate(pts).astype(np.float32)
+    pt = torch.from_numpy(p).to(dev)
+    d, tri, cl = (x.cpu().numpy() for x in batch.mesh_closest(s, -1, pt))
+    od, otri, ocl = osc.mesh_closest(-1, p)
+    assert np.array_equal(tri, otri)
+    assert np.array_equal(d.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(cl.view(np.uint32), ocl.view(np.uint32))
