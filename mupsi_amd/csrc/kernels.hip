@@ -3131,10 +3131,12 @@ static mupsi_status build_dist_grid(DeviceBuffers& mem, DevMesh* m) {
   float mx = 0.0f;
   for (int a = 0; a < 3; ++a) mx = std::max(mx, m->lbmax[a] - m->lbmin[a]);
   if (m->n_nodes == 0 || !(mx > 0.0f) || !std::isfinite(mx)) return MUPSI_OK;
-  // 128 cubes along the longest axis over the box padded by a quarter: an
+  // 128 cubes along the longest axis over the box padded by a quarter (256 for
+  // meshes of 65,536 triangles or more, whose exact searches are the longest): an
   // evaluation needs the exact distance only where the bound straddles the
   // surface, a band about |p - c| wide
-  mupsi_status st = make_dist_grid(mem, *m, 128, 0.25f, &m->dlb, m->dn, m->dorg, &m->dinv,
+  const int fine = m->n_faces >= 65536 ? 256 : 128;
+  mupsi_status st = make_dist_grid(mem, *m, fine, 0.25f, &m->dlb, m->dn, m->dorg, &m->dinv,
                                    &m->dcell, &m->dslack);
   if (st != MUPSI_OK) return st;
   // 96 cubes over the box padded by 1.5 times its length: points farther out
