@@ -545,6 +545,11 @@ def main() -> int:
             "traffic": MARCH_TRAFFIC_BYTES,
             "flops_per_launch": flops_t / max(1, launches),
             "field_evals_per_step": evals_t / min(args.steps, 3),
+            # the reference's t-grid points (every one an evaluation in the reference's
+            # march); the kernel evaluates the uncertain ones and decides the rest in
+            # closed form, so the flops above count only the evaluated points
+            "reference_grid_points_per_step": steps_ref / args.steps,
+            "evaluated_fraction": (evals / steps_ref) if steps_ref else None,
             "avg_launch_ms": march_ms / max(1, launches),
         },
         "cpu_baseline": cpu,
