@@ -2781,9 +2781,11 @@ __global__ __launch_bounds__(256) void k_mesh_closest(DevMesh m0, const float* p
     cl[3 * i + 2] = o.z;
   }
 }
-// inside() reads no LDS copy: its walk tests ~50 boxes per query, too few to pay
-// for the copy and the resident blocks it costs (measured: 5.2 ms with the copy,
-// 3.9 ms without, 4M queries on the 327,680-triangle mesh).
+// inside() reads no LDS copy of the tree's top: a mean mesh's query reads its
+// leaf bin (about 7 entries on the 327,680-triangle mesh), and a primitive mesh's
+// stackless walk tests ~50 boxes, too few to pay for the copy and the resident
+// blocks it costs (measured for the walk: 5.2 ms with the copy, 3.9 ms without,
+// 4M queries on the 327,680-triangle mesh).
 __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px, const float* py,
                                                      const float* pz, int64_t n, uint8_t* in) {
   MESH_QUERY_LOOP(i, n) in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
