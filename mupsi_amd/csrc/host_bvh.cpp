@@ -208,7 +208,8 @@ bool build_xbins(const BvhRecords& L, const float lbmin[3], const float lbmax[3]
   std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
   const float med = ext[ext.size() / 2];
   const float span = std::max(lbmax[1] - lbmin[1], lbmax[2] - lbmin[2]);
-  if (!std::isfinite(span)) return false;  // bin coordinates would overflow
+  // bin coordinates and counts must stay far inside int64 (the walk serves the rest)
+  if (!std::isfinite(span) || span > 1e15f) return false;
   const float base = std::max(med, span * 1e-4f);
   int e = base > 0.0f ? (int)std::floor(std::log2(base)) : 0;
   e = std::max(-60, std::min(60, e));
@@ -218,7 +219,7 @@ bool build_xbins(const BvhRecords& L, const float lbmin[3], const float lbmax[3]
     // the device's bin coordinate: (y - y0) * inv, one rounding (inv a power of two)
     const auto bin = [&](float v, float v0) { return (int64_t)std::floor((v - v0) * inv); };
     const int64_t ny = bin(lbmax[1], y0) + 1, nz = bin(lbmax[2], z0) + 1;
-    if (ny < 1 || nz < 1 || (uint64_t)(ny * nz) > max_entries) continue;
+    if (ny < 1 || nz < 1 || (double)ny * (double)nz > (double)max_entries) continue;
     size_t total = 0;
     for (const Leaf& f : leaves)
       total += (size_t)(bin(f.bmax[1], y0) - bin(f.bmin[1], y0) + 1) *
