@@ -2828,6 +2828,62 @@ __global__ __launch_bounds__(256) void k_hit_mean(DevScene S0, const DevScene* s
   }
 }
 
+// The hits of a march queue binned by the distance grids' upper bound at the hit
+// point (local units, relative to the mesh's box length mx): a closest-point
+// search takes longer the farther its point lies from the surface, so searches
+// of similar length share a wave when k_hit_mean_binned takes the bins in turn.
+constexpr int kHitBins = 4;
+__global__ __launch_bounds__(256) void k_hit_bin(DevMesh m, const int* qM, const int* nM,
+                                                 const uint8_t* mhit, const float* px,
+                                                 const float* py, const float* pz, int* bins,
+                                                 int cap, int* counts) {
+  const int n = *nM;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (__ballot(i < n) == 0) return;  // whole waves past the queue
+  int pid = -1, b = -1;
+  if (i < n) {
+    pid = qM[i];
+    if (mhit[pid] == 1) {
+      float lo, hi;
+      mesh_dist_bounds_local(m, xf_to_local(m.xf, mk(px[pid], py[pid], pz[pid])), lo, hi);
+      float mx = fmaxf(m.lbmax[0] - m.lbmin[0], fmaxf(m.lbmax[1] - m.lbmin[1], m.lbmax[2] - m.lbmin[2]));
+      const float r = hi / mx;
+      b = r < 0.02f ? 0 : (r < 0.06f ? 1 : (r < 0.15f ? 2 : 3));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kHitBins; ++k) wave_push(b == k, pid, bins + (size_t)k * cap, counts + k);
+}
+// k_hit_mean over the bins of k_hit_bin, bin after bin (entry j of the
+// concatenation), with the top of the tree in LDS.
+__global__ __launch_bounds__(256) void k_hit_mean_binned(DevMesh m0, const int* bins, int cap,
+                                                         const int* counts, const float* px,
+                                                         const float* py, const float* pz,
+                                                         float4* out) {
+  __shared__ BvhRec top[kTopRecs];
+  int c[kHitBins], total = 0;
+#pragma unroll
+  for (int k = 0; k < kHitBins; ++k) {
+    c[k] = counts[k];
+    total += c[k];
+  }
+  if (total <= 0) return;  // the whole grid: no hit to serve
+  const DevMesh m = stage_top(m0, top);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+    int b = 0, jj = j;
+#pragma unroll
+    for (int k = 0; k < kHitBins - 1; ++k)
+      if (b == k && jj >= c[k]) {
+        jj -= c[k];
+        b = k + 1;
+      }
+    const int pid = bins[(size_t)b * cap + jj];
+    F3 g;
+    const float mu = mesh_mean_eval<true, true>(m, mk(px[pid], py[pid], pz[pid]), &g);
+    out[pid] = make_float4(mu, g.x, g.y, g.z);
+  }
+}
+
 // Traversal cost per point of Mesh::distToPoint (kind 0), Mesh::inside (kind 1), or
 // Mesh::distToPoint searched from the distance grid's bound (kind 2), or inside
 // as mupsi_mesh_inside runs it (kind 3), or inside as the stackless walk over the
@@ -2991,6 +3047,7 @@ struct PathBuffers {
   int* qMm[2]{};  // batched launches: medium paths for the lane-per-ray marcher
   int* qMs[2]{};  // batched launches: window-marcher paths of mesh-mean scenes
   float4* hmean = nullptr;  // k_hit_mean's output, one per path
+  int* hbins = nullptr;     // single-scene launches: kHitBins queues of capacity paths
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -3398,6 +3455,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
   HIP_TRY(m.alloc(n, &pb->hmean));
+  if (!mesh_queue) HIP_TRY(m.alloc((size_t)kHitBins * n, &pb->hbins));
   pb->capacity = n;
   out = std::move(pb);
   return MUPSI_OK;
@@ -3654,11 +3712,15 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
     // a mesh mean's hits: their mean and gradient in a kernel of their own
     const bool hit_mean = it > 0 && S.medium_kind == MUPSI_MEDIUM_GP && S.has_gp &&
                           !S.skip_medium && S.mean_type == MUPSI_MEAN_MESH &&
-                          S.mean_mesh.n_nodes > 0 && pb.hmean;
+                          S.mean_mesh.n_nodes > 0 && pb.hmean && pb.hbins;
     if (hit_mean) {
-      hipLaunchKernelGGL(k_hit_mean<false>,
+      int* hcount = pb.counters + 8;  // kHitBins counters (free in single-scene launches)
+      hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, hcount, kHitBins, 0, -1);
+      hipLaunchKernelGGL(k_hit_bin, dim3(grid_for(n)), dim3(256), 0, st, S.mean_mesh, pb.qM[cur], nM,
+                         pb.P.mhit, pb.P.mpx, pb.P.mpy, pb.P.mpz, pb.hbins, (int)pb.capacity, hcount);
+      hipLaunchKernelGGL(k_hit_mean_binned,
                          dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
-                         dim3(256), 0, st, S, nullptr, nullptr, pb.qM[cur], nM, pb.P.mhit,
+                         dim3(256), 0, st, S.mean_mesh, pb.hbins, (int)pb.capacity, hcount,
                          pb.P.mpx, pb.P.mpy, pb.P.mpz, pb.hmean);
       HIP_TRY(hipGetLastError());
     }
