@@ -3418,8 +3418,10 @@ extern "C" mupsi_status mupsi_scene_size(mupsi_scene* scene, int32_t* width, int
   return MUPSI_OK;
 }
 
+// hit_means: buffers of the mesh-mean hit kernels (the means, and for
+// single-scene launches, mesh_queue false, their bins)
 static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_t n, bool shadow,
-                                       bool mesh_queue) {
+                                       bool mesh_queue, bool hit_means) {
   out.reset();
   auto pb = std::make_unique<PathBuffers>();
   DeviceBuffers& m = pb->mem;
@@ -3454,17 +3456,21 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   }
   HIP_TRY(m.alloc(16, &pb->counters));
   HIP_TRY(m.alloc(8, &pb->stats));
-  HIP_TRY(m.alloc(n, &pb->hmean));
-  if (!mesh_queue) HIP_TRY(m.alloc((size_t)kHitBins * n, &pb->hbins));
+  if (hit_means) {
+    HIP_TRY(m.alloc(n, &pb->hmean));
+    if (!mesh_queue) HIP_TRY(m.alloc((size_t)kHitBins * n, &pb->hbins));
+  }
   pb->capacity = n;
   out = std::move(pb);
   return MUPSI_OK;
 }
 
 static mupsi_status ensure_paths(mupsi_scene* sc, int64_t n) {
-  if (sc->pb && sc->pb->capacity >= n) return MUPSI_OK;
+  // a mesh mean (possibly set after the buffers were made) needs the hit buffers
+  const bool hit_means = sc->ds.mean_type == MUPSI_MEAN_MESH;
+  if (sc->pb && sc->pb->capacity >= n && (!hit_means || sc->pb->hbins)) return MUPSI_OK;
   sc->graph.reset();  // it holds pointers into the old buffers
-  return alloc_path_buffers(sc->pb, n, sc->ds.n_lights > 0 && sc->ds.has_gp, false);
+  return alloc_path_buffers(sc->pb, n, sc->ds.n_lights > 0 && sc->ds.has_gp, false, hit_means);
 }
 
 static mupsi_status ensure_guide_buffers(mupsi_scene* sc, int64_t n, int64_t rec_slots,
@@ -4148,7 +4154,7 @@ static mupsi_status batch_buffers(mupsi_batch* b, int64_t cap) {
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
-  return alloc_path_buffers(b->pb, cap, false, b->any_plain || b->any_cached_mesh);
+  return alloc_path_buffers(b->pb, cap, false, b->any_plain || b->any_cached_mesh, b->any_mesh_mean);
 }
 
 // Capture the launch's kernels into a graph for the current buffers and tuning.
