@@ -827,7 +827,8 @@ __device__ __forceinline__ bool dist_grid_at(const float* g, const int* n, const
   return true;
 }
 // Bounds of the local-space distance from pl to the mesh: lo from the mesh's
-// bounding box and the distance grids, hi from the grids (infinite outside both).
+// bounding box and the distance grids, hi from the finest grid that holds pl
+// (infinite outside all three).
 __device__ __forceinline__ void mesh_dist_bounds_local(const DevMesh& m, F3 pl, float& lo,
                                                        float& hi) {
   const float dx = fmaxf(fmaxf(m.lbmin[0] - pl.x, 0.0f), pl.x - m.lbmax[0]);
