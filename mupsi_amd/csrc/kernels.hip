@@ -1558,7 +1558,7 @@ __device__ __forceinline__ void march_window(const MarchArgs& a, const DevScene&
         const int nitem = min(64, ncell - c0) * PPC;
         for (int m = 0; m < nitem; m += 64) {  // every lane takes part in the permutes
           const int j = m + lane;
-          const int c = j / PPC;
+          const int c = div_small<PPC>(j);  // j < 64 PPC: nitem <= 64 PPC
           const int src = min(c, 63) * 4;
           const uint32_t cs0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)s0);
           const float ex = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(bx)));

@@ -126,6 +126,28 @@ constexpr bool new_cell_counts_match() {
 }
 static_assert(new_cell_counts_match(), "entered-cell count formula");
 
+// floor(j / D) for 0 <= j < 64 D as one 24-bit multiply and a shift (full-rate
+// VALU, where a general division takes a quarter-rate 32-bit multiply-high):
+// M = ceil(2^s / D) for the smallest s exact over that whole range, found and
+// checked at compile time.
+struct DivSmall {
+  uint32_t m, s;
+};
+constexpr DivSmall make_div_small(int d) {
+  for (uint32_t s = 1; s < 24; ++s) {
+    const uint32_t m = ((1u << s) + (uint32_t)d - 1u) / (uint32_t)d;
+    bool ok = (uint64_t)(64 * d - 1) * m < (1ull << 24);
+    for (int j = 0; ok && j < 64 * d; ++j) ok = ((uint32_t)j * m) >> s == (uint32_t)(j / d);
+    if (ok) return DivSmall{m, s};
+  }
+  return DivSmall{0, 0};
+}
+template <int D>
+__device__ __forceinline__ int div_small(int j) {
+  constexpr DivSmall k = make_div_small(D);
+  static_assert(k.m != 0, "no exact multiply-shift for this divisor");
+  return (int)(__umul24((uint32_t)j, k.m) >> k.s);
+}
 __device__ __forceinline__ int mod3(int n) {
   const int r = n % 3;
   return r < 0 ? r + 3 : r;
