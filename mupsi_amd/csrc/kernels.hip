@@ -2828,56 +2828,144 @@ __global__ __launch_bounds__(256) void k_hit_mean(DevScene S0, const DevScene* s
   }
 }
 
-// The hits of a march queue binned by the distance grids' upper bound at the hit
-// point (local units, relative to the mesh's box length mx): a closest-point
-// search takes longer the farther its point lies from the surface, so searches
-// of similar length share a wave when k_hit_mean_binned takes the bins in turn.
-constexpr int kHitBins = 4;
-__global__ __launch_bounds__(256) void k_hit_bin(DevMesh m, const int* qM, const int* nM,
+// The hits of a march queue put in order of a key before their closest-point
+// searches: the distance class (the distance grids' upper bound at the hit
+// point relative to the mesh's box length mx: a search takes longer the farther
+// its point lies from the surface) and then the Morton index of the point's cell
+// in a 32^3 grid over the mesh's box padded by mx/4 (points outside are clamped
+// to its border cells).  A wave's searches then have similar lengths and start
+// near each other, so its lanes read the same nodes.  Counting sort: k_hit_key
+// counts each key (one atomic per key present in a wave, which also gives each
+// lane its rank among the key's hits), k_hit_scan turns the counts into starts,
+// k_hit_place writes each path id at start + rank.  The order within a key
+// varies from run to run; every hit's result is its own, so the bits do not.
+constexpr int kHitClasses = 4;
+constexpr int kHitCellBits = 5;  // 32 cells per axis
+constexpr int kHitKeys = kHitClasses << (3 * kHitCellBits);
+__device__ __forceinline__ uint32_t spread3_5(uint32_t v) {  // bits 0-4 to bits 0,3,..,12
+  v = (v | (v << 8)) & 0x0300Fu;
+  v = (v | (v << 4)) & 0x030C3u;
+  v = (v | (v << 2)) & 0x09249u;
+  return v;
+}
+__device__ __forceinline__ int hit_key(const DevMesh& m, F3 p) {
+  const F3 pl = xf_to_local(m.xf, p);
+  float lo, hi;
+  mesh_dist_bounds_local(m, pl, lo, hi);
+  const float mx = fmaxf(m.lbmax[0] - m.lbmin[0], fmaxf(m.lbmax[1] - m.lbmin[1], m.lbmax[2] - m.lbmin[2]));
+  const float r = hi / mx;
+  const int b = r < 0.02f ? 0 : (r < 0.06f ? 1 : (r < 0.15f ? 2 : 3));
+  const float pad = 0.25f * mx;
+  const float q[3] = {pl.x, pl.y, pl.z};
+  uint32_t morton = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lo_k = m.lbmin[k] - pad, ext = m.lbmax[k] - m.lbmin[k] + 2.0f * pad;
+    const float u = (q[k] - lo_k) / fmaxf(ext, 1e-30f) * (float)(1 << kHitCellBits);
+    // NaN and far points land in range: fminf/fmaxf return the number operand
+    const int c = (int)fminf(fmaxf(u, 0.0f), (float)((1 << kHitCellBits) - 1));
+    morton |= spread3_5((uint32_t)c) << k;
+  }
+  return (b << (3 * kHitCellBits)) | (int)morton;
+}
+// Layout of the sort's buffer for n paths: keys, ranks and order at a stride of
+// n rounded up to 64 ints (the counts stay 16-byte aligned for the scan's
+// vector loads), then counts, starts and the total.
+inline size_t hit_sort_stride(size_t n) { return (n + 63) & ~(size_t)63; }
+inline size_t hit_sort_ints(size_t n) { return 3 * hit_sort_stride(n) + 2 * kHitKeys + 64; }
+__global__ __launch_bounds__(256) void k_hit_zero(int* hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kHitKeys) hist[i] = 0;
+}
+__global__ __launch_bounds__(256) void k_hit_key(DevMesh m, const int* qM, const int* nM,
                                                  const uint8_t* mhit, const float* px,
-                                                 const float* py, const float* pz, int* bins,
-                                                 int cap, int* counts) {
+                                                 const float* py, const float* pz, int* keys,
+                                                 int* ranks, int* hist) {
   const int n = *nM;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (__ballot(i < n) == 0) return;  // whole waves past the queue
-  int pid = -1, b = -1;
+  int key = -1;
   if (i < n) {
-    pid = qM[i];
-    if (mhit[pid] == 1) {
-      float lo, hi;
-      mesh_dist_bounds_local(m, xf_to_local(m.xf, mk(px[pid], py[pid], pz[pid])), lo, hi);
-      float mx = fmaxf(m.lbmax[0] - m.lbmin[0], fmaxf(m.lbmax[1] - m.lbmin[1], m.lbmax[2] - m.lbmin[2]));
-      const float r = hi / mx;
-      b = r < 0.02f ? 0 : (r < 0.06f ? 1 : (r < 0.15f ? 2 : 3));
-    }
+    const int pid = qM[i];
+    if (mhit[pid] == 1) key = hit_key(m, mk(px[pid], py[pid], pz[pid]));
   }
-#pragma unroll
-  for (int k = 0; k < kHitBins; ++k) wave_push(b == k, pid, bins + (size_t)k * cap, counts + k);
+  // one atomic per distinct key of the wave: the leader of each key adds the
+  // key's lane count and hands the old value to its lanes
+  const int lane = __lane_id();
+  uint64_t todo = __ballot(key >= 0);
+  int rank = -1;
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int lk = __shfl(key, leader);
+    const uint64_t mm = __ballot(key == lk);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(hist + lk, __popcll(mm));
+    base = __shfl(base, leader);
+    if (key == lk) rank = base + __popcll(mm & ((1ull << lane) - 1ull));
+    todo &= ~mm;
+  }
+  if (i < n) {
+    keys[i] = key;
+    ranks[i] = rank;
+  }
 }
-// k_hit_mean over the bins of k_hit_bin, bin after bin (entry j of the
-// concatenation), with the top of the tree in LDS.
-__global__ __launch_bounds__(256) void k_hit_mean_binned(DevMesh m0, const int* bins, int cap,
-                                                         const int* counts, const float* px,
+// One block: hist (kHitKeys counts) -> start (exclusive prefix sums); the total
+// goes to *total.
+constexpr int kHitScanThreads = 1024;
+__global__ __launch_bounds__(kHitScanThreads) void k_hit_scan(const int* hist, int* start,
+                                                              int* total) {
+  constexpr int kPer = kHitKeys / kHitScanThreads;
+  static_assert(kPer % 4 == 0 && kHitKeys % kHitScanThreads == 0, "scan layout");
+  __shared__ int part[kHitScanThreads];
+  const int t = threadIdx.x;
+  const int4* h4 = reinterpret_cast<const int4*>(hist + t * kPer);
+  int sum = 0;
+  for (int k = 0; k < kPer / 4; ++k) {
+    const int4 v = h4[k];
+    sum += v.x + v.y + v.z + v.w;
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < kHitScanThreads; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+  int4* s4 = reinterpret_cast<int4*>(start + t * kPer);
+  for (int k = 0; k < kPer / 4; ++k) {
+    const int4 v = h4[k];
+    int4 o;
+    o.x = run;
+    o.y = o.x + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    run = o.w + v.w;
+    s4[k] = o;
+  }
+  if (t == kHitScanThreads - 1) *total = part[t];
+}
+__global__ __launch_bounds__(256) void k_hit_place(const int* qM, const int* nM, const int* keys,
+                                                   const int* ranks, const int* start,
+                                                   int* order) {
+  const int n = *nM;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int key = keys[i];
+  if (key >= 0) order[start[key] + ranks[i]] = qM[i];
+}
+// k_hit_mean over the hits in key order, with the top of the tree in LDS.
+__global__ __launch_bounds__(256) void k_hit_mean_sorted(DevMesh m0, const int* order,
+                                                         const int* total, const float* px,
                                                          const float* py, const float* pz,
                                                          float4* out) {
   __shared__ BvhRec top[kTopRecs];
-  int c[kHitBins], total = 0;
-#pragma unroll
-  for (int k = 0; k < kHitBins; ++k) {
-    c[k] = counts[k];
-    total += c[k];
-  }
-  if (total <= 0) return;  // the whole grid: no hit to serve
+  const int n = *total;
+  if (n <= 0) return;  // the whole grid: no hit to serve
   const DevMesh m = stage_top(m0, top);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
-    int b = 0, jj = j;
-#pragma unroll
-    for (int k = 0; k < kHitBins - 1; ++k)
-      if (b == k && jj >= c[k]) {
-        jj -= c[k];
-        b = k + 1;
-      }
-    const int pid = bins[(size_t)b * cap + jj];
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const int pid = order[j];
     F3 g;
     const float mu = mesh_mean_eval<true, true>(m, mk(px[pid], py[pid], pz[pid]), &g);
     out[pid] = make_float4(mu, g.x, g.y, g.z);
@@ -3047,7 +3135,8 @@ struct PathBuffers {
   int* qMm[2]{};  // batched launches: medium paths for the lane-per-ray marcher
   int* qMs[2]{};  // batched launches: window-marcher paths of mesh-mean scenes
   float4* hmean = nullptr;  // k_hit_mean's output, one per path
-  int* hbins = nullptr;     // single-scene launches: kHitBins queues of capacity paths
+  int* hbins = nullptr;     // single-scene launches: keys, ranks and order of the hits
+                            // (capacity each), then the key counts, starts and total
 };
 
 // Buffers of guided rendering: per-path mixture and features of the current
@@ -3458,7 +3547,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   HIP_TRY(m.alloc(8, &pb->stats));
   if (hit_means) {
     HIP_TRY(m.alloc(n, &pb->hmean));
-    if (!mesh_queue) HIP_TRY(m.alloc((size_t)kHitBins * n, &pb->hbins));
+    if (!mesh_queue) HIP_TRY(m.alloc(hit_sort_ints(n), &pb->hbins));
   }
   pb->capacity = n;
   out = std::move(pb);
@@ -3720,14 +3809,23 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
                           !S.skip_medium && S.mean_type == MUPSI_MEAN_MESH &&
                           S.mean_mesh.n_nodes > 0 && pb.hmean && pb.hbins;
     if (hit_mean) {
-      int* hcount = pb.counters + 8;  // kHitBins counters (free in single-scene launches)
-      hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, hcount, kHitBins, 0, -1);
-      hipLaunchKernelGGL(k_hit_bin, dim3(grid_for(n)), dim3(256), 0, st, S.mean_mesh, pb.qM[cur], nM,
-                         pb.P.mhit, pb.P.mpx, pb.P.mpy, pb.P.mpz, pb.hbins, (int)pb.capacity, hcount);
-      hipLaunchKernelGGL(k_hit_mean_binned,
+      const size_t cap = hit_sort_stride(pb.capacity);
+      int* keys = pb.hbins;
+      int* ranks = keys + cap;
+      int* order = ranks + cap;
+      int* hist = order + cap;
+      int* start = hist + kHitKeys;
+      int* total = start + kHitKeys;
+      hipLaunchKernelGGL(k_hit_zero, dim3(kHitKeys / 256), dim3(256), 0, st, hist);
+      hipLaunchKernelGGL(k_hit_key, dim3(grid_for(n)), dim3(256), 0, st, S.mean_mesh, pb.qM[cur], nM,
+                         pb.P.mhit, pb.P.mpx, pb.P.mpy, pb.P.mpz, keys, ranks, hist);
+      hipLaunchKernelGGL(k_hit_scan, dim3(1), dim3(kHitScanThreads), 0, st, hist, start, total);
+      hipLaunchKernelGGL(k_hit_place, dim3(grid_for(n)), dim3(256), 0, st, pb.qM[cur], nM, keys,
+                         ranks, start, order);
+      hipLaunchKernelGGL(k_hit_mean_sorted,
                          dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
-                         dim3(256), 0, st, S.mean_mesh, pb.hbins, (int)pb.capacity, hcount,
-                         pb.P.mpx, pb.P.mpy, pb.P.mpz, pb.hmean);
+                         dim3(256), 0, st, S.mean_mesh, order, total, pb.P.mpx, pb.P.mpy, pb.P.mpz,
+                         pb.hmean);
       HIP_TRY(hipGetLastError());
     }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
