@@ -2790,65 +2790,47 @@ __global__ __launch_bounds__(256) void k_mesh_inside(DevMesh m, const float* px,
                                                      const float* pz, int64_t n, uint8_t* in) {
   MESH_QUERY_LOOP(i, n) in[i] = mesh_inside(m, mk(px[i], py[i], pz[i])) ? 1 : 0;
 }
-// The mesh mean and its gradient at every medium hit of a bounce iteration (the
-// paths of a march queue with march result 1), for the shading kernel: the
-// closest-point search at a hit point is a chain of dependent node loads, which
-// inside the shading kernel ran on its few hit lanes at that kernel's occupancy
-// and without the LDS copy of the tree's top.  Same function, same bits.
-// kMulti: a batched launch, each path's scene from its id (no LDS copy: the
-// paths' meshes differ); paths of scenes without a GP medium with a mesh mean
-// are passed over, as the shading kernel reads no mean for them.
-template <bool kMulti>
-__global__ __launch_bounds__(256) void k_hit_mean(DevScene S0, const DevScene* scenes,
-                                                  const uint16_t* sid, const int* qM,
-                                                  const int* nM, const uint8_t* mhit,
-                                                  const float* px, const float* py,
-                                                  const float* pz, float4* out) {
-  __shared__ BvhRec top[kMulti ? 1 : kTopRecs];
-  const int n = *nM;
-  if (n <= 0) return;  // the whole grid: no hit to serve
-  DevMesh m;
-  if constexpr (!kMulti) m = stage_top(S0.mean_mesh, top);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int pid = qM[i];
-    if (mhit[pid] != 1) continue;
-    F3 g;
-    float mu;
-    const F3 p = mk(px[pid], py[pid], pz[pid]);
-    if constexpr (kMulti) {
-      const DevScene& S = scenes[sid[pid]];
-      if (!(S.medium_kind == MUPSI_MEDIUM_GP && S.mean_type == MUPSI_MEAN_MESH &&
-            S.mean_mesh.n_nodes > 0))
-        continue;
-      mu = mesh_mean_eval<true, false>(S.mean_mesh, p, &g);
-    } else {
-      mu = mesh_mean_eval<true, true>(m, p, &g);
-    }
-    out[pid] = make_float4(mu, g.x, g.y, g.z);
-  }
-}
-
 // The hits of a march queue put in order of a key before their closest-point
-// searches: the distance class (the distance grids' upper bound at the hit
-// point relative to the mesh's box length mx: a search takes longer the farther
-// its point lies from the surface) and then the Morton index of the point's cell
-// in a 32^3 grid over the mesh's box padded by mx/4 (points outside are clamped
-// to its border cells).  A wave's searches then have similar lengths and start
+// searches: in batched launches the scene first, then the distance class (the
+// distance grids' upper bound at the hit point relative to the mesh's box
+// length mx: a search takes longer the farther its point lies from the
+// surface), then the Morton index of the point's cell in a grid of 2^cb cubes
+// per axis over the mesh's box padded by mx/4 (points outside are clamped to
+// its border cells).  A wave's searches then have similar lengths and start
 // near each other, so its lanes read the same nodes.  Counting sort: k_hit_key
 // counts each key (one atomic per key present in a wave, which also gives each
 // lane its rank among the key's hits), k_hit_scan turns the counts into starts,
 // k_hit_place writes each path id at start + rank.  The order within a key
 // varies from run to run; every hit's result is its own, so the bits do not.
 constexpr int kHitClasses = 4;
-constexpr int kHitCellBits = 5;  // 32 cells per axis
-constexpr int kHitKeys = kHitClasses << (3 * kHitCellBits);
-__device__ __forceinline__ uint32_t spread3_5(uint32_t v) {  // bits 0-4 to bits 0,3,..,12
-  v = (v | (v << 8)) & 0x0300Fu;
-  v = (v | (v << 4)) & 0x030C3u;
-  v = (v | (v << 2)) & 0x09249u;
+constexpr int kHitCellBits = 5;             // single-scene launches: 32^3 cells
+constexpr int kHitMaxKeys = 1 << 19;        // batched launches: cb shrinks to fit
+constexpr int kHitScanThreads = 1024;
+constexpr int kHitKeyQuantum = 4 * kHitScanThreads;  // key counts are padded to this
+// Cell bits for a launch over n_scenes scenes: the largest cb <= kHitCellBits
+// with n_scenes * kHitClasses * 8^cb <= kHitMaxKeys (cb 0 for 65,536 scenes).
+inline int hit_cell_bits(int n_scenes) {
+  int cb = kHitCellBits;
+  while (cb > 0 && (int64_t)n_scenes * kHitClasses * (1ll << (3 * cb)) > kHitMaxKeys) --cb;
+  return cb;
+}
+inline int hit_key_count(int n_scenes, int cb) {
+  const int64_t k = (int64_t)n_scenes * kHitClasses * (1ll << (3 * cb));
+  return (int)((k + kHitKeyQuantum - 1) / kHitKeyQuantum * kHitKeyQuantum);
+}
+// Layout of the sort's buffer for n paths: keys, ranks and order at a stride of
+// n rounded up to 64 ints (the counts stay 16-byte aligned for the scan's
+// vector loads), then counts, starts and the total.
+inline size_t hit_sort_stride(size_t n) { return (n + 63) & ~(size_t)63; }
+inline size_t hit_sort_ints(size_t n) { return 3 * hit_sort_stride(n) + 2 * (size_t)kHitMaxKeys + 64; }
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // bits 0-9 to bits 0,3,..,27
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
   return v;
 }
-__device__ __forceinline__ int hit_key(const DevMesh& m, F3 p) {
+__device__ __forceinline__ int hit_key(const DevMesh& m, F3 p, int cb) {
   const F3 pl = xf_to_local(m.xf, p);
   float lo, hi;
   mesh_dist_bounds_local(m, pl, lo, hi);
@@ -2861,33 +2843,49 @@ __device__ __forceinline__ int hit_key(const DevMesh& m, F3 p) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float lo_k = m.lbmin[k] - pad, ext = m.lbmax[k] - m.lbmin[k] + 2.0f * pad;
-    const float u = (q[k] - lo_k) / fmaxf(ext, 1e-30f) * (float)(1 << kHitCellBits);
+    const float u = (q[k] - lo_k) / fmaxf(ext, 1e-30f) * (float)(1 << cb);
     // NaN and far points land in range: fminf/fmaxf return the number operand
-    const int c = (int)fminf(fmaxf(u, 0.0f), (float)((1 << kHitCellBits) - 1));
-    morton |= spread3_5((uint32_t)c) << k;
+    const int c = (int)fminf(fmaxf(u, 0.0f), (float)((1 << cb) - 1));
+    morton |= spread3((uint32_t)c) << k;
   }
-  return (b << (3 * kHitCellBits)) | (int)morton;
+  return (b << (3 * cb)) | (int)morton;
 }
-// Layout of the sort's buffer for n paths: keys, ranks and order at a stride of
-// n rounded up to 64 ints (the counts stay 16-byte aligned for the scan's
-// vector loads), then counts, starts and the total.
-inline size_t hit_sort_stride(size_t n) { return (n + 63) & ~(size_t)63; }
-inline size_t hit_sort_ints(size_t n) { return 3 * hit_sort_stride(n) + 2 * kHitKeys + 64; }
-__global__ __launch_bounds__(256) void k_hit_zero(int* hist) {
+__global__ __launch_bounds__(256) void k_hit_zero(int* hist, int nkeys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kHitKeys) hist[i] = 0;
+  if (i < nkeys) hist[i] = 0;
 }
-__global__ __launch_bounds__(256) void k_hit_key(DevMesh m, const int* qM, const int* nM,
+// Entry i of the concatenation of queues a (na entries) and b (nb).
+__device__ __forceinline__ int hit_queue_at(const int* qa, int na, const int* qb, int i) {
+  return i < na ? qa[i] : qb[i - na];
+}
+// kMulti: a batched launch, each path's mesh from its scene; paths of scenes
+// without a GP medium with a mesh mean get no key (the shading kernel reads no
+// mean for them).
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_hit_key(DevMesh m0, const DevScene* scenes,
+                                                 const uint16_t* sid, const int* qa,
+                                                 const int* na, const int* qb, const int* nb,
                                                  const uint8_t* mhit, const float* px,
-                                                 const float* py, const float* pz, int* keys,
-                                                 int* ranks, int* hist) {
-  const int n = *nM;
+                                                 const float* py, const float* pz, int cb,
+                                                 int* keys, int* ranks, int* hist) {
+  const int n0 = *na, n = n0 + (nb ? *nb : 0);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (__ballot(i < n) == 0) return;  // whole waves past the queue
   int key = -1;
   if (i < n) {
-    const int pid = qM[i];
-    if (mhit[pid] == 1) key = hit_key(m, mk(px[pid], py[pid], pz[pid]));
+    const int pid = hit_queue_at(qa, n0, qb, i);
+    if (mhit[pid] == 1) {
+      const F3 p = mk(px[pid], py[pid], pz[pid]);
+      if constexpr (kMulti) {
+        const int s = sid[pid];
+        const DevScene& S = scenes[s];
+        if (S.medium_kind == MUPSI_MEDIUM_GP && S.mean_type == MUPSI_MEAN_MESH &&
+            S.mean_mesh.n_nodes > 0)
+          key = s * (kHitClasses << (3 * cb)) + hit_key(S.mean_mesh, p, cb);
+      } else {
+        key = hit_key(m0, p, cb);
+      }
+    }
   }
   // one atomic per distinct key of the wave: the leader of each key adds the
   // key's lane count and hands the old value to its lanes
@@ -2909,18 +2907,16 @@ __global__ __launch_bounds__(256) void k_hit_key(DevMesh m, const int* qM, const
     ranks[i] = rank;
   }
 }
-// One block: hist (kHitKeys counts) -> start (exclusive prefix sums); the total
-// goes to *total.
-constexpr int kHitScanThreads = 1024;
+// One block: hist (nkeys counts, a multiple of kHitKeyQuantum) -> start
+// (exclusive prefix sums); the total goes to *total.
 __global__ __launch_bounds__(kHitScanThreads) void k_hit_scan(const int* hist, int* start,
-                                                              int* total) {
-  constexpr int kPer = kHitKeys / kHitScanThreads;
-  static_assert(kPer % 4 == 0 && kHitKeys % kHitScanThreads == 0, "scan layout");
+                                                              int* total, int nkeys) {
   __shared__ int part[kHitScanThreads];
   const int t = threadIdx.x;
-  const int4* h4 = reinterpret_cast<const int4*>(hist + t * kPer);
+  const int per4 = nkeys / kHitKeyQuantum;  // int4 loads per thread
+  const int4* h4 = reinterpret_cast<const int4*>(hist) + (size_t)t * per4;
   int sum = 0;
-  for (int k = 0; k < kPer / 4; ++k) {
+  for (int k = 0; k < per4; ++k) {
     const int4 v = h4[k];
     sum += v.x + v.y + v.z + v.w;
   }
@@ -2933,8 +2929,8 @@ __global__ __launch_bounds__(kHitScanThreads) void k_hit_scan(const int* hist, i
     __syncthreads();
   }
   int run = part[t] - sum;
-  int4* s4 = reinterpret_cast<int4*>(start + t * kPer);
-  for (int k = 0; k < kPer / 4; ++k) {
+  int4* s4 = reinterpret_cast<int4*>(start) + (size_t)t * per4;
+  for (int k = 0; k < per4; ++k) {
     const int4 v = h4[k];
     int4 o;
     o.x = run;
@@ -2946,30 +2942,75 @@ __global__ __launch_bounds__(kHitScanThreads) void k_hit_scan(const int* hist, i
   }
   if (t == kHitScanThreads - 1) *total = part[t];
 }
-__global__ __launch_bounds__(256) void k_hit_place(const int* qM, const int* nM, const int* keys,
+__global__ __launch_bounds__(256) void k_hit_place(const int* qa, const int* na, const int* qb,
+                                                   const int* nb, const int* keys,
                                                    const int* ranks, const int* start,
                                                    int* order) {
-  const int n = *nM;
+  const int n0 = *na, n = n0 + (nb ? *nb : 0);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int key = keys[i];
-  if (key >= 0) order[start[key] + ranks[i]] = qM[i];
+  if (key >= 0) order[start[key] + ranks[i]] = hit_queue_at(qa, n0, qb, i);
 }
-// k_hit_mean over the hits in key order, with the top of the tree in LDS.
-__global__ __launch_bounds__(256) void k_hit_mean_sorted(DevMesh m0, const int* order,
+// The mesh mean and its gradient at every medium hit of a bounce iteration (the
+// march queues' paths with a hit), in key order, for the shading kernel: the
+// closest-point search at a hit point is a chain of dependent node loads, which
+// inside the shading kernel ran on its few hit lanes at that kernel's occupancy
+// and without the LDS copy of the tree's top.  Same function, same bits.
+// Single-scene launches copy the tree's top to LDS; batched ones take each
+// path's mesh from its scene (no LDS copy: the paths' meshes differ).
+template <bool kMulti>
+__global__ __launch_bounds__(256) void k_hit_mean_sorted(DevMesh m0, const DevScene* scenes,
+                                                         const uint16_t* sid, const int* order,
                                                          const int* total, const float* px,
                                                          const float* py, const float* pz,
                                                          float4* out) {
-  __shared__ BvhRec top[kTopRecs];
+  __shared__ BvhRec top[kMulti ? 1 : kTopRecs];
   const int n = *total;
   if (n <= 0) return;  // the whole grid: no hit to serve
-  const DevMesh m = stage_top(m0, top);
+  DevMesh m;
+  if constexpr (!kMulti) m = stage_top(m0, top);
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const int pid = order[j];
     F3 g;
-    const float mu = mesh_mean_eval<true, true>(m, mk(px[pid], py[pid], pz[pid]), &g);
+    const F3 p = mk(px[pid], py[pid], pz[pid]);
+    float mu;
+    if constexpr (kMulti) mu = mesh_mean_eval<true, false>(scenes[sid[pid]].mean_mesh, p, &g);
+    else mu = mesh_mean_eval<true, true>(m, p, &g);
     out[pid] = make_float4(mu, g.x, g.y, g.z);
   }
+}
+// The sort and the searches for the hits of queues a and b (b may be null).
+inline void launch_hit_means(hipStream_t st, int n, int query_blocks, bool multi, const DevMesh& m,
+                             const DevScene* scenes, const uint16_t* sid, int n_scenes,
+                             const int* qa, const int* na, const int* qb, const int* nb,
+                             const PathSoA& P, int* buf, size_t capacity, float4* out) {
+  const size_t cap = hit_sort_stride(capacity);
+  const int cb = multi ? hit_cell_bits(n_scenes) : kHitCellBits;
+  const int nkeys = hit_key_count(multi ? n_scenes : 1, cb);
+  int* keys = buf;
+  int* ranks = keys + cap;
+  int* order = ranks + cap;
+  int* hist = order + cap;
+  int* start = hist + kHitMaxKeys;
+  int* total = start + kHitMaxKeys;
+  const dim3 g(grid_for(n));
+  hipLaunchKernelGGL(k_hit_zero, dim3(nkeys / 256), dim3(256), 0, st, hist, nkeys);
+  if (multi)
+    hipLaunchKernelGGL(k_hit_key<true>, g, dim3(256), 0, st, DevMesh{}, scenes, sid, qa, na, qb,
+                       nb, P.mhit, P.mpx, P.mpy, P.mpz, cb, keys, ranks, hist);
+  else
+    hipLaunchKernelGGL(k_hit_key<false>, g, dim3(256), 0, st, m, nullptr, nullptr, qa, na, qb, nb,
+                       P.mhit, P.mpx, P.mpy, P.mpz, cb, keys, ranks, hist);
+  hipLaunchKernelGGL(k_hit_scan, dim3(1), dim3(kHitScanThreads), 0, st, hist, start, total, nkeys);
+  hipLaunchKernelGGL(k_hit_place, g, dim3(256), 0, st, qa, na, qb, nb, keys, ranks, start, order);
+  const dim3 hg(std::min<unsigned>(g.x, (unsigned)std::max(1, query_blocks)));
+  if (multi)
+    hipLaunchKernelGGL(k_hit_mean_sorted<true>, hg, dim3(256), 0, st, DevMesh{}, scenes, sid,
+                       order, total, P.mpx, P.mpy, P.mpz, out);
+  else
+    hipLaunchKernelGGL(k_hit_mean_sorted<false>, hg, dim3(256), 0, st, m, nullptr, nullptr, order,
+                       total, P.mpx, P.mpy, P.mpz, out);
 }
 
 // Traversal cost per point of Mesh::distToPoint (kind 0), Mesh::inside (kind 1), or
@@ -3547,7 +3588,7 @@ static mupsi_status alloc_path_buffers(std::unique_ptr<PathBuffers>& out, int64_
   HIP_TRY(m.alloc(8, &pb->stats));
   if (hit_means) {
     HIP_TRY(m.alloc(n, &pb->hmean));
-    if (!mesh_queue) HIP_TRY(m.alloc(hit_sort_ints(n), &pb->hbins));
+    HIP_TRY(m.alloc(hit_sort_ints(n), &pb->hbins));
   }
   pb->capacity = n;
   out = std::move(pb);
@@ -3809,23 +3850,8 @@ static mupsi_status enqueue_wavefront(mupsi_scene* sc, const PathSrc& src, int64
                           !S.skip_medium && S.mean_type == MUPSI_MEAN_MESH &&
                           S.mean_mesh.n_nodes > 0 && pb.hmean && pb.hbins;
     if (hit_mean) {
-      const size_t cap = hit_sort_stride(pb.capacity);
-      int* keys = pb.hbins;
-      int* ranks = keys + cap;
-      int* order = ranks + cap;
-      int* hist = order + cap;
-      int* start = hist + kHitKeys;
-      int* total = start + kHitKeys;
-      hipLaunchKernelGGL(k_hit_zero, dim3(kHitKeys / 256), dim3(256), 0, st, hist);
-      hipLaunchKernelGGL(k_hit_key, dim3(grid_for(n)), dim3(256), 0, st, S.mean_mesh, pb.qM[cur], nM,
-                         pb.P.mhit, pb.P.mpx, pb.P.mpy, pb.P.mpz, keys, ranks, hist);
-      hipLaunchKernelGGL(k_hit_scan, dim3(1), dim3(kHitScanThreads), 0, st, hist, start, total);
-      hipLaunchKernelGGL(k_hit_place, dim3(grid_for(n)), dim3(256), 0, st, pb.qM[cur], nM, keys,
-                         ranks, start, order);
-      hipLaunchKernelGGL(k_hit_mean_sorted,
-                         dim3(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks))),
-                         dim3(256), 0, st, S.mean_mesh, order, total, pb.P.mpx, pb.P.mpy, pb.P.mpz,
-                         pb.hmean);
+      launch_hit_means(st, (int)n, ctx->query_blocks, false, S.mean_mesh, nullptr, nullptr, 1,
+                       pb.qM[cur], nM, nullptr, nullptr, pb.P, pb.hbins, pb.capacity, pb.hmean);
       HIP_TRY(hipGetLastError());
     }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], nM2, gs,
@@ -4216,17 +4242,17 @@ static void enqueue_batch_launch(mupsi_batch* b, hipStream_t st) {
     GuideShade gs{0, 1.0f, nullptr, 0, 1, nullptr};
     // the mesh-mean hits of this iteration's marches (window marcher of mesh means,
     // lane-per-ray marcher): their mean and gradient in a kernel of their own
-    const bool hit_mean = it > 0 && b->any_mesh_mean && pb.hmean;
+    const bool hit_mean = it > 0 && b->any_mesh_mean && pb.hmean && pb.hbins &&
+                          (b->any_cached_mesh || b->any_plain);
     if (hit_mean) {
-      const dim3 hgrid(std::min<unsigned>(grid_for(n), (unsigned)std::max(1, ctx->query_blocks)));
-      if (b->any_cached_mesh)
-        hipLaunchKernelGGL(k_hit_mean<true>, hgrid, dim3(256), 0, st, DevScene{}, b->d_scenes,
-                           pb.P.sid, pb.qMs[cur], pb.counters + 12 + cur, pb.P.mhit, pb.P.mpx,
-                           pb.P.mpy, pb.P.mpz, pb.hmean);
-      if (b->any_plain)
-        hipLaunchKernelGGL(k_hit_mean<true>, hgrid, dim3(256), 0, st, DevScene{}, b->d_scenes,
-                           pb.P.sid, pb.qMm[cur], pb.counters + 8 + cur, pb.P.mhit, pb.P.mpx,
-                           pb.P.mpy, pb.P.mpz, pb.hmean);
+      // both hit queues in one sort: window-marcher (Ms) then lane-per-ray (Mm) paths
+      const int* qa = b->any_cached_mesh ? pb.qMs[cur] : pb.qMm[cur];
+      const int* na = b->any_cached_mesh ? pb.counters + 12 + cur : pb.counters + 8 + cur;
+      const bool both = b->any_cached_mesh && b->any_plain;
+      launch_hit_means(st, (int)n, ctx->query_blocks, true, DevMesh{}, b->d_scenes, pb.P.sid,
+                       (int)b->scenes.size(), qa, na, both ? pb.qMm[cur] : nullptr,
+                       both ? pb.counters + 8 + cur : nullptr, pb.P, pb.hbins, pb.capacity,
+                       pb.hmean);
     }
     ShadeArgs sa{pb.P, src, pb.qA[cur], nA, pb.qA[nxt], nA2, pb.qM[nxt], pb.counters + 2 * nxt + 1,
                  gs, 0, ShadowSoA{}, nullptr, nullptr, b->d_scenes,
