@@ -40,7 +40,7 @@ FP32_PEAK_TFLOPS = 157.3
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over `bench.py --steps 1 --warmup 0 --copy-probe-mb 256`,
 # FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (what the counters read for the 256 MiB copy);
 # profiles/k_march_sphere_sphere_renewal_256x256x64.md holds the table.
-MARCH_TRAFFIC_BYTES = 135_100_000
+MARCH_TRAFFIC_BYTES = 112_700_000
 PRESET = "sphere_renewal_256"
 WORKLOAD = ("sphere-mean GPIS, renewal_plus, 256x256, 64 spp, max_bounce 10, "
             "kernel sigma 1.5 / length_scale 2 / aniso [1,1,4] / ppc 3, "
