@@ -2986,7 +2986,9 @@ inline void launch_hit_means(hipStream_t st, int n, int query_blocks, bool multi
                              const int* qa, const int* na, const int* qb, const int* nb,
                              const PathSoA& P, int* buf, size_t capacity, float4* out) {
   const size_t cap = hit_sort_stride(capacity);
-  const int cb = multi ? hit_cell_bits(n_scenes) : kHitCellBits;
+  // a single mesh of fewer than 65,536 triangles: distance classes only (its
+  // tree is small enough that the cells gained less than the larger scan cost)
+  const int cb = multi ? hit_cell_bits(n_scenes) : (m.n_faces >= 65536 ? kHitCellBits : 0);
   const int nkeys = hit_key_count(multi ? n_scenes : 1, cb);
   int* keys = buf;
   int* ranks = keys + cap;
